@@ -1,0 +1,135 @@
+#!/usr/bin/env python3
+"""Benchmark: Newton steps/s of the cloth Newton step on one MI355X.
+
+Workload: the 297 x 297-ring cloth tube (88,209 vertices, 175,824 triangles) around a spinning
+kinematic cylinder (5,760 vertices), dt = 1/60 s, barrier contact with kappa = 1e9, FP32.
+A bench "step" is one simulated frame (each frame runs >= 3 Newton iterations); `value` is
+Newton iterations per second over the timed frames, with all inputs resident on the GPU.
+
+The JSON line carries:
+  roofline      the PCG block-SpMV (dominant kernel): algorithmic bytes per launch
+                40 B per off-diagonal block (36 B values + 4 B column) + 88 B per row
+                (36 B diagonal block, z and p_old of the row 24 B, p_new and q writes 24 B,
+                4 B slice offset amortised) divided by the launch's average duration measured
+                with hipEvents on the solver stream; peak = 8 TB/s HBM3E.
+  cpu_baseline  the CPU oracle (C++ restatement of the reference path, 1 thread) timed on one
+                Newton iteration of the same workload on this host.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+BASELINE_METRIC = "Newton steps/s @88K-vert cloth, mixed vs FP32; SpMV HBM GB/s; scene-steps/s"
+HBM_PEAK_GBS = 8000.0
+
+
+def spmv_bytes(n_rows: int, n_blocks: int) -> int:
+    return 40 * n_blocks + 88 * n_rows
+
+
+def traffic_from_profiles():
+    """HBM bytes per SpMV launch from the committed rocprofv3 counter summary, if present."""
+    path = os.path.join(ROOT, "profiles", "spmv_tube88k_counters.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        with open(path) as f:
+            return json.load(f).get("hbm_bytes_per_launch")
+    except Exception:
+        return None
+
+
+def cpu_baseline(scene):
+    import oracle
+
+    o = oracle.Oracle(scene, 32)
+    o.stage("frame_begin")
+    o.stage("predict")
+    t0 = time.perf_counter()
+    o.stage("newton_iteration")
+    dt = time.perf_counter() - t0
+    return {"value": 1.0 / dt, "unit": "Newton steps/s", "cores": 1, "kind": "port",
+            "sample": f"first Newton iteration of frame 0 on the same {scene.num_verts}-vertex scene "
+                      f"(C++ oracle, FP32, single thread, {dt:.1f} s)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--rings", type=int, default=297)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the workload is a single scene on one MI355X and this build "
+              "has no multi-GPU path (see DESIGN.md for how it would shard). Run with --gpus 1.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from luisacomputesimulator_amd import NewtonSolver, scenes
+
+    sc = scenes.cylinder_tube(rings=args.rings, per_ring=args.rings)
+    g = NewtonSolver.from_scene(sc)
+    g.set_timing(2)
+    n_rows = g.num_verts
+    n_blocks = len(g.read("csr")) - (n_rows + 1)
+    for _ in range(args.warmup):
+        g.physics_step_gpu(1)
+    g.kernel_times(reset=True)
+    torch.cuda.synchronize()
+    newton = pcg = 0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        g.physics_step_gpu(1)
+        st = g.stats()
+        newton += st["newton_iters"]
+        pcg += st["pcg_iters_total"]
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    kt = g.kernel_times()
+    spmv_ms, spmv_n = kt.get("spmv", (float("nan"), 0))
+    nbytes = spmv_bytes(n_rows, n_blocks)
+    achieved = nbytes / (spmv_ms * 1e-3) / 1e9 if spmv_n else None
+    st = g.stats()
+    line = {
+        "metric": BASELINE_METRIC,
+        "value": newton / elapsed,
+        "unit": "Newton steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"cloth tube {args.rings}x{args.rings} ({sc.meshes[0].verts.shape[0]} verts, "
+                               f"{sc.meshes[0].tris.shape[0]} tris) around a spinning kinematic cylinder, "
+                               "dt 1/60 s, kappa 1e9, FP32",
+                   "verts_total": n_rows, "material_blocks": n_blocks, "frames": args.steps,
+                   "newton_iters": newton, "pcg_iters": pcg,
+                   "frames_per_s": args.steps / elapsed,
+                   "contacts_last_iter": st["num_contacts"],
+                   "pairs_last_iter": st["num_pairs_vf"] + st["num_pairs_ee"]},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": traffic_from_profiles(),
+                     "kernel": "k_spmv_pcg", "bytes_per_launch": nbytes, "avg_launch_ms": spmv_ms,
+                     "launches": spmv_n},
+        "kernel_ms": {k: round(v[0], 4) for k, v in kt.items()},
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(sc)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

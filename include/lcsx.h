@@ -1,0 +1,213 @@
+/*
+ * lcsx: MI355X-native Newton step for IPC cloth simulation (C ABI).
+ *
+ * This is the drop-in boundary for the reference's per-Newton-step subsystems. The reference
+ * exposes C++ classes, not a C ABI; every entry point below names the reference interface it
+ * replaces (paths relative to the reference repository root):
+ *
+ *   NewtonSolver / SolverInterface   Solver/SimulationSolver/newton_solver.h:29-51,
+ *                                    Solver/SimulationSolver/solver_interface.h:71-108
+ *   Python binding (lcs_solver_py)   PythonBindings/python_bindings.cpp:303-359
+ *   LBVH                             Solver/CollisionDetector/lbvh.h:183-223
+ *   NarrowPhasesDetector             Solver/CollisionDetector/narrow_phase.h:54-201
+ *   ConjugateGradientSolver          Solver/LinearSolver/precond_cg.h:38-47
+ *
+ * Conventions: plain pointers and sizes, no C++ or torch types. Host arrays unless the name
+ * says "device". Positions are packed float triples (12 bytes per vertex). Every call returns
+ * an lcsx_status; lcsx_last_error() gives a message for the last failure on the calling thread.
+ * Errors are reported, never aborted on: the reference's LUISA_ERROR paths become
+ * LCSX_ERR_NAN / LCSX_ERR_STATE, and its resize-and-retry on buffer overflow is done inside
+ * the library (LCSX_ERR_CAPACITY only if growth is impossible).
+ */
+#ifndef LCSX_H
+#define LCSX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum lcsx_status
+{
+    LCSX_OK           = 0,
+    LCSX_ERR_ARG      = 1, /* invalid argument or size                                    */
+    LCSX_ERR_STATE    = 2, /* call out of order (e.g. step before init)                    */
+    LCSX_ERR_HIP      = 3, /* a HIP runtime call failed                                    */
+    LCSX_ERR_CAPACITY = 4, /* a contact buffer could not be grown                          */
+    LCSX_ERR_NAN      = 5, /* NaN/Inf in the Newton system or the step (reference: LUISA_ERROR) */
+    LCSX_ERR_IO       = 6, /* file could not be read or parsed                             */
+    LCSX_ERR_NO_GPU   = 7  /* no HIP device available                                      */
+} lcsx_status;
+
+typedef struct lcsx_solver lcsx_solver;
+
+/* ClothMaterial, Solver/SimulationCore/physical_material.h:45-63 (same field meaning). */
+typedef struct lcsx_material
+{
+    float thickness;              /* m; contact offset per vertex is thickness / 2          */
+    float youngs_modulus;         /* Pa                                                     */
+    float poisson_ratio;
+    float area_bending_stiffness;
+    float density;                /* kg/m^3; mass = area * thickness * density when mass==0 */
+    float mass;                   /* total mesh mass, 0 = from density                      */
+    float d_hat;                  /* barrier activation distance, m                         */
+} lcsx_material;
+
+/* FixedPointAnimationInfo, Solver/Initializer/init_mesh_data.h:39-93. */
+typedef struct lcsx_fixed_anim
+{
+    int   use_translate;
+    float translate[3];
+    int   use_scale;
+    float scale[3];
+    int   use_rotate;
+    float rot_center[3];
+    float rot_axis[3];
+    float rot_deg_per_s;
+} lcsx_fixed_anim;
+
+/* SceneParams fields that the Newton step reads (Solver/SimulationCore/scene_params.h:15-182). */
+typedef struct lcsx_params
+{
+    float implicit_dt;          /* s                                                         */
+    int   nonlinear_iter_count; /* minimum Newton iterations per frame (reference default 3) */
+    int   newton_max_iters;     /* hard cap (reference: 100)                                 */
+    int   pcg_iter_count;       /* PCG iteration cap (reference default 100)                 */
+    int   pcg_check_every;      /* PCG convergence test interval (reference: 25)             */
+    float pcg_rel_tol;          /* stop when r.z <= tol^2 * r0.z0                            */
+    float pcg_abs_tol;          /* ... or r.z < abs_tol (reference: 1e-10)                   */
+    float gravity[3];
+    int   use_floor;
+    int   use_self_collision;
+    int   use_ccd_linesearch;
+    float floor_y;
+    float stiffness_collision;  /* kappa                                                     */
+    float stiffness_dirichlet;  /* pinned-vertex inertia multiplier (reference: 1e9)         */
+    float damping_cloth;        /* velocity damping, v *= exp(-damping * dt)                 */
+    float bending_scaling;      /* stiffness_bending_ui                                      */
+    float newton_tol;           /* converged when max|dx| < newton_tol * dt                  */
+} lcsx_params;
+
+/* Scalars of the last Newton iteration / frame. */
+typedef struct lcsx_stats
+{
+    int   frame;
+    int   newton_iters;       /* iterations of the last frame                               */
+    int   pcg_iters;          /* PCG iterations of the last Newton iteration                */
+    int   pcg_iters_total;    /* summed over the last frame                                 */
+    float toi;                /* CCD step bound (alpha) of the last iteration, in (0, 1]    */
+    float max_dx;             /* max vertex |dx| of the last iteration                      */
+    float pin_delta;          /* max |x - x_tilde| over pinned vertices                     */
+    float pcg_rel;            /* sqrt(r.z / r0.z0) at PCG exit                              */
+    int   num_pairs_vf;       /* broad-phase candidates (DCD query of the last iteration)   */
+    int   num_pairs_ee;
+    int   num_contacts;       /* active barrier contacts                                    */
+    int   converged;
+    int   broadphase_overflow;/* a 32-entry traversal stack overflowed (reference: silent)  */
+} lcsx_stats;
+
+/* ---------------------------------------------------------------- library */
+const char* lcsx_last_error(void);
+const char* lcsx_version(void);
+/* Number of HIP devices visible (0 on a host without a GPU). */
+int lcsx_device_count(void);
+
+/* ------------------------------------------------------- NewtonSolver API */
+/* Replaces NewtonSolver construction + lcs::get_scene_params() setup. params may be NULL
+ * (reference defaults). device selects the HIP device (ordinal). */
+lcsx_status lcsx_solver_create(const lcsx_params* params, int device, lcsx_solver** out);
+lcsx_status lcsx_solver_destroy(lcsx_solver* s);
+/* Default parameters (scene_params.h defaults, with pcg_rel_tol 1e-4 and check every iteration). */
+void lcsx_params_default(lcsx_params* p);
+void lcsx_material_default(lcsx_material* m);
+
+/* NewtonSolver::register_mesh (python_bindings.cpp:259) + WorldData material: world-space
+ * vertices (n_verts x 3 floats) and triangles (n_tris x 3 int32). mesh_id receives the index. */
+lcsx_status lcsx_solver_add_mesh(lcsx_solver* s, const float* verts, int n_verts, const int32_t* tris, int n_tris,
+                                 const lcsx_material* material, int* mesh_id);
+/* WorldData::load_fixed_points (init_mesh_data.cpp:117-240): method is a FixedPointsType value
+ * (3 Left, 4 Right, 5 Front, 6 Back, 7 Up, 8 Down, 9..20 corner combinations, 21 All),
+ * range in normalised mesh coordinates. anim may be NULL (static pins). */
+lcsx_status lcsx_solver_add_fixed_points(lcsx_solver* s, int mesh_id, int method, float range,
+                                         const lcsx_fixed_anim* anim, int* num_selected);
+/* Demo::Simulation::load_scene_params_from_json (app_simulation_demo_config.cpp:49-766):
+ * scene keys, shells with model_name (OBJ, resolved against mesh_dir), translation, rotation,
+ * scale, Cloth material and fixed_points. */
+lcsx_status lcsx_solver_load_scene_json(lcsx_solver* s, const char* json_path, const char* mesh_dir);
+/* NewtonSolver::init_solver (newton_solver.h:31-51): topology, masses, CSR, device buffers. */
+lcsx_status lcsx_solver_init(lcsx_solver* s);
+/* physics_step_GPU (newton_solver.cpp:4051-4345) for `frames` frames, each preceded by the
+ * pinned-target update of app_simulation.cpp:114-123 at time frame * dt. */
+lcsx_status lcsx_solver_step(lcsx_solver* s, int frames);
+/* NewtonSolver::get_simulation_results / SolverInterface::get_simulation_results_to_host. */
+lcsx_status lcsx_solver_get_positions(lcsx_solver* s, float* out, size_t n_floats);
+lcsx_status lcsx_solver_get_velocities(lcsx_solver* s, float* out, size_t n_floats);
+/* SolverInterface state restore (load_state positions/velocities, solver_interface.cpp:291-412). */
+lcsx_status lcsx_solver_set_state(lcsx_solver* s, const float* x, const float* v, size_t n_floats);
+/* SolverInterface::update_pinned_verts_information (solver_interface.h:95-108): explicit targets
+ * for the pinned vertices of one mesh (in selection order), replacing the animation. */
+lcsx_status lcsx_solver_set_pinned_targets(lcsx_solver* s, int mesh_id, const float* targets, int n_pinned);
+lcsx_status lcsx_solver_get_stats(lcsx_solver* s, lcsx_stats* out);
+int         lcsx_solver_num_verts(lcsx_solver* s);
+
+/* ----------------------------------------------- per-stage entry points
+ * The reference's subsystem methods, applied to the solver's own device buffers. Used for
+ * parity checks and per-kernel measurement. stage is one of:
+ *   "frame_begin"      pinned targets + LBVH::reduce_*_tree_aabb + LBVH::construct_tree (x2)
+ *   "predict"          fn_predict_position (newton_solver.cpp:577-608)
+ *   "iter_begin"       x_iter_start = x
+ *   "material"         inertia, ground, stretch, bending evaluation + assembly (newton_solver.cpp:705-1172)
+ *   "broad_phase_dcd"  LBVH::update_*_leave_aabb + refit + broad_phase_query_from_{verts,edges} on (x, x)
+ *   "broad_phase_ccd"  the same on the swept boxes (x_iter_start, x)
+ *   "dcd"              NarrowPhasesDetector::vf/ee_dcd_query_repulsion
+ *   "update_contacts"  broad_phase_dcd + dcd
+ *   "contact_assemble" device_perPair_evaluate_gradient_hessian (narrow_phase.cpp:2550-2676)
+ *   "pcg"              ConjugateGradientSolver::device_solve (precond_cg.cpp:626-771)
+ *   "ccd"              ground + VF + EE CCD on (x_iter_start, x) -> toi
+ *   "line_search"      NewtonSolver::line_search (newton_solver.cpp:3685-3884), CCD variant
+ *   "newton_iteration" one full iteration of the Newton loop
+ *   "frame_end"        fn_update_velocity + frame counter
+ *   "step"             one full frame
+ *   "spmv_dx"          dx <- A dx (the system matrix applied to the dx buffer)
+ */
+lcsx_status lcsx_solver_run_stage(lcsx_solver* s, const char* stage);
+/* Named solver buffers, converted to a canonical host layout (see DESIGN.md for the list):
+ * float arrays are FP32, index arrays uint32, morton keys uint64. Off-diagonal blocks are
+ * returned in the reference's CSR slot order (row-major 3x3 per block). count_out receives the
+ * element count; with out == NULL only the count is reported. */
+lcsx_status lcsx_solver_read(lcsx_solver* s, const char* name, void* out, size_t bytes, size_t* count_out);
+lcsx_status lcsx_solver_write(lcsx_solver* s, const char* name, const void* in, size_t bytes);
+/* Average device time (ms) of each kernel group over the calls since the last reset
+ * (hipEvent timing on the solver's stream); names are written as "name=ms;..." into buf. */
+lcsx_status lcsx_solver_kernel_times(lcsx_solver* s, char* buf, size_t buf_len, int reset);
+/* Timing level: 0 off, 1 stage groups (default), 2 also one event pair around every PCG SpMV
+ * launch (reported as "spmv"; only launches before convergence are counted). */
+lcsx_status lcsx_solver_set_timing(lcsx_solver* s, int level);
+
+/* ----------------------------------------------- standalone LBVH (device pointers)
+ * LBVH over faces (arity 3) or edges (arity 2), lbvh.h:166-304. All pointers are device
+ * pointers on the solver-independent stream `stream` (a hipStream_t, NULL = default). */
+typedef struct lcsx_lbvh lcsx_lbvh;
+lcsx_status lcsx_lbvh_create(int arity, const uint32_t* device_prims, int n_leaves, lcsx_lbvh** out);
+lcsx_status lcsx_lbvh_destroy(lcsx_lbvh* t);
+/* reduce_*_tree_aabb + construct_tree: n_verts_for_reduce reproduces the reference's
+ * second-pass sizing (ceil(n/256) blocks of leaves enter the global box). */
+lcsx_status lcsx_lbvh_construct(lcsx_lbvh* t, const float* device_x, int n_verts_for_reduce, void* stream);
+/* update_*_leave_aabb + refit: leaf box of the element over x0 and x1, padded by pad[first vertex]. */
+lcsx_status lcsx_lbvh_refit(lcsx_lbvh* t, const float* device_x0, const float* device_x1, const float* device_pad,
+                            void* stream);
+/* broad_phase_query_*: boxes are 6 floats (lo, hi) per query; edge_filter keeps query < leaf.
+ * Pairs (query, leaf) are appended to device_pairs (2 uint32 each) up to capacity; the total
+ * found is written to *device_count (uint32, device memory, must be zeroed by the caller). */
+lcsx_status lcsx_lbvh_query(lcsx_lbvh* t, const float* device_boxes, int n_queries, int edge_filter,
+                            uint32_t* device_pairs, uint32_t capacity, uint32_t* device_count, void* stream);
+/* Reads back tree arrays: "morton_sorted" (uint64), "sorted_to_orig", "parent", "children",
+ * "object_idx" (uint32), "node_aabb" (6 floats per node), "global_aabb" (6 floats). */
+lcsx_status lcsx_lbvh_read(lcsx_lbvh* t, const char* name, void* out, size_t bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LCSX_H */

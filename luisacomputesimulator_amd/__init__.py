@@ -1,0 +1,302 @@
+"""MI355X-native Newton step for IPC cloth simulation (LuisaComputeSimulator's hot path).
+
+Python front end over the C ABI in ``include/lcsx.h`` (``liblcsx.so``, HIP for gfx950). The
+class surface follows the reference's pybind module ``lcs_solver_py``
+(``PythonBindings/python_bindings.cpp:303-359``): ``NewtonSolver.register_mesh``,
+``init_solver``, ``physics_step_gpu`` and ``get_simulation_results``, plus the per-stage entry
+points of the reference's NewtonSolver / LBVH / NarrowPhasesDetector / ConjugateGradientSolver
+classes for parity checks and measurement.
+
+There is no CPU fallback: if ``liblcsx.so`` is missing or no GPU is present, calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Dict, List, Optional
+
+import numpy as np
+
+from . import scenes  # noqa: F401
+from .scenes import FIXED_METHODS, Anim, Material, Params, Scene
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liblcsx.so")
+_LIB: Optional[C.CDLL] = None
+
+
+class LcsxError(RuntimeError):
+    pass
+
+
+class lcsx_material(C.Structure):
+    _fields_ = [("thickness", C.c_float), ("youngs_modulus", C.c_float), ("poisson_ratio", C.c_float),
+                ("area_bending_stiffness", C.c_float), ("density", C.c_float), ("mass", C.c_float),
+                ("d_hat", C.c_float)]
+
+
+class lcsx_fixed_anim(C.Structure):
+    _fields_ = [("use_translate", C.c_int), ("translate", C.c_float * 3), ("use_scale", C.c_int),
+                ("scale", C.c_float * 3), ("use_rotate", C.c_int), ("rot_center", C.c_float * 3),
+                ("rot_axis", C.c_float * 3), ("rot_deg_per_s", C.c_float)]
+
+
+class lcsx_params(C.Structure):
+    _fields_ = [("implicit_dt", C.c_float), ("nonlinear_iter_count", C.c_int), ("newton_max_iters", C.c_int),
+                ("pcg_iter_count", C.c_int), ("pcg_check_every", C.c_int), ("pcg_rel_tol", C.c_float),
+                ("pcg_abs_tol", C.c_float), ("gravity", C.c_float * 3), ("use_floor", C.c_int),
+                ("use_self_collision", C.c_int), ("use_ccd_linesearch", C.c_int), ("floor_y", C.c_float),
+                ("stiffness_collision", C.c_float), ("stiffness_dirichlet", C.c_float), ("damping_cloth", C.c_float),
+                ("bending_scaling", C.c_float), ("newton_tol", C.c_float)]
+
+
+class lcsx_stats(C.Structure):
+    _fields_ = [("frame", C.c_int), ("newton_iters", C.c_int), ("pcg_iters", C.c_int), ("pcg_iters_total", C.c_int),
+                ("toi", C.c_float), ("max_dx", C.c_float), ("pin_delta", C.c_float), ("pcg_rel", C.c_float),
+                ("num_pairs_vf", C.c_int), ("num_pairs_ee", C.c_int), ("num_contacts", C.c_int),
+                ("converged", C.c_int), ("broadphase_overflow", C.c_int)]
+
+
+# Every exported symbol of include/lcsx.h with its ctypes signature.
+_SIGS = {
+    "lcsx_last_error": (C.c_char_p, []),
+    "lcsx_version": (C.c_char_p, []),
+    "lcsx_device_count": (C.c_int, []),
+    "lcsx_params_default": (None, [C.POINTER(lcsx_params)]),
+    "lcsx_material_default": (None, [C.POINTER(lcsx_material)]),
+    "lcsx_solver_create": (C.c_int, [C.POINTER(lcsx_params), C.c_int, C.POINTER(C.c_void_p)]),
+    "lcsx_solver_destroy": (C.c_int, [C.c_void_p]),
+    "lcsx_solver_add_mesh": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_int, C.POINTER(C.c_int32), C.c_int,
+                                       C.POINTER(lcsx_material), C.POINTER(C.c_int)]),
+    "lcsx_solver_add_fixed_points": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_float, C.POINTER(lcsx_fixed_anim),
+                                               C.POINTER(C.c_int)]),
+    "lcsx_solver_load_scene_json": (C.c_int, [C.c_void_p, C.c_char_p, C.c_char_p]),
+    "lcsx_solver_init": (C.c_int, [C.c_void_p]),
+    "lcsx_solver_step": (C.c_int, [C.c_void_p, C.c_int]),
+    "lcsx_solver_get_positions": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_size_t]),
+    "lcsx_solver_get_velocities": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_size_t]),
+    "lcsx_solver_set_state": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_size_t]),
+    "lcsx_solver_set_pinned_targets": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_float), C.c_int]),
+    "lcsx_solver_get_stats": (C.c_int, [C.c_void_p, C.POINTER(lcsx_stats)]),
+    "lcsx_solver_num_verts": (C.c_int, [C.c_void_p]),
+    "lcsx_solver_run_stage": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "lcsx_solver_read": (C.c_int, [C.c_void_p, C.c_char_p, C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "lcsx_solver_write": (C.c_int, [C.c_void_p, C.c_char_p, C.c_void_p, C.c_size_t]),
+    "lcsx_solver_kernel_times": (C.c_int, [C.c_void_p, C.c_char_p, C.c_size_t, C.c_int]),
+    "lcsx_solver_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
+    "lcsx_lbvh_create": (C.c_int, [C.c_int, C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]),
+    "lcsx_lbvh_destroy": (C.c_int, [C.c_void_p]),
+    "lcsx_lbvh_construct": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
+    "lcsx_lbvh_refit": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "lcsx_lbvh_query": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_uint32, C.c_void_p,
+                                  C.c_void_p]),
+    "lcsx_lbvh_read": (C.c_int, [C.c_void_p, C.c_char_p, C.c_void_p, C.c_size_t]),
+}
+
+
+def load_library(path: str = LIB_PATH) -> C.CDLL:
+    """Loads liblcsx.so (raises if it has not been built)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(path):
+        raise LcsxError(f"{path} is missing: build it with `python -m luisacomputesimulator_amd.build`")
+    lib = C.CDLL(path)
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _LIB = lib
+    return lib
+
+
+def _check(code: int) -> None:
+    if code != 0:
+        msg = load_library().lcsx_last_error().decode(errors="replace")
+        raise LcsxError(f"lcsx error {code}: {msg}")
+
+
+def params_to_c(p: Params) -> lcsx_params:
+    c = lcsx_params()
+    c.implicit_dt = p.implicit_dt
+    c.nonlinear_iter_count = p.nonlinear_iter_count
+    c.newton_max_iters = p.newton_max_iters
+    c.pcg_iter_count = p.pcg_iter_count
+    c.pcg_check_every = p.pcg_check_every
+    c.pcg_rel_tol = p.pcg_rel_tol
+    c.pcg_abs_tol = p.pcg_abs_tol
+    c.gravity[:] = list(p.gravity)
+    c.use_floor = int(p.use_floor)
+    c.use_self_collision = int(p.use_self_collision)
+    c.use_ccd_linesearch = int(p.use_ccd_linesearch)
+    c.floor_y = p.floor_y
+    c.stiffness_collision = p.stiffness_collision
+    c.stiffness_dirichlet = p.stiffness_dirichlet
+    c.damping_cloth = p.damping_cloth
+    c.bending_scaling = p.bending_scaling
+    c.newton_tol = p.newton_tol
+    return c
+
+
+def material_to_c(m: Material) -> lcsx_material:
+    c = lcsx_material()
+    c.thickness, c.youngs_modulus, c.poisson_ratio = m.thickness, m.youngs_modulus, m.poisson_ratio
+    c.area_bending_stiffness, c.density, c.mass, c.d_hat = m.area_bending_stiffness, m.density, m.mass, m.d_hat
+    return c
+
+
+def anim_to_c(a: Anim) -> lcsx_fixed_anim:
+    c = lcsx_fixed_anim()
+    c.use_translate, c.use_scale, c.use_rotate = int(a.use_translate), int(a.use_scale), int(a.use_rotate)
+    c.translate[:] = list(a.translate)
+    c.scale[:] = list(a.scale)
+    c.rot_center[:] = list(a.rot_center)
+    c.rot_axis[:] = list(a.rot_axis)
+    c.rot_deg_per_s = a.rot_deg_per_s
+    return c
+
+
+_NP = {"f": np.float32, "u": np.uint32, "q": np.uint64, "b": np.uint8}
+# dtype of each readable buffer (see lcsx_solver_read)
+_DTYPES: Dict[str, str] = {n: "u" for n in ("faces", "edges", "hinges", "csr", "stretch_faces", "bending_hinges",
+                                             "fixed_verts", "pairs_vf", "pairs_ee", "contact_idx", "contact_type")}
+_DTYPES["is_fixed"] = "b"
+
+
+def _dtype_of(name: str):
+    if name.endswith(".morton_sorted"):
+        return np.uint64
+    if name.endswith((".sorted_to_orig", ".parent", ".children", ".object_idx")):
+        return np.uint32
+    return _NP[_DTYPES.get(name, "f")]
+
+
+class NewtonSolver:
+    """GPU Newton solver for a cloth scene (reference NewtonSolver + lcs_solver_py.NewtonSolver)."""
+
+    def __init__(self, params: Optional[Params] = None, device: int = 0):
+        self.lib = load_library()
+        self.params = params or Params()
+        h = C.c_void_p()
+        p = params_to_c(self.params)
+        _check(self.lib.lcsx_solver_create(C.byref(p), device, C.byref(h)))
+        self.h = h
+        self.mesh_names: List[str] = []
+        self._inited = False
+
+    @classmethod
+    def from_scene(cls, scene: Scene, device: int = 0) -> "NewtonSolver":
+        s = cls(scene.params, device)
+        for m in scene.meshes:
+            mid = s.register_mesh(m.name, m.verts, m.tris, m.material)
+            for fp in m.fixed:
+                s.add_fixed_points(mid, fp.method, fp.range, fp.anim)
+        s.init_solver()
+        return s
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                self.lib.lcsx_solver_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    # -- lcs_solver_py surface ------------------------------------------------------------
+    def register_mesh(self, name: str, vertices, triangles, material: Optional[Material] = None) -> int:
+        v = np.ascontiguousarray(vertices, dtype=np.float32).reshape(-1, 3)
+        t = np.ascontiguousarray(triangles, dtype=np.int32).reshape(-1, 3)
+        mid = C.c_int()
+        mat = material_to_c(material or Material())
+        _check(self.lib.lcsx_solver_add_mesh(self.h, v.ctypes.data_as(C.POINTER(C.c_float)), len(v),
+                                             t.ctypes.data_as(C.POINTER(C.c_int32)), len(t), C.byref(mat), C.byref(mid)))
+        self.mesh_names.append(name)
+        return mid.value
+
+    def add_fixed_points(self, mesh_id: int, method: str, range_: float = 0.001, anim: Optional[Anim] = None) -> int:
+        n = C.c_int()
+        a = anim_to_c(anim or Anim())
+        _check(self.lib.lcsx_solver_add_fixed_points(self.h, mesh_id, FIXED_METHODS[method], range_, C.byref(a),
+                                                     C.byref(n)))
+        return n.value
+
+    def load_scene_json(self, path: str, mesh_dir: str = "") -> None:
+        _check(self.lib.lcsx_solver_load_scene_json(self.h, path.encode(), mesh_dir.encode()))
+
+    def num_meshes(self) -> int:
+        return len(self.mesh_names)
+
+    def get_mesh_names(self) -> List[str]:
+        return list(self.mesh_names)
+
+    def init_solver(self) -> None:
+        _check(self.lib.lcsx_solver_init(self.h))
+        self._inited = True
+
+    def physics_step_gpu(self, frames: int = 1) -> None:
+        _check(self.lib.lcsx_solver_step(self.h, frames))
+
+    def get_simulation_results(self) -> np.ndarray:
+        n = self.num_verts
+        out = np.empty((n, 3), dtype=np.float32)
+        _check(self.lib.lcsx_solver_get_positions(self.h, out.ctypes.data_as(C.POINTER(C.c_float)), out.size))
+        return out
+
+    def get_velocities(self) -> np.ndarray:
+        out = np.empty((self.num_verts, 3), dtype=np.float32)
+        _check(self.lib.lcsx_solver_get_velocities(self.h, out.ctypes.data_as(C.POINTER(C.c_float)), out.size))
+        return out
+
+    def set_state(self, x=None, v=None) -> None:
+        n = 3 * self.num_verts
+        xa = None if x is None else np.ascontiguousarray(x, dtype=np.float32).reshape(-1)
+        va = None if v is None else np.ascontiguousarray(v, dtype=np.float32).reshape(-1)
+        fp = C.POINTER(C.c_float)
+        _check(self.lib.lcsx_solver_set_state(self.h, None if xa is None else xa.ctypes.data_as(fp),
+                                              None if va is None else va.ctypes.data_as(fp), n))
+
+    def set_pinned_targets(self, mesh_id: int, targets) -> None:
+        t = np.ascontiguousarray(targets, dtype=np.float32).reshape(-1, 3)
+        _check(self.lib.lcsx_solver_set_pinned_targets(self.h, mesh_id, t.ctypes.data_as(C.POINTER(C.c_float)), len(t)))
+
+    @property
+    def num_verts(self) -> int:
+        return int(self.lib.lcsx_solver_num_verts(self.h))
+
+    def stats(self) -> dict:
+        s = lcsx_stats()
+        _check(self.lib.lcsx_solver_get_stats(self.h, C.byref(s)))
+        return {f: getattr(s, f) for f, _ in lcsx_stats._fields_}
+
+    # -- per-stage entry points -----------------------------------------------------------
+    def stage(self, name: str) -> None:
+        _check(self.lib.lcsx_solver_run_stage(self.h, name.encode()))
+
+    def read(self, name: str) -> np.ndarray:
+        cnt = C.c_size_t()
+        _check(self.lib.lcsx_solver_read(self.h, name.encode(), None, 0, C.byref(cnt)))
+        a = np.empty(cnt.value, dtype=_dtype_of(name))
+        _check(self.lib.lcsx_solver_read(self.h, name.encode(), a.ctypes.data_as(C.c_void_p), a.nbytes, None))
+        return a
+
+    def write(self, name: str, value) -> None:
+        a = np.ascontiguousarray(value, dtype=np.float32).reshape(-1)
+        _check(self.lib.lcsx_solver_write(self.h, name.encode(), a.ctypes.data_as(C.c_void_p), a.nbytes))
+
+    def set_timing(self, level: int) -> None:
+        _check(self.lib.lcsx_solver_set_timing(self.h, level))
+
+    def kernel_times(self, reset: bool = False) -> Dict[str, tuple]:
+        buf = C.create_string_buffer(8192)
+        _check(self.lib.lcsx_solver_kernel_times(self.h, buf, len(buf), int(reset)))
+        out = {}
+        for item in buf.value.decode().split(";"):
+            if "=" in item:
+                k, v = item.split("=")
+                ms, n = v.split(",")
+                out[k] = (float(ms), int(n))
+        return out
+
+
+def device_count() -> int:
+    return int(load_library().lcsx_device_count())

@@ -1,0 +1,572 @@
+// extern "C" boundary of the lcsx library (declared in include/lcsx.h).
+#include <cstring>
+#include <exception>
+#include <string>
+#include "lcsx_io.h"
+#include "lcsx_solver.h"
+
+namespace lcsx
+{
+static thread_local std::string g_err;
+void set_error(const std::string& msg) { g_err = msg; }
+}  // namespace lcsx
+
+using namespace lcsx;
+
+#define LCSX_GUARD(body)                                                                                    \
+    try                                                                                                     \
+    {                                                                                                       \
+        body                                                                                                \
+    }                                                                                                       \
+    catch (const HipFailure&)                                                                               \
+    {                                                                                                       \
+        return g_err.find("NaN") != std::string::npos ? LCSX_ERR_NAN : LCSX_ERR_HIP;                        \
+    }                                                                                                       \
+    catch (const NoDevice&)                                                                                 \
+    {                                                                                                       \
+        return LCSX_ERR_NO_GPU;                                                                             \
+    }                                                                                                       \
+    catch (const std::exception& e)                                                                         \
+    {                                                                                                       \
+        set_error(e.what());                                                                                \
+        return LCSX_ERR_ARG;                                                                                \
+    }
+
+struct lcsx_solver
+{
+    Solver s;
+};
+struct lcsx_lbvh
+{
+    Lbvh        t;
+    hipStream_t last = nullptr;
+};
+
+namespace
+{
+lcsx_status arg_error(const char* m)
+{
+    set_error(m);
+    return LCSX_ERR_ARG;
+}
+
+// Host copy of a named solver buffer in the canonical layout.
+struct Blob
+{
+    std::vector<uint8_t> bytes;
+    size_t               count = 0;
+    template <class T>
+    void set(const std::vector<T>& v)
+    {
+        bytes.resize(v.size() * sizeof(T));
+        if (!v.empty()) std::memcpy(bytes.data(), v.data(), bytes.size());
+        count = v.size();
+    }
+};
+
+template <class T>
+std::vector<T> dl(const DBuf<T>& b, size_t n, hipStream_t s)
+{
+    std::vector<T> h(n);
+    b.download(h.data(), n, s);
+    LCSX_HIP(hipStreamSynchronize(s));
+    return h;
+}
+
+std::vector<float> planes_to_rows(const std::vector<float>& P, size_t n)
+{
+    std::vector<float> o(9 * n);
+    for (size_t i = 0; i < n; i++)
+        for (int c = 0; c < 9; c++) o[9 * i + c] = P[c * n + i];
+    return o;
+}
+
+bool read_tree(const Lbvh& t, const std::string& f, hipStream_t s, Blob& out)
+{
+    size_t n = t.n, nn = t.n_nodes();
+    if (f == "morton_sorted") out.set(dl(t.morton_sorted, n, s));
+    else if (f == "sorted_to_orig")
+    {
+        std::vector<uint32_t> h(n);
+        if (n) LCSX_HIP(hipMemcpyAsync(h.data(), t.sorted_to_orig, n * 4, hipMemcpyDeviceToHost, s));
+        LCSX_HIP(hipStreamSynchronize(s));
+        out.set(h);
+    }
+    else if (f == "parent") out.set(dl(t.parent, nn, s));
+    else if (f == "object_idx") out.set(dl(t.object_idx, nn, s));
+    else if (f == "children")
+    {
+        auto                  c = dl(t.children, nn, s);
+        std::vector<uint32_t> h(2 * nn);
+        for (size_t i = 0; i < nn; i++)
+        {
+            h[2 * i]     = c[i].x;
+            h[2 * i + 1] = c[i].y;
+        }
+        out.set(h);
+    }
+    else if (f == "node_aabb")
+    {
+        auto               b = dl(t.nodes, nn, s);
+        std::vector<float> h(6 * nn);
+        for (size_t i = 0; i < nn; i++)
+        {
+            float v[6] = {b[i].lo.x, b[i].lo.y, b[i].lo.z, b[i].hi.x, b[i].hi.y, b[i].hi.z};
+            std::memcpy(&h[6 * i], v, sizeof(v));
+        }
+        out.set(h);
+    }
+    else if (f == "global_aabb")
+    {
+        auto               g = dl(t.gbox, 6, s);
+        std::vector<float> h(6);
+        for (int k = 0; k < 6; k++) h[k] = ord2f(g[k]);
+        out.set(h);
+    }
+    else
+        return false;
+    return true;
+}
+
+// Buffers produced by the host setup alone (readable after init even without a GPU).
+bool is_static_buffer(const std::string& n)
+{
+    static const char* k[] = {"rest_x", "faces", "edges", "hinges", "csr", "stretch_faces", "bending_hinges",
+                              "fixed_verts", "is_fixed", "face_area", "edge_area", "vert_area", "vert_mass", "offset",
+                              "dhat", "sf_Dm", "sf_area", "sf_mu", "sf_lambda", "bh_angle", "bh_hbar", "bh_k"};
+    for (const char* c : k)
+        if (n == c) return true;
+    return false;
+}
+
+bool read_named(Solver& S, const std::string& name, Blob& out)
+{
+    hipStream_t s  = S.stream;
+    size_t      nv = S.S.nv;
+    auto        v3 = [&](const DBuf<float>& b) { out.set(dl(b, 3 * nv, s)); return true; };
+    if (name == "x") return v3(S.x);
+    if (name == "v") return v3(S.v);
+    if (name == "x_step") return v3(S.x_step);
+    if (name == "x_iter") return v3(S.x_iter);
+    if (name == "x_tilde") return v3(S.x_tilde);
+    if (name == "dx") return v3(S.dx);
+    if (name == "cgB") return v3(S.cgB);
+    if (name == "rest_x") { out.set(S.S.rest_x); return true; }
+    if (name == "diag") { out.set(planes_to_rows(dl(S.diag, 9 * nv, s), nv)); return true; }
+    if (name == "offdiag")
+    {
+        auto               P = dl(S.sell_val, 9 * (size_t)S.S.nblocks, s);
+        std::vector<float> o(9 * S.S.csr.size(), 0.0f);
+        for (size_t k = 0; k < S.S.csr.size(); k++)
+        {
+            uint32_t b = S.S.csr_to_sell[k];
+            if (b == 0xffffffffu) continue;
+            for (int c = 0; c < 9; c++) o[9 * k + c] = P[(size_t)c * S.S.nblocks + b];
+        }
+        out.set(o);
+        return true;
+    }
+    if (name == "faces") { out.set(S.S.faces); return true; }
+    if (name == "edges") { out.set(S.S.edges); return true; }
+    if (name == "hinges") { out.set(S.S.hinges); return true; }
+    if (name == "csr") { out.set(S.S.csr); return true; }
+    if (name == "stretch_faces") { out.set(S.S.sf); return true; }
+    if (name == "bending_hinges") { out.set(S.S.bh); return true; }
+    if (name == "fixed_verts") { out.set(S.S.fixed_verts); return true; }
+    if (name == "is_fixed") { out.set(S.S.is_fixed); return true; }
+    if (name == "face_area") { out.set(S.S.face_area); return true; }
+    if (name == "edge_area") { out.set(S.S.edge_area); return true; }
+    if (name == "vert_area") { out.set(S.S.vert_area); return true; }
+    if (name == "vert_mass") { out.set(S.S.vert_mass); return true; }
+    if (name == "offset") { out.set(S.S.offset); return true; }
+    if (name == "dhat") { out.set(S.S.dhat); return true; }
+    if (name == "sf_Dm") { out.set(S.S.sf_Dm); return true; }
+    if (name == "sf_area") { out.set(S.S.sf_area); return true; }
+    if (name == "sf_mu") { out.set(S.S.sf_mu); return true; }
+    if (name == "sf_lambda") { out.set(S.S.sf_lambda); return true; }
+    if (name == "bh_angle") { out.set(S.S.bh_angle); return true; }
+    if (name == "bh_hbar") { out.set(S.S.bh_hbar); return true; }
+    if (name == "bh_k") { out.set(S.S.bh_k); return true; }
+    if (name == "face_stage") { out.set(dl(S.face_stage, S.face_stage.n, s)); return true; }
+    if (name == "toi")
+    {
+        S.read_status();
+        out.set(std::vector<float>{ord2f(S.host_status.toi_bits)});
+        return true;
+    }
+    if (name.rfind("face_tree.", 0) == 0) return read_tree(S.tree_face, name.substr(10), s, out);
+    if (name.rfind("edge_tree.", 0) == 0) return read_tree(S.tree_edge, name.substr(10), s, out);
+    if (name == "pairs_vf" || name == "pairs_ee")
+    {
+        S.read_status();
+        bool     vf = name == "pairs_vf";
+        uint32_t n  = S.last_bp == 2 ? (vf ? S.host_status.n_vf_ccd : S.host_status.n_ee_ccd)
+                                     : (S.last_bp == 1 ? (vf ? S.host_status.n_vf : S.host_status.n_ee) : 0u);
+        n = std::min(n, vf ? S.cap_vf : S.cap_ee);
+        out.set(dl(vf ? S.pairs_vf : S.pairs_ee, 2 * (size_t)n, s));
+        return true;
+    }
+    if (name.rfind("contact_", 0) == 0)
+    {
+        S.read_status();
+        size_t n   = std::min(S.host_status.n_contacts, S.cap_c);
+        auto   idx = dl(S.c_idx, n, s);
+        auto   nk  = dl(S.c_nk, n, s);
+        auto   w   = dl(S.c_w, n, s);
+        auto   ax  = dl(S.c_aux, n, s);
+        if (name == "contact_idx")
+        {
+            std::vector<uint32_t> h(4 * n);
+            for (size_t i = 0; i < n; i++)
+            {
+                h[4 * i] = idx[i].x & 0x7fffffffu;
+                h[4 * i + 1] = idx[i].y & 0x7fffffffu;
+                h[4 * i + 2] = idx[i].z & 0x7fffffffu;
+                h[4 * i + 3] = idx[i].w & 0x7fffffffu;
+            }
+            out.set(h);
+        }
+        else if (name == "contact_type")
+        {
+            std::vector<uint32_t> h(n);
+            for (size_t i = 0; i < n; i++) h[i] = (idx[i].z & 0x80000000u) ? 4u : 8u;
+            out.set(h);
+        }
+        else if (name == "contact_n")
+        {
+            std::vector<float> h(3 * n);
+            for (size_t i = 0; i < n; i++) { h[3 * i] = nk[i].x; h[3 * i + 1] = nk[i].y; h[3 * i + 2] = nk[i].z; }
+            out.set(h);
+        }
+        else if (name == "contact_k")
+        {
+            std::vector<float> h(3 * n);
+            for (size_t i = 0; i < n; i++) { h[3 * i] = ax[i].x; h[3 * i + 1] = nk[i].w; h[3 * i + 2] = ax[i].y; }
+            out.set(h);
+        }
+        else if (name == "contact_w")
+        {
+            std::vector<float> h(4 * n);
+            for (size_t i = 0; i < n; i++) { h[4 * i] = w[i].x; h[4 * i + 1] = w[i].y; h[4 * i + 2] = w[i].z; h[4 * i + 3] = w[i].w; }
+            out.set(h);
+        }
+        else if (name == "contact_bary")
+        {
+            std::vector<float> h(2 * n);
+            for (size_t i = 0; i < n; i++) { h[2 * i] = ax[i].z; h[2 * i + 1] = ax[i].w; }
+            out.set(h);
+        }
+        else
+            return false;
+        return true;
+    }
+    return false;
+}
+}  // namespace
+
+extern "C"
+{
+const char* lcsx_last_error(void) { return g_err.c_str(); }
+const char* lcsx_version(void) { return "lcsx 0.1 (gfx950)"; }
+int         lcsx_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+void lcsx_params_default(lcsx_params* p)
+{
+    std::memset(p, 0, sizeof(*p));
+    p->implicit_dt          = 1.0f / 60.0f;
+    p->nonlinear_iter_count = 3;
+    p->newton_max_iters     = 100;
+    p->pcg_iter_count       = 100;
+    p->pcg_check_every      = 1;
+    p->pcg_rel_tol          = 1e-4f;
+    p->pcg_abs_tol          = 1e-10f;
+    p->gravity[1]           = -9.8f;
+    p->use_floor            = 1;
+    p->use_self_collision   = 1;
+    p->use_ccd_linesearch   = 1;
+    p->floor_y              = 0.0f;
+    p->stiffness_collision  = 1e7f;
+    p->stiffness_dirichlet  = 1e9f;
+    p->damping_cloth        = 2.0f;
+    p->bending_scaling      = 1.0f;
+    p->newton_tol           = 1e-2f;
+}
+void lcsx_material_default(lcsx_material* m)
+{
+    m->thickness              = 2e-3f;
+    m->youngs_modulus         = 1e5f;
+    m->poisson_ratio          = 0.25f;
+    m->area_bending_stiffness = 5e-3f;
+    m->density                = 1e3f;
+    m->mass                   = 0.0f;
+    m->d_hat                  = 2e-3f;
+}
+
+lcsx_status lcsx_solver_create(const lcsx_params* params, int device, lcsx_solver** out)
+{
+    if (!out) return arg_error("out is NULL");
+    auto* h = new lcsx_solver();
+    if (params) h->s.prm = *params;
+    else lcsx_params_default(&h->s.prm);
+    h->s.device = device;
+    *out        = h;
+    return LCSX_OK;
+}
+lcsx_status lcsx_solver_destroy(lcsx_solver* s)
+{
+    delete s;
+    return LCSX_OK;
+}
+
+lcsx_status lcsx_solver_add_mesh(lcsx_solver* s, const float* verts, int n_verts, const int32_t* tris, int n_tris,
+                                 const lcsx_material* material, int* mesh_id)
+{
+    if (!s || !verts || !tris || n_verts <= 0 || n_tris <= 0) return arg_error("invalid mesh arguments");
+    if (s->s.inited) { set_error("add_mesh after init"); return LCSX_ERR_STATE; }
+    for (int i = 0; i < 3 * n_tris; i++)
+        if (tris[i] < 0 || tris[i] >= n_verts) return arg_error("triangle index out of range");
+    MeshDesc m;
+    m.x.assign(verts, verts + 3 * (size_t)n_verts);
+    m.tris.assign(tris, tris + 3 * (size_t)n_tris);
+    if (material) m.mat = *material;
+    else lcsx_material_default(&m.mat);
+    s->s.meshes.push_back(std::move(m));
+    if (mesh_id) *mesh_id = (int)s->s.meshes.size() - 1;
+    return LCSX_OK;
+}
+
+lcsx_status lcsx_solver_add_fixed_points(lcsx_solver* s, int mesh_id, int method, float range, const lcsx_fixed_anim* anim,
+                                         int* num_selected)
+{
+    if (!s || mesh_id < 0 || mesh_id >= (int)s->s.meshes.size()) return arg_error("invalid mesh id");
+    if (s->s.inited) { set_error("add_fixed_points after init"); return LCSX_ERR_STATE; }
+    if (method < 0 || method > 21 || method == 1 || method == 2) return arg_error("unsupported fixed-point method");
+    lcsx_fixed_anim a{};
+    a.scale[0] = a.scale[1] = a.scale[2] = 1.0f;
+    a.rot_axis[0] = 1.0f;
+    if (anim) a = *anim;
+    MeshDesc& m  = s->s.meshes[mesh_id];
+    size_t    n0 = m.pinned.size();
+    pin_by_method(m, method, range, a);
+    if (num_selected) *num_selected = (int)(m.pinned.size() - n0);
+    return LCSX_OK;
+}
+
+lcsx_status lcsx_solver_load_scene_json(lcsx_solver* s, const char* json_path, const char* mesh_dir)
+{
+    if (!s || !json_path) return arg_error("invalid arguments");
+    if (s->s.inited) { set_error("load_scene_json after init"); return LCSX_ERR_STATE; }
+    std::string err;
+    if (!load_scene_json(json_path, mesh_dir ? mesh_dir : "", s->s.prm, s->s.meshes, err))
+    {
+        set_error(err);
+        return LCSX_ERR_IO;
+    }
+    return LCSX_OK;
+}
+
+lcsx_status lcsx_solver_init(lcsx_solver* s)
+{
+    if (!s) return arg_error("solver is NULL");
+    if (s->s.meshes.empty()) { set_error("no meshes"); return LCSX_ERR_STATE; }
+    LCSX_GUARD(s->s.init(); return LCSX_OK;)
+}
+
+lcsx_status lcsx_solver_step(lcsx_solver* s, int frames)
+{
+    if (!s || frames < 0) return arg_error("invalid arguments");
+    if (!s->s.inited) { set_error("step before init"); return LCSX_ERR_STATE; }
+    LCSX_GUARD(for (int f = 0; f < frames; f++) s->s.step(); return LCSX_OK;)
+}
+
+lcsx_status lcsx_solver_get_positions(lcsx_solver* s, float* out, size_t n)
+{
+    if (!s || !out || !s->s.inited || n < 3 * (size_t)s->s.S.nv) return arg_error("invalid arguments");
+    LCSX_GUARD(s->s.x.download(out, 3 * (size_t)s->s.S.nv, s->s.stream); LCSX_HIP(hipStreamSynchronize(s->s.stream)); return LCSX_OK;)
+}
+lcsx_status lcsx_solver_get_velocities(lcsx_solver* s, float* out, size_t n)
+{
+    if (!s || !out || !s->s.inited || n < 3 * (size_t)s->s.S.nv) return arg_error("invalid arguments");
+    LCSX_GUARD(s->s.v.download(out, 3 * (size_t)s->s.S.nv, s->s.stream); LCSX_HIP(hipStreamSynchronize(s->s.stream)); return LCSX_OK;)
+}
+lcsx_status lcsx_solver_set_state(lcsx_solver* s, const float* x, const float* v, size_t n)
+{
+    if (!s || !s->s.inited || n != 3 * (size_t)s->s.S.nv) return arg_error("invalid arguments");
+    LCSX_GUARD(if (x) s->s.x.upload(x, n, s->s.stream); if (v) s->s.v.upload(v, n, s->s.stream);
+               LCSX_HIP(hipStreamSynchronize(s->s.stream)); return LCSX_OK;)
+}
+lcsx_status lcsx_solver_set_pinned_targets(lcsx_solver* s, int mesh_id, const float* targets, int n_pinned)
+{
+    if (!s || !s->s.inited || mesh_id < 0 || mesh_id >= (int)s->s.meshes.size() || !targets)
+        return arg_error("invalid arguments");
+    Solver& S = s->s;
+    int     k = 0;
+    for (size_t i = 0; i < S.S.fixed_verts.size(); i++)
+        if ((int)S.S.fixed_mesh[i] == mesh_id)
+        {
+            if (k >= n_pinned) return arg_error("n_pinned smaller than the mesh's pinned count");
+            for (int d = 0; d < 3; d++) S.explicit_targets[3 * i + d] = targets[3 * k + d];
+            k++;
+        }
+    if (k != n_pinned) return arg_error("n_pinned does not match the mesh's pinned count");
+    S.has_explicit[mesh_id] = 1;
+    return LCSX_OK;
+}
+lcsx_status lcsx_solver_get_stats(lcsx_solver* s, lcsx_stats* out)
+{
+    if (!s || !out) return arg_error("invalid arguments");
+    *out = s->s.stats;
+    return LCSX_OK;
+}
+int lcsx_solver_num_verts(lcsx_solver* s) { return s && s->s.inited ? (int)s->s.S.nv : 0; }
+
+lcsx_status lcsx_solver_run_stage(lcsx_solver* s, const char* stage)
+{
+    if (!s || !stage) return arg_error("invalid arguments");
+    if (!s->s.inited) { set_error("stage before init"); return LCSX_ERR_STATE; }
+    std::string st = stage;
+    Solver&     S  = s->s;
+    LCSX_GUARD(
+        if (st == "frame_begin") S.frame_begin();
+        else if (st == "predict") S.predict();
+        else if (st == "iter_begin") S.iter_begin();
+        else if (st == "material") S.material();
+        else if (st == "broad_phase_dcd") { launch_reset_status(S.status.p, S.stream); S.broad_phase(false); }
+        else if (st == "broad_phase_ccd") { launch_reset_line_search(S.status.p, S.stream); S.broad_phase(true); }
+        else if (st == "dcd") S.dcd();
+        else if (st == "update_contacts") { launch_reset_status(S.status.p, S.stream); S.update_contacts(); }
+        else if (st == "contact_assemble") S.contact_assemble();
+        else if (st == "pcg") S.pcg();
+        else if (st == "ccd") { S.ccd(); }
+        else if (st == "line_search") { S.line_search(); }
+        else if (st == "newton_iteration") { while (!S.newton_iteration()) {} }
+        else if (st == "frame_end") S.frame_end();
+        else if (st == "step") S.step();
+        else if (st == "spmv_dx") S.spmv_dx();
+        else { set_error("unknown stage " + st); return LCSX_ERR_ARG; }
+        S.read_status();
+        if (S.host_status.n_vf > S.cap_vf || S.host_status.n_ee > S.cap_ee || S.host_status.n_contacts > S.cap_c ||
+            S.host_status.n_vf_ccd > S.cap_vf || S.host_status.n_ee_ccd > S.cap_ee)
+        {
+            S.grow_if_needed(false);
+            set_error("contact buffers were too small and have been grown; re-run the stage");
+            return LCSX_ERR_CAPACITY;
+        }
+        return LCSX_OK;)
+}
+
+lcsx_status lcsx_solver_read(lcsx_solver* s, const char* name, void* out, size_t bytes, size_t* count_out)
+{
+    if (!s || !name) return arg_error("invalid arguments");
+    if (!s->s.inited && !(s->s.setup_ready && is_static_buffer(name)))
+    {
+        set_error("read before init");
+        return LCSX_ERR_STATE;
+    }
+    LCSX_GUARD(
+        Blob b;
+        if (!read_named(s->s, name, b)) { set_error(std::string("unknown buffer ") + name); return LCSX_ERR_ARG; }
+        if (count_out) *count_out = b.count;
+        if (out)
+        {
+            if (bytes < b.bytes.size()) return arg_error("output buffer too small");
+            if (!b.bytes.empty()) std::memcpy(out, b.bytes.data(), b.bytes.size());
+        }
+        return LCSX_OK;)
+}
+
+lcsx_status lcsx_solver_write(lcsx_solver* s, const char* name, const void* in, size_t bytes)
+{
+    if (!s || !name || !in) return arg_error("invalid arguments");
+    if (!s->s.inited) { set_error("write before init"); return LCSX_ERR_STATE; }
+    Solver&      S = s->s;
+    std::string  n = name;
+    DBuf<float>* b = nullptr;
+    if (n == "x") b = &S.x;
+    else if (n == "v") b = &S.v;
+    else if (n == "x_step") b = &S.x_step;
+    else if (n == "x_iter") b = &S.x_iter;
+    else if (n == "x_tilde") b = &S.x_tilde;
+    else if (n == "dx") b = &S.dx;
+    else if (n == "cgB") b = &S.cgB;
+    if (!b) { set_error("buffer " + n + " is not writable"); return LCSX_ERR_ARG; }
+    if (bytes != 3 * (size_t)S.S.nv * sizeof(float)) return arg_error("size mismatch");
+    LCSX_GUARD(b->upload((const float*)in, 3 * (size_t)S.S.nv, S.stream); LCSX_HIP(hipStreamSynchronize(S.stream)); return LCSX_OK;)
+}
+
+lcsx_status lcsx_solver_kernel_times(lcsx_solver* s, char* buf, size_t len, int reset)
+{
+    if (!s || !buf || len == 0) return arg_error("invalid arguments");
+    std::string o;
+    for (auto& kv : s->s.times)
+        o += kv.first + "=" + std::to_string(kv.second.second ? kv.second.first / kv.second.second : 0.0) + "," +
+             std::to_string(kv.second.second) + ";";
+    std::strncpy(buf, o.c_str(), len - 1);
+    buf[len - 1] = 0;
+    if (reset) s->s.times.clear();
+    return o.size() < len ? LCSX_OK : LCSX_ERR_ARG;
+}
+
+lcsx_status lcsx_solver_set_timing(lcsx_solver* s, int level)
+{
+    if (!s || level < 0 || level > 2) return arg_error("invalid arguments");
+    s->s.timing      = level >= 1;
+    s->s.timing_spmv = level >= 2;
+    return LCSX_OK;
+}
+
+// ------------------------------------------------------------ standalone LBVH
+lcsx_status lcsx_lbvh_create(int arity, const uint32_t* device_prims, int n_leaves, lcsx_lbvh** out)
+{
+    if (!out || (arity != 2 && arity != 3) || n_leaves <= 0 || !device_prims) return arg_error("invalid arguments");
+    auto* t = new lcsx_lbvh();
+    LCSX_GUARD(t->t.init(arity, device_prims, (uint32_t)n_leaves); *out = t; return LCSX_OK;)
+}
+lcsx_status lcsx_lbvh_destroy(lcsx_lbvh* t)
+{
+    delete t;
+    return LCSX_OK;
+}
+lcsx_status lcsx_lbvh_construct(lcsx_lbvh* t, const float* x, int n_verts, void* stream)
+{
+    if (!t || !x || n_verts <= 0) return arg_error("invalid arguments");
+    t->last = (hipStream_t)stream;
+    LCSX_GUARD(t->t.construct(x, (uint32_t)n_verts, (hipStream_t)stream); return LCSX_OK;)
+}
+lcsx_status lcsx_lbvh_refit(lcsx_lbvh* t, const float* x0, const float* x1, const float* pad, void* stream)
+{
+    if (!t || !x0 || !x1 || !pad) return arg_error("invalid arguments");
+    t->last = (hipStream_t)stream;
+    LCSX_GUARD(t->t.refit(x0, x1, pad, (hipStream_t)stream); return LCSX_OK;)
+}
+lcsx_status lcsx_lbvh_query(lcsx_lbvh* t, const float* boxes, int n_queries, int edge_filter, uint32_t* pairs,
+                            uint32_t capacity, uint32_t* count, void* stream)
+{
+    if (!t || !boxes || n_queries < 0 || !pairs || !count) return arg_error("invalid arguments");
+    QueryArgs q{};
+    q.mode        = QueryBoxes;
+    q.n_queries   = (uint32_t)n_queries;
+    q.boxes       = boxes;
+    q.edge_filter = edge_filter;
+    q.pairs       = pairs;
+    q.capacity    = capacity;
+    q.count       = count;
+    t->last       = (hipStream_t)stream;
+    LCSX_GUARD(lbvh_query(t->t, q, (hipStream_t)stream); return LCSX_OK;)
+}
+lcsx_status lcsx_lbvh_read(lcsx_lbvh* t, const char* name, void* out, size_t bytes)
+{
+    if (!t || !name || !out) return arg_error("invalid arguments");
+    LCSX_GUARD(
+        Blob b;
+        if (!read_tree(t->t, name, t->last, b)) { set_error(std::string("unknown tree buffer ") + name); return LCSX_ERR_ARG; }
+        if (bytes < b.bytes.size()) return arg_error("output buffer too small");
+        std::memcpy(out, b.bytes.data(), b.bytes.size());
+        return LCSX_OK;)
+}
+}

@@ -1,0 +1,113 @@
+// Host-side declarations shared by the lcsx translation units (not part of the C ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+
+namespace lcsx
+{
+
+void set_error(const std::string& msg);
+#define LCSX_HIP(call)                                                                                     \
+    do                                                                                                     \
+    {                                                                                                      \
+        hipError_t e_ = (call);                                                                            \
+        if (e_ != hipSuccess)                                                                              \
+        {                                                                                                  \
+            ::lcsx::set_error(std::string(#call) + ": " + hipGetErrorString(e_));                         \
+            throw ::lcsx::HipFailure();                                                                    \
+        }                                                                                                  \
+    } while (0)
+struct HipFailure
+{
+};
+struct NoDevice
+{
+};
+
+template <class T>
+struct DBuf
+{
+    T*     p = nullptr;
+    size_t n = 0;
+    void   alloc(size_t count)
+    {
+        free();
+        n = count;
+        if (count) LCSX_HIP(hipMalloc((void**)&p, count * sizeof(T)));
+    }
+    void free()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void upload(const T* h, size_t count, hipStream_t s)
+    {
+        if (count > n) alloc(count);
+        if (count) LCSX_HIP(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+    void upload(const std::vector<T>& h, hipStream_t s) { upload(h.data(), h.size(), s); }
+    void download(T* h, size_t count, hipStream_t s) const
+    {
+        if (count) LCSX_HIP(hipMemcpyAsync(h, p, count * sizeof(T), hipMemcpyDeviceToHost, s));
+    }
+    ~DBuf() { free(); }
+    DBuf() = default;
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+};
+
+struct NodeBox
+{
+    float4 lo, hi;  // .w unused
+};
+
+// ------------------------------------------------------------------ LBVH
+struct Lbvh
+{
+    int               arity = 3;
+    uint32_t          n     = 0;  // leaves
+    const uint32_t*   prims = nullptr;  // device, arity per leaf (not owned)
+    DBuf<uint32_t>    keys[2], vals[2], hist;
+    DBuf<float4>      center;
+    DBuf<uint64_t>    morton_sorted;
+    DBuf<uint32_t>    parent, object_idx, counters, gbox;
+    DBuf<uint2>       children;
+    DBuf<NodeBox>     nodes;
+    uint32_t*         sorted_to_orig = nullptr;  // points into vals[?] after construct
+
+    void init(int arity_, const uint32_t* device_prims, uint32_t n_leaves);
+    void construct(const float* x, uint32_t n_verts_for_reduce, hipStream_t s);
+    void refit(const float* x0, const float* x1, const float* pad, hipStream_t s);
+    uint32_t n_inner() const { return n ? n - 1 : 0; }
+    uint32_t n_nodes() const { return n ? 2 * n - 1 : 0; }
+};
+
+// Query modes for the solver's broad phase.
+enum QueryMode
+{
+    QueryVerts = 0,  // query box per vertex: AABB(x0[i], x1[i]) padded by offset[i] + dhat[i]
+    QueryEdges = 1,  // per edge: AABB of both endpoints at x0, x1, padded by offset[e0] + dhat[e0]; keeps e < leaf
+    QueryBoxes = 2   // explicit boxes (6 floats each), optional edge filter
+};
+struct QueryArgs
+{
+    int             mode;
+    uint32_t        n_queries;
+    const float*    x0;
+    const float*    x1;
+    const uint32_t* edges;
+    const float*    offset;
+    const float*    dhat;
+    const float*    boxes;
+    int             edge_filter;
+    uint32_t*       pairs;
+    uint32_t        capacity;
+    uint32_t*       count;
+    uint32_t*       overflow;  // set to 1 if a traversal stack overflowed (may be null)
+};
+void lbvh_query(const Lbvh& t, const QueryArgs& a, hipStream_t s);
+
+}  // namespace lcsx

@@ -1,0 +1,169 @@
+// Kernel argument structs and launchers of lcsx_kernels.hip.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace lcsx
+{
+
+// Per-iteration status counters, zeroed by launch_reset_status.
+struct Status
+{
+    uint32_t n_vf, n_ee, n_contacts, overflow;  // broad-phase counts, contacts, stack overflow flag
+    uint32_t toi_bits;                          // ordered key (f2ord) of the step bound
+    uint32_t maxdx_bits;                        // float bits of max |dx|
+    uint32_t pin_bits;                          // float bits of max pinned |x - x_tilde|
+    uint32_t nan;
+    uint32_t n_vf_ccd, n_ee_ccd, pad0, pad1;
+};
+
+struct PcgState
+{
+    float    rz, rz0, alpha, beta;
+    uint32_t converged, iters, ticket, nan;
+};
+
+struct SellView
+{
+    const uint32_t* sl_off;
+    const uint32_t* col;
+    const float*    val;   // 9 planes of nblocks
+    const float*    diag;  // 9 planes of n
+    uint32_t        nblocks;
+};
+
+struct ContactView
+{
+    const uint4*    idx;
+    const float4*   nk;
+    const float4*   w;
+    const float4*   aux;
+    const uint32_t* count;
+    uint32_t        capacity;
+};
+struct ContactBuf
+{
+    uint4*    idx;
+    float4*   nk;
+    float4*   w;
+    float4*   aux;
+    uint32_t* count;
+    uint32_t  capacity;
+};
+
+struct VertArgs
+{
+    uint32_t        nv;
+    const float*    x;
+    const float*    x_tilde;
+    const float*    mass;
+    const uint8_t*  fixed;
+    const float*    area;
+    const float*    offset;
+    const float*    dhat;
+    float           h2inv, kD, kappa, floor_y;
+    int             use_floor;
+    const uint32_t* v_sf_ptr;
+    const uint32_t* v_sf;
+    const uint32_t* v_bh_ptr;
+    const uint32_t* v_bh;
+    const float*    face_stage;
+    const float*    hinge_stage;
+    float*          cgB;
+    float*          diag;
+};
+
+struct DcdArgs
+{
+    const uint32_t* pairs;
+    const uint32_t* pair_count;
+    uint32_t        pair_capacity;
+    const float*    x;
+    const uint32_t* faces;
+    const uint32_t* edges;
+    const float*    area_a;  // VF: vertex areas, EE: edge areas
+    const float*    area_b;  // VF: face areas,   EE: edge areas
+    const float*    offset;
+    const float*    dhat;
+    float           kappa;
+    ContactBuf      out;
+};
+
+struct CcdArgs
+{
+    int             kind;  // 0 VF, 1 EE
+    const uint32_t* pairs;
+    const uint32_t* pair_count;
+    uint32_t        pair_capacity;
+    const float*    x0;
+    const float*    x1;
+    const uint32_t* faces;
+    const uint32_t* edges;
+    const float*    offset;
+    uint32_t*       toi_bits;
+};
+
+struct LsArgs
+{
+    uint32_t       nv;
+    const float*   x_iter;
+    const float*   x;
+    const float*   dx;
+    const uint8_t* fixed;
+    const float*   offset;
+    float          floor_y;
+    int            use_floor;
+    Status*        st;
+};
+
+struct PcgArgs
+{
+    uint32_t     n;
+    const float* b;
+    const float* diag;
+    float*       minv;
+    float*       x;
+    float*       r;
+    float*       z;
+    float*       p_old;
+    float*       p_new;
+    float*       q;
+    float*       partials;
+    PcgState*    s;
+    float        rel2, abs_tol;
+    uint32_t     check_every;
+};
+
+void launch_pin_velocity(uint32_t nfix, const uint32_t* ids, const float* targets, const float* x, float* v, float dt,
+                         hipStream_t s);
+void launch_predict(uint32_t nv, const float* x_step, const float* v, const uint8_t* fixed, float h, const float g[3],
+                    float* x_iter, float* x_tilde, float* x, hipStream_t s);
+void launch_frame_end(uint32_t nv, float* x_step, const float* x, float* v, float h, float damp, hipStream_t s);
+void launch_face_stage(uint32_t nsf, const uint32_t* sf, const float4* Dm, const float4* c, const float* x, float* stage,
+                       hipStream_t s);
+void launch_hinge_stage(uint32_t nbh, const uint32_t* bh, const float4* c, float scaling, const float* x, float* stage,
+                        hipStream_t s);
+void launch_vert_assemble(const VertArgs& a, hipStream_t s);
+void launch_offdiag(uint32_t nblocks, const uint32_t* ptr, const uint32_t* con, const float* fs, const float* hs, float* val,
+                    hipStream_t s);
+void launch_dcd(const DcdArgs& a, int kind, unsigned grid, hipStream_t s);
+void launch_contact_assemble(const ContactView& c, float* cgB, float* diag, uint32_t nv, unsigned grid, hipStream_t s);
+void launch_pcg_init(const PcgArgs& a, hipStream_t s);
+// one PCG iteration = spmv (material part, forms p = z + beta p on the fly), contact part, finish
+// (alpha, x/r/z update, beta and the convergence flag)
+void launch_pcg_spmv(const PcgArgs& a, const SellView& A, hipStream_t s);
+void launch_pcg_contact_spmv(const PcgArgs& a, const ContactView& c, unsigned contact_grid, hipStream_t s);
+void launch_pcg_finish(const PcgArgs& a, hipStream_t s);
+// y = A p (diagonal + material blocks + contacts), for the spmv stage
+void launch_spmv(const SellView& A, const ContactView* c, unsigned contact_grid, const float* p, float* y, uint32_t n,
+                 hipStream_t s);
+void launch_apply_dx(uint32_t nv, const float* x_iter, const float* dx, float* x, const uint32_t* alpha_key,
+                     float alpha_const, hipStream_t s);
+void launch_ground_ccd_maxdx(const LsArgs& a, hipStream_t s);
+void launch_ccd(const CcdArgs& a, unsigned grid, hipStream_t s);
+void launch_pin_delta(uint32_t nfix, const uint32_t* ids, const float* x, const float* x_tilde, uint32_t* bits,
+                      hipStream_t s);
+void launch_reset_status(Status* st, hipStream_t s);
+void launch_reset_line_search(Status* st, hipStream_t s);
+
+}  // namespace lcsx

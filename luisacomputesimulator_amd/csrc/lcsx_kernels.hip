@@ -1,0 +1,726 @@
+// Newton-step kernels for gfx950: material evaluation and owner-computes assembly, DCD/CCD
+// narrow phase, contact assembly, SELL-64 block SpMV, block-Jacobi PCG and line search.
+// Reference counterparts are cited per kernel (paths relative to the reference root).
+#include "lcsx_kernels.h"
+#include "lcsx_physics.h"
+
+namespace lcsx
+{
+
+namespace
+{
+constexpr int kB = 256;
+
+// Block-wide sum (256 threads); the result is valid in thread 0. Fixed order: wave butterflies,
+// then the four wave sums in wave order.
+LCSX_D float block_sum(float v, float* sh)
+{
+    v     = wave_sum(v);
+    int w = threadIdx.x >> 6;
+    if (lane_id() == 0) sh[w] = v;
+    __syncthreads();
+    float r = 0.0f;
+    if (threadIdx.x == 0) r = ((sh[0] + sh[1]) + sh[2]) + sh[3];
+    __syncthreads();
+    return r;
+}
+
+// Deterministic grid reduction: every block writes its partial; the last block to arrive sums
+// the partials in index order (strided per thread, then block_sum) and returns true in that block.
+LCSX_D bool last_block_sum(float local, float* partials, uint32_t* ticket, float* total, float* sh)
+{
+    float bs = block_sum(local, sh);
+    __shared__ bool last;
+    if (threadIdx.x == 0)
+    {
+        partials[blockIdx.x] = bs;
+        __threadfence();
+        last = atomicAdd(ticket, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last) return false;
+    __threadfence();
+    float s = 0.0f;
+    for (uint32_t i = threadIdx.x; i < gridDim.x; i += kB) s = s + partials[i];
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0)
+    {
+        *total  = s;
+        *ticket = 0u;
+    }
+    return true;
+}
+
+LCSX_D m3 ld_planes(const float* __restrict__ P, size_t stride, size_t i)
+{
+    m3 a;
+#pragma unroll
+    for (int c = 0; c < 9; c++) a.m[c] = P[c * stride + i];
+    return a;
+}
+LCSX_D void st_planes(float* __restrict__ P, size_t stride, size_t i, const m3& a)
+{
+#pragma unroll
+    for (int c = 0; c < 9; c++) P[c * stride + i] = a.m[c];
+}
+
+// ---------------------------------------------------------------- frame
+// solver_interface.cpp:130-162: pinned velocity = (target - x) / dt
+__global__ void k_pin_velocity(uint32_t nfix, const uint32_t* __restrict__ ids, const float* __restrict__ targets,
+                               const float* __restrict__ x, float* __restrict__ v, float dt)
+{
+    uint32_t k = blockIdx.x * kB + threadIdx.x;
+    if (k >= nfix) return;
+    uint32_t i = ids[k];
+    st3(v, i, (ld3(targets, k) - ld3(x, i)) / dt);
+}
+
+// fn_predict_position, newton_solver.cpp:577-608
+__global__ void k_predict(uint32_t nv, const float* __restrict__ x_step, const float* __restrict__ v,
+                          const uint8_t* __restrict__ fixed, float h, f3 g, float* __restrict__ x_iter,
+                          float* __restrict__ x_tilde, float* __restrict__ x)
+{
+    uint32_t i = blockIdx.x * kB + threadIdx.x;
+    if (i >= nv) return;
+    f3 xs = ld3(x_step, i), vi = ld3(v, i);
+    f3 vp = fixed[i] ? vi : vi + h * g;
+    st3(x_iter, i, xs);
+    st3(x_tilde, i, xs + h * vp);
+    st3(x, i, xs);
+}
+
+// fn_update_velocity, newton_solver.cpp:610-636
+__global__ void k_frame_end(uint32_t nv, float* __restrict__ x_step, const float* __restrict__ x, float* __restrict__ v,
+                            float h, float damp)
+{
+    uint32_t i = blockIdx.x * kB + threadIdx.x;
+    if (i >= nv) return;
+    f3 xe = ld3(x, i);
+    st3(v, i, ((xe - ld3(x_step, i)) / h) * damp);
+    st3(x_step, i, xe);
+}
+
+// ---------------------------------------------------------------- material
+// fn_evaluate_stretch_face, newton_solver.cpp:1053-1100. Staging per face (64 floats):
+// g0 g1 g2 | H00 H11 H22 | H01 H02 H12 (row-major 3x3 blocks), 1 float pad.
+__global__ __launch_bounds__(kB) void k_face_stage(uint32_t nsf, const uint32_t* __restrict__ sf,
+                                                    const float4* __restrict__ Dm, const float4* __restrict__ c,
+                                                    const float* __restrict__ x, float* __restrict__ stage)
+{
+    uint32_t f = blockIdx.x * kB + threadIdx.x;
+    if (f >= nsf) return;
+    float4  d4 = Dm[f], c4 = c[f];
+    float   d[4] = {d4.x, d4.y, d4.z, d4.w};
+    FaceOut o;
+    bw_face(ld3(x, sf[3 * f]), ld3(x, sf[3 * f + 1]), ld3(x, sf[3 * f + 2]), d, c4.y, c4.z, c4.x, o);
+    float out[64];
+    for (int a = 0; a < 3; a++)
+    {
+        out[3 * a] = o.g[a].x;
+        out[3 * a + 1] = o.g[a].y;
+        out[3 * a + 2] = o.g[a].z;
+    }
+    for (int a = 0; a < 3; a++)
+        for (int k = 0; k < 9; k++) out[9 + 9 * a + k] = o.H[a][a].m[k];
+    for (int k = 0; k < 9; k++)
+    {
+        out[36 + k] = o.H[0][1].m[k];
+        out[45 + k] = o.H[0][2].m[k];
+        out[54 + k] = o.H[1][2].m[k];
+    }
+    out[63]   = 0.0f;
+    float4* S = reinterpret_cast<float4*>(stage + 64 * (size_t)f);
+#pragma unroll
+    for (int k = 0; k < 16; k++) S[k] = make_float4(out[4 * k], out[4 * k + 1], out[4 * k + 2], out[4 * k + 3]);
+}
+
+// fn_evaluate_bending, newton_solver.cpp:1102-1172. Staging per hinge (16 floats):
+// dtheta/dx_0..3 (12), stiff, stiff * (theta - theta_rest), 2 pad.
+__global__ __launch_bounds__(kB) void k_hinge_stage(uint32_t nbh, const uint32_t* __restrict__ bh,
+                                                     const float4* __restrict__ c, float scaling,
+                                                     const float* __restrict__ x, float* __restrict__ stage)
+{
+    uint32_t e = blockIdx.x * kB + threadIdx.x;
+    if (e >= nbh) return;
+    f3 p0 = ld3(x, bh[4 * e]), p1 = ld3(x, bh[4 * e + 1]), p2 = ld3(x, bh[4 * e + 2]), p3 = ld3(x, bh[4 * e + 3]);
+    f3 g[4];
+    float  ang = dihedral_angle(p0, p1, p2, p3);
+    dihedral_grad(p0, p1, p2, p3, g);
+    float4 cc    = c[e];  // rest angle, h_bar, stiffness
+    float  stiff = (cc.z * scaling) * cc.y;
+    float  coef  = stiff * (ang - cc.x);
+    float4* S    = reinterpret_cast<float4*>(stage + 16 * (size_t)e);
+    S[0]         = make_float4(g[0].x, g[0].y, g[0].z, g[1].x);
+    S[1]         = make_float4(g[1].y, g[1].z, g[2].x, g[2].y);
+    S[2]         = make_float4(g[2].z, g[3].x, g[3].y, g[3].z);
+    S[3]         = make_float4(stiff, coef, 0.0f, 0.0f);
+}
+
+LCSX_D f3 hinge_g(const float* H, uint32_t a) { return mk(H[3 * a], H[3 * a + 1], H[3 * a + 2]); }
+
+// Inertia (fn_evaluate_inertia 819-850), ground barrier (883-950) and the per-vertex part of the
+// assembly (compile_assembly 705-814): gradient and diagonal block, faces then hinges in order.
+__global__ __launch_bounds__(kB) void k_vert_assemble(VertArgs a)
+{
+    uint32_t i = blockIdx.x * kB + threadIdx.x;
+    if (i >= a.nv) return;
+    float m    = a.mass[i];
+    f3    grad = ((-m) * a.h2inv) * (ld3(a.x, i) - ld3(a.x_tilde, i));
+    m3    H    = m3_zero();
+    float d    = (1.0f * m) * a.h2inv;
+    H.m[0] = H.m[4] = H.m[8] = d;
+    bool fixed = a.fixed[i] != 0;
+    if (fixed)
+    {
+        grad = grad + a.kD * grad;
+        for (int k = 0; k < 9; k++) H.m[k] = H.m[k] + a.kD * H.m[k];
+    }
+    if (a.use_floor && !fixed)
+    {
+        float dh = a.dhat[i], th = a.offset[i], dist = a.x[3 * (size_t)i + 1] - a.floor_y;
+        if (dist - th < dh)
+        {
+            float stiff = a.kappa * a.area[i];
+            grad.y      = grad.y - stiff * barrier_d1(dist - th, dh);
+            H.m[4]      = H.m[4] + stiff * barrier_d2(dist - th, dh);
+        }
+    }
+    for (uint32_t p = a.v_sf_ptr[i]; p < a.v_sf_ptr[i + 1]; p++)
+    {
+        uint32_t     f = a.v_sf[p] >> 2, l = a.v_sf[p] & 3u;
+        const float* S = a.face_stage + 64 * (size_t)f;
+        grad           = grad + (-mk(S[3 * l], S[3 * l + 1], S[3 * l + 2]));
+        for (int k = 0; k < 9; k++) H.m[k] = H.m[k] + S[9 + 9 * l + k];
+    }
+    for (uint32_t p = a.v_bh_ptr[i]; p < a.v_bh_ptr[i + 1]; p++)
+    {
+        uint32_t     e = a.v_bh[p] >> 2, l = a.v_bh[p] & 3u;
+        const float* S = a.hinge_stage + 16 * (size_t)e;
+        f3           g = hinge_g(S, l);
+        float        stiff = S[12], coef = S[13];
+        grad               = grad + (-(coef * g));
+        float gv[3]        = {g.x, g.y, g.z};
+        for (int r = 0; r < 3; r++)
+            for (int cc = 0; cc < 3; cc++) H.m[3 * r + cc] = H.m[3 * r + cc] + stiff * (gv[r] * gv[cc]);
+    }
+    st3(a.cgB, i, grad);
+    st_planes(a.diag, a.nv, i, H);
+}
+
+// Off-diagonal blocks: one thread per SELL block sums its contributors in order
+// (faces of the row vertex, then hinges) and writes the block once.
+__global__ __launch_bounds__(kB) void k_offdiag(uint32_t nblocks, const uint32_t* __restrict__ ptr,
+                                                 const uint32_t* __restrict__ con, const float* __restrict__ face_stage,
+                                                 const float* __restrict__ hinge_stage, float* __restrict__ val)
+{
+    uint32_t b = blockIdx.x * kB + threadIdx.x;
+    if (b >= nblocks) return;
+    m3 H = m3_zero();
+    for (uint32_t p = ptr[b]; p < ptr[b + 1]; p++)
+    {
+        uint32_t c = con[p], el = c & 0x07ffffffu, la = (c >> 29) & 3u, lb = (c >> 27) & 3u;
+        if (c & 0x80000000u)
+        {
+            const float* S = hinge_stage + 16 * (size_t)el;
+            f3    ga = hinge_g(S, la), gb = hinge_g(S, lb);
+            float st = S[12];
+            float u[3] = {ga.x, ga.y, ga.z}, w[3] = {gb.x, gb.y, gb.z};
+            for (int r = 0; r < 3; r++)
+                for (int cc = 0; cc < 3; cc++) H.m[3 * r + cc] = H.m[3 * r + cc] + st * (u[r] * w[cc]);
+        }
+        else
+        {
+            const float* S   = face_stage + 64 * (size_t)el;
+            uint32_t     lo  = la < lb ? la : lb, hi = la < lb ? lb : la;
+            uint32_t     off = 36 + 9 * (lo == 0 ? (hi == 1 ? 0 : 1) : 2);
+            if (la < lb)
+                for (int k = 0; k < 9; k++) H.m[k] = H.m[k] + S[off + k];
+            else
+                for (int r = 0; r < 3; r++)
+                    for (int cc = 0; cc < 3; cc++) H.m[3 * r + cc] = H.m[3 * r + cc] + S[off + 3 * cc + r];
+        }
+    }
+    st_planes(val, nblocks, b, H);
+}
+
+// ---------------------------------------------------------------- narrow phase
+// Contact record (the reference's CollisionPairTemplate, collision_data.h:48-127):
+// idx (type bit 31 in idx.z for VF, idx.w for EE), (n, k2), weights, (k1, area*kappa, bary0, bary1).
+LCSX_D void push_contact(const ContactBuf& cb, bool emit, uint4 idx, f3 n, float k1, float k2, float ak, float b0,
+                         float b1, float4 w)
+{
+    uint32_t slot = wave_append(cb.count, emit);
+    if (emit && slot < cb.capacity)
+    {
+        cb.idx[slot] = idx;
+        cb.nk[slot]  = make_float4(n.x, n.y, n.z, k2);
+        cb.w[slot]   = w;
+        cb.aux[slot] = make_float4(k1, ak, b0, b1);
+    }
+}
+
+// fn_narrow_phase_vf_dcd_query, narrow_phase.cpp:771-933 (barrier energy)
+__global__ __launch_bounds__(kB) void k_dcd_vf(DcdArgs a)
+{
+    uint32_t n = min(*a.pair_count, a.pair_capacity);
+    for (uint32_t p = blockIdx.x * kB + threadIdx.x; p < n; p += gridDim.x * kB)
+    {
+        uint32_t vid = a.pairs[2 * (size_t)p], fid = a.pairs[2 * (size_t)p + 1];
+        uint32_t f0 = a.faces[3 * fid], f1 = a.faces[3 * fid + 1], f2 = a.faces[3 * fid + 2];
+        bool     emit = false;
+        uint4    idx  = make_uint4(vid, f0, f1 | 0x80000000u, f2);
+        f3       nrm  = mk(0.0f, 0.0f, 0.0f);
+        float    k1 = 0.0f, k2 = 0.0f, ak = 0.0f;
+        f3       b  = mk(0.0f, 0.0f, 0.0f);
+        if (!(vid == f0 || vid == f1 || vid == f2))
+        {
+            f3 P = ld3(a.x, vid), t0 = ld3(a.x, f0), t1 = ld3(a.x, f1), t2 = ld3(a.x, f2);
+            b         = pt_coeff_unclassified(P, t0, t1, t2);
+            f3    X   = (b.x * (P - t0) + b.y * (P - t1)) + b.z * (P - t2);
+            float d2  = dot(X, X);
+            float dh  = (a.dhat[vid] + a.dhat[f0]) * 0.5f;
+            float th  = a.offset[vid] + a.offset[f0];
+            if (d2 < (th + dh) * (th + dh))
+            {
+                float d  = sqrtf(d2);
+                nrm      = X / d;
+                float aa = 0.5f * (a.area_a[vid] + a.area_b[fid]);
+                k1       = ((-aa) * a.kappa) * barrier_d1(d - th, dh);
+                k2       = (aa * a.kappa) * barrier_d2(d - th, dh);
+                ak       = aa * a.kappa;
+                emit     = true;
+            }
+        }
+        push_contact(a.out, emit, idx, nrm, k1, k2, ak, b.x, b.y, make_float4(1.0f, -b.x, -b.y, (b.x + b.y) - 1.0f));
+    }
+}
+
+// fn_narrow_phase_ee_dcd_query, narrow_phase.cpp:935-1098
+__global__ __launch_bounds__(kB) void k_dcd_ee(DcdArgs a)
+{
+    uint32_t n = min(*a.pair_count, a.pair_capacity);
+    for (uint32_t p = blockIdx.x * kB + threadIdx.x; p < n; p += gridDim.x * kB)
+    {
+        uint32_t l = a.pairs[2 * (size_t)p], r = a.pairs[2 * (size_t)p + 1];
+        uint32_t a0 = a.edges[2 * l], a1 = a.edges[2 * l + 1], b0 = a.edges[2 * r], b1 = a.edges[2 * r + 1];
+        bool     emit = false;
+        uint4    idx  = make_uint4(a0, a1, b0, b1 | 0x80000000u);
+        f3       nrm  = mk(0.0f, 0.0f, 0.0f);
+        float    k1 = 0.0f, k2 = 0.0f, ak = 0.0f;
+        c4       c  = {{0.0f, 0.0f, 0.0f, 0.0f}};
+        if (!(a0 == b0 || a0 == b1 || a1 == b0 || a1 == b1))
+        {
+            f3 pa0 = ld3(a.x, a0), pa1 = ld3(a.x, a1), pb0 = ld3(a.x, b0), pb1 = ld3(a.x, b1);
+            c          = ee_coeff_unclassified(pa0, pa1, pb0, pb1);
+            f3    X    = (c.v[0] * pa0 + c.v[1] * pa1) - (c.v[2] * pb0 + c.v[3] * pb1);
+            float d2   = dot(X, X);
+            float dh   = (a.dhat[a0] + a.dhat[b0]) * 0.5f;
+            float th   = a.offset[a0] + a.offset[b0];
+            if (d2 < (th + dh) * (th + dh))
+            {
+                float d  = sqrtf(d2);
+                nrm      = X / d;
+                float aa = 0.5f * (a.area_a[l] + a.area_b[r]);
+                k1       = ((-aa) * a.kappa) * barrier_d1(d - th, dh);
+                k2       = (aa * a.kappa) * barrier_d2(d - th, dh);
+                ak       = aa * a.kappa;
+                emit     = true;
+            }
+        }
+        push_contact(a.out, emit, idx, nrm, k1, k2, ak, c.v[0], c.v[2],
+                     make_float4(c.v[0], 1.0f - c.v[0], -c.v[2], c.v[2] - 1.0f));
+    }
+}
+
+// fn_perPair_assemble_gradient_hessian, narrow_phase.cpp:2550-2676 (soft bodies)
+__global__ __launch_bounds__(kB) void k_contact_assemble(ContactView c, float* __restrict__ cgB, float* __restrict__ diag,
+                                                          uint32_t nv)
+{
+    uint32_t n = min(*c.count, c.capacity);
+    for (uint32_t k = blockIdx.x * kB + threadIdx.x; k < n; k += gridDim.x * kB)
+    {
+        uint4  id = c.idx[k];
+        float4 nk = c.nk[k], w4 = c.w[k], ax = c.aux[k];
+        f3     nn  = mk(nk.x, nk.y, nk.z);
+        float  nv3[3] = {nk.x, nk.y, nk.z};
+        uint32_t ids[4] = {id.x & 0x7fffffffu, id.y & 0x7fffffffu, id.z & 0x7fffffffu, id.w & 0x7fffffffu};
+        float    w[4]   = {w4.x, w4.y, w4.z, w4.w};
+        for (int j = 0; j < 4; j++)
+        {
+            f3 g = (ax.x * w[j]) * nn;
+            atomicAdd(&cgB[3 * (size_t)ids[j]], g.x);
+            atomicAdd(&cgB[3 * (size_t)ids[j] + 1], g.y);
+            atomicAdd(&cgB[3 * (size_t)ids[j] + 2], g.z);
+            float s = (nk.w * w[j]) * w[j];
+            for (int r = 0; r < 3; r++)
+                for (int cc = 0; cc < 3; cc++) atomicAdd(&diag[(3 * r + cc) * (size_t)nv + ids[j]], s * (nv3[r] * nv3[cc]));
+        }
+    }
+}
+
+// ---------------------------------------------------------------- PCG
+// precond_cg.cpp:118-331, 626-771. State scalars live on the device; kernels early-exit once
+// the solve has converged, so the host can enqueue iterations in chunks without reading back.
+__global__ __launch_bounds__(kB) void k_pcg_init(PcgArgs a)
+{
+    __shared__ float sh[4];
+    uint32_t         i   = blockIdx.x * kB + threadIdx.x;
+    float            loc = 0.0f;
+    if (i < a.n)
+    {
+        m3 D = ld_planes(a.diag, a.n, i);
+        // 3x3 inverse by cofactors
+        const float* m   = D.m;
+        float        c00 = m[4] * m[8] - m[5] * m[7];
+        float        c01 = m[5] * m[6] - m[3] * m[8];
+        float        c02 = m[3] * m[7] - m[4] * m[6];
+        float        det = m[0] * c00 + m[1] * c01 + m[2] * c02;
+        float        inv = 1.0f / det;
+        m3           Mi;
+        Mi.m[0] = c00 * inv;
+        Mi.m[1] = (m[2] * m[7] - m[1] * m[8]) * inv;
+        Mi.m[2] = (m[1] * m[5] - m[2] * m[4]) * inv;
+        Mi.m[3] = c01 * inv;
+        Mi.m[4] = (m[0] * m[8] - m[2] * m[6]) * inv;
+        Mi.m[5] = (m[2] * m[3] - m[0] * m[5]) * inv;
+        Mi.m[6] = c02 * inv;
+        Mi.m[7] = (m[1] * m[6] - m[0] * m[7]) * inv;
+        Mi.m[8] = (m[0] * m[4] - m[1] * m[3]) * inv;
+        st_planes(a.minv, a.n, i, Mi);
+        f3 r = ld3(a.b, i);
+        st3(a.r, i, r);
+        f3 z = mul(Mi, r);
+        st3(a.z, i, z);
+        st3(a.x, i, mk(0.0f, 0.0f, 0.0f));
+        st3(a.p_old, i, mk(0.0f, 0.0f, 0.0f));
+        loc = dot(r, z);
+    }
+    float tot;
+    if (last_block_sum(loc, a.partials, &a.s->ticket, &tot, sh) && threadIdx.x == 0)
+    {
+        a.s->rz     = tot;
+        a.s->rz0    = tot;
+        a.s->beta   = 0.0f;  // rz_old = 0 < 1e-10
+        a.s->iters  = 0;
+        a.s->converged = (!(tot == tot) || tot < a.abs_tol || tot <= a.rel2 * tot) ? 1u : 0u;
+        a.s->nan = (tot == tot) ? 0u : 1u;
+    }
+}
+
+// q = A p_new with p_new = z + beta p_old formed on the fly for every gathered column.
+__global__ __launch_bounds__(kB) void k_spmv_pcg(PcgArgs a, SellView A)
+{
+    if (a.s->converged) return;
+    uint32_t i = blockIdx.x * kB + threadIdx.x;
+    if (i >= a.n) return;
+    float beta = a.s->beta;
+    f3    pi   = ld3(a.z, i) + beta * ld3(a.p_old, i);
+    st3(a.p_new, i, pi);
+    f3       y  = mul(ld_planes(A.diag, a.n, i), pi);
+    uint32_t s  = i >> 6, l = i & 63u;
+    uint32_t k0 = A.sl_off[s], k1 = A.sl_off[s + 1];
+    for (uint32_t k = k0; k < k1; k++)
+    {
+        size_t   b  = (size_t)k * 64 + l;
+        uint32_t c  = A.col[b];
+        f3       pc = ld3(a.z, c) + beta * ld3(a.p_old, c);
+        m3       M;
+#pragma unroll
+        for (int cc = 0; cc < 9; cc++) M.m[cc] = A.val[cc * (size_t)A.nblocks + b];
+        y = y + mul(M, pc);
+    }
+    st3(a.q, i, y);
+}
+
+// y = D p + sum_k A_k p[col_k] (material part only), one thread per row.
+__global__ __launch_bounds__(kB) void k_spmv_plain(uint32_t n, SellView A, const float* __restrict__ p, float* __restrict__ y)
+{
+    uint32_t i = blockIdx.x * kB + threadIdx.x;
+    if (i >= n) return;
+    f3       acc = mul(ld_planes(A.diag, n, i), ld3(p, i));
+    uint32_t s = i >> 6, l = i & 63u;
+    for (uint32_t k = A.sl_off[s]; k < A.sl_off[s + 1]; k++)
+    {
+        size_t b = (size_t)k * 64 + l;
+        m3     M;
+#pragma unroll
+        for (int cc = 0; cc < 9; cc++) M.m[cc] = A.val[cc * (size_t)A.nblocks + b];
+        acc = acc + mul(M, ld3(p, A.col[b]));
+    }
+    st3(y, i, acc);
+}
+
+// Contact part of q = A p (narrow_phase.cpp:2730-2775, per-pair form): for each contact
+// q_j += k2 w_j n (sum_{jj != j} w_jj n.p_jj).
+__global__ __launch_bounds__(kB) void k_contact_spmv(ContactView c, const float* __restrict__ p, float* __restrict__ q,
+                                                      const uint32_t* __restrict__ converged)
+{
+    if (converged && *converged) return;
+    uint32_t n = min(*c.count, c.capacity);
+    for (uint32_t k = blockIdx.x * kB + threadIdx.x; k < n; k += gridDim.x * kB)
+    {
+        uint4    id     = c.idx[k];
+        float4   nk     = c.nk[k], w4 = c.w[k];
+        uint32_t ids[4] = {id.x & 0x7fffffffu, id.y & 0x7fffffffu, id.z & 0x7fffffffu, id.w & 0x7fffffffu};
+        float    w[4]   = {w4.x, w4.y, w4.z, w4.w};
+        f3       nn     = mk(nk.x, nk.y, nk.z);
+        float    t[4];
+        for (int j = 0; j < 4; j++) t[j] = w[j] * dot(nn, ld3(p, ids[j]));
+        for (int j = 0; j < 4; j++)
+        {
+            float s = 0.0f;
+            for (int jj = 0; jj < 4; jj++)
+                if (jj != j) s = s + t[jj];
+            f3 o = ((nk.w * w[j]) * s) * nn;
+            atomicAdd(&q[3 * (size_t)ids[j]], o.x);
+            atomicAdd(&q[3 * (size_t)ids[j] + 1], o.y);
+            atomicAdd(&q[3 * (size_t)ids[j] + 2], o.z);
+        }
+    }
+}
+
+__global__ __launch_bounds__(kB) void k_pcg_alpha(PcgArgs a)
+{
+    __shared__ float sh[4];
+    if (a.s->converged) return;
+    uint32_t i   = blockIdx.x * kB + threadIdx.x;
+    float    loc = i < a.n ? dot(ld3(a.p_new, i), ld3(a.q, i)) : 0.0f;
+    float    tot;
+    if (last_block_sum(loc, a.partials, &a.s->ticket, &tot, sh) && threadIdx.x == 0)
+        a.s->alpha = tot < 1e-10f ? 0.0f : a.s->rz / tot;
+}
+
+__global__ __launch_bounds__(kB) void k_pcg_update(PcgArgs a)
+{
+    __shared__ float sh[4];
+    if (a.s->converged) return;
+    uint32_t i   = blockIdx.x * kB + threadIdx.x;
+    float    loc = 0.0f;
+    if (i < a.n)
+    {
+        float alpha = a.s->alpha;
+        f3    pn    = ld3(a.p_new, i);
+        st3(a.x, i, ld3(a.x, i) + alpha * pn);
+        f3 r = ld3(a.r, i) - alpha * ld3(a.q, i);
+        st3(a.r, i, r);
+        f3 z = mul(ld_planes(a.minv, a.n, i), r);
+        st3(a.z, i, z);
+        loc = dot(r, z);
+    }
+    float tot;
+    if (last_block_sum(loc, a.partials, &a.s->ticket, &tot, sh) && threadIdx.x == 0)
+    {
+        float rz_old = a.s->rz;
+        a.s->beta    = rz_old < 1e-10f ? 0.0f : tot / rz_old;
+        a.s->rz      = tot;
+        a.s->iters   = a.s->iters + 1;
+        bool check   = (a.s->iters % a.check_every) == 0;
+        if (!(tot == tot)) a.s->nan = 1u;
+        if (!(tot == tot) || (check && (tot < a.abs_tol || tot <= a.rel2 * a.s->rz0))) a.s->converged = 1u;
+    }
+}
+
+// ---------------------------------------------------------------- line search
+__global__ void k_apply_dx(uint32_t nv, const float* __restrict__ x_iter, const float* __restrict__ dx, float* __restrict__ x,
+                           const uint32_t* __restrict__ alpha_bits, float alpha_const)
+{
+    uint32_t i = blockIdx.x * kB + threadIdx.x;
+    if (i >= nv) return;
+    float alpha = alpha_bits ? ord2f(*alpha_bits) : alpha_const;
+    st3(x, i, ld3(x_iter, i) + alpha * ld3(dx, i));
+}
+
+// fn_gound_collision_ccd, newton_solver.cpp:952-1000 (+ NaN check of dx and max |dx|)
+__global__ __launch_bounds__(kB) void k_ground_ccd_maxdx(LsArgs a)
+{
+    __shared__ float sh[4][2];
+    uint32_t         i = blockIdx.x * kB + threadIdx.x;
+    float            toi = 1.0f, mdx = 0.0f;
+    if (i < a.nv)
+    {
+        f3 d = ld3(a.dx, i);
+        mdx  = length(d);
+        if (!(mdx == mdx)) atomicOr(&a.st->nan, 1u);
+        if (a.use_floor && !a.fixed[i])
+        {
+            float off = a.offset[i], cy = a.x[3 * (size_t)i + 1];
+            if (cy - off < a.floor_y)
+            {
+                float iy = a.x_iter[3 * (size_t)i + 1];
+                toi      = ((iy - off) - a.floor_y) / (iy - cy) / kLineSearchMaxT;
+            }
+        }
+    }
+    toi   = wave_min(toi);
+    mdx   = wave_max(mdx);
+    int w = threadIdx.x >> 6;
+    if (lane_id() == 0)
+    {
+        sh[w][0] = toi;
+        sh[w][1] = mdx;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        float t = fminf(fminf(sh[0][0], sh[1][0]), fminf(sh[2][0], sh[3][0]));
+        float m = fmaxf(fmaxf(sh[0][1], sh[1][1]), fmaxf(sh[2][1], sh[3][1]));
+        if (t != 1.0f) atomicMin(&a.st->toi_bits, f2ord(t));
+        atomicMax(&a.st->maxdx_bits, __float_as_uint(m));
+    }
+}
+
+// fn_narrow_phase_{vf,ee}_ccd_query, narrow_phase.cpp:293-593: min over pairs of toi / 1.25.
+__global__ __launch_bounds__(kB) void k_ccd(CcdArgs a)
+{
+    __shared__ float sh[4];
+    uint32_t         n   = min(*a.pair_count, a.pair_capacity);
+    float            tmin = kLineSearchMaxT;
+    for (uint32_t p = blockIdx.x * kB + threadIdx.x; p < n; p += gridDim.x * kB)
+    {
+        uint32_t q = a.pairs[2 * (size_t)p], o = a.pairs[2 * (size_t)p + 1];
+        uint32_t v[4];
+        if (a.kind == 0)
+        {
+            v[0] = q;
+            v[1] = a.faces[3 * o];
+            v[2] = a.faces[3 * o + 1];
+            v[3] = a.faces[3 * o + 2];
+            if (v[0] == v[1] || v[0] == v[2] || v[0] == v[3]) continue;
+        }
+        else
+        {
+            v[0] = a.edges[2 * q];
+            v[1] = a.edges[2 * q + 1];
+            v[2] = a.edges[2 * o];
+            v[3] = a.edges[2 * o + 1];
+            if (v[0] == v[2] || v[0] == v[3] || v[1] == v[2] || v[1] == v[3]) continue;
+        }
+        f3 x0[4], x1[4];
+        for (int k = 0; k < 4; k++)
+        {
+            x0[k] = ld3(a.x0, v[k]);
+            x1[k] = ld3(a.x1, v[k]);
+        }
+        float off = a.offset[v[0]] + a.offset[a.kind == 0 ? v[1] : v[2]];
+        tmin      = fminf(tmin, accd(x0, x1, off, a.kind));
+    }
+    tmin  = wave_min(tmin);
+    int w = threadIdx.x >> 6;
+    if (lane_id() == 0) sh[w] = tmin;
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        float t = fminf(fminf(sh[0], sh[1]), fminf(sh[2], sh[3]));
+        if (t != kLineSearchMaxT) atomicMin(a.toi_bits, f2ord(t / kLineSearchMaxT));
+    }
+}
+
+// max |x - x_tilde| over pinned vertices (line_search, newton_solver.cpp:3860-3882)
+__global__ void k_pin_delta(uint32_t nfix, const uint32_t* __restrict__ ids, const float* __restrict__ x,
+                            const float* __restrict__ x_tilde, uint32_t* __restrict__ bits)
+{
+    uint32_t k = blockIdx.x * kB + threadIdx.x;
+    float    d = 0.0f;
+    if (k < nfix)
+    {
+        uint32_t i = ids[k];
+        d          = length(ld3(x, i) - ld3(x_tilde, i));
+    }
+    d = wave_max(d);
+    if (lane_id() == 0) atomicMax(bits, __float_as_uint(d));
+}
+
+__global__ void k_reset_ls(Status* st)
+{
+    st->n_vf_ccd = st->n_ee_ccd = 0;
+    st->toi_bits   = f2ord(1.0f);
+    st->maxdx_bits = 0;
+    st->pin_bits   = 0;
+}
+
+__global__ void k_reset_status(Status* st)
+{
+    st->n_vf = st->n_ee = st->n_contacts = st->overflow = 0;
+    st->n_vf_ccd = st->n_ee_ccd = 0;
+    st->toi_bits = f2ord(1.0f);
+    st->maxdx_bits = 0;
+    st->pin_bits   = 0;
+    st->nan        = 0;
+}
+}  // namespace
+
+// ---------------------------------------------------------------- launchers
+void launch_pin_velocity(uint32_t nfix, const uint32_t* ids, const float* targets, const float* x, float* v, float dt, hipStream_t s)
+{
+    if (nfix) k_pin_velocity<<<grid_for(nfix, kB), kB, 0, s>>>(nfix, ids, targets, x, v, dt);
+}
+void launch_predict(uint32_t nv, const float* x_step, const float* v, const uint8_t* fixed, float h, const float g[3],
+                    float* x_iter, float* x_tilde, float* x, hipStream_t s)
+{
+    k_predict<<<grid_for(nv, kB), kB, 0, s>>>(nv, x_step, v, fixed, h, mk(g[0], g[1], g[2]), x_iter, x_tilde, x);
+}
+void launch_frame_end(uint32_t nv, float* x_step, const float* x, float* v, float h, float damp, hipStream_t s)
+{
+    k_frame_end<<<grid_for(nv, kB), kB, 0, s>>>(nv, x_step, x, v, h, damp);
+}
+void launch_face_stage(uint32_t nsf, const uint32_t* sf, const float4* Dm, const float4* c, const float* x, float* stage,
+                       hipStream_t s)
+{
+    if (nsf) k_face_stage<<<grid_for(nsf, kB), kB, 0, s>>>(nsf, sf, Dm, c, x, stage);
+}
+void launch_hinge_stage(uint32_t nbh, const uint32_t* bh, const float4* c, float scaling, const float* x, float* stage,
+                        hipStream_t s)
+{
+    if (nbh) k_hinge_stage<<<grid_for(nbh, kB), kB, 0, s>>>(nbh, bh, c, scaling, x, stage);
+}
+void launch_vert_assemble(const VertArgs& a, hipStream_t s) { k_vert_assemble<<<grid_for(a.nv, kB), kB, 0, s>>>(a); }
+void launch_offdiag(uint32_t nblocks, const uint32_t* ptr, const uint32_t* con, const float* fs, const float* hs, float* val,
+                    hipStream_t s)
+{
+    if (nblocks) k_offdiag<<<grid_for(nblocks, kB), kB, 0, s>>>(nblocks, ptr, con, fs, hs, val);
+}
+void launch_dcd(const DcdArgs& a, int kind, unsigned grid, hipStream_t s)
+{
+    if (kind == 0) k_dcd_vf<<<grid, kB, 0, s>>>(a);
+    else k_dcd_ee<<<grid, kB, 0, s>>>(a);
+}
+void launch_contact_assemble(const ContactView& c, float* cgB, float* diag, uint32_t nv, unsigned grid, hipStream_t s)
+{
+    k_contact_assemble<<<grid, kB, 0, s>>>(c, cgB, diag, nv);
+}
+void launch_pcg_init(const PcgArgs& a, hipStream_t s) { k_pcg_init<<<grid_for(a.n, kB), kB, 0, s>>>(a); }
+void launch_pcg_spmv(const PcgArgs& a, const SellView& A, hipStream_t s)
+{
+    k_spmv_pcg<<<grid_for(a.n, kB), kB, 0, s>>>(a, A);
+}
+void launch_pcg_contact_spmv(const PcgArgs& a, const ContactView& c, unsigned contact_grid, hipStream_t s)
+{
+    k_contact_spmv<<<contact_grid, kB, 0, s>>>(c, a.p_new, a.q, &a.s->converged);
+}
+void launch_pcg_finish(const PcgArgs& a, hipStream_t s)
+{
+    unsigned g = grid_for(a.n, kB);
+    k_pcg_alpha<<<g, kB, 0, s>>>(a);
+    k_pcg_update<<<g, kB, 0, s>>>(a);
+}
+void launch_spmv(const SellView& A, const ContactView* c, unsigned contact_grid, const float* p, float* y, uint32_t n,
+                 hipStream_t s)
+{
+    k_spmv_plain<<<grid_for(n, kB), kB, 0, s>>>(n, A, p, y);
+    if (c) k_contact_spmv<<<contact_grid, kB, 0, s>>>(*c, p, y, nullptr);
+}
+void launch_apply_dx(uint32_t nv, const float* x_iter, const float* dx, float* x, const uint32_t* alpha_bits,
+                     float alpha_const, hipStream_t s)
+{
+    k_apply_dx<<<grid_for(nv, kB), kB, 0, s>>>(nv, x_iter, dx, x, alpha_bits, alpha_const);
+}
+void launch_ground_ccd_maxdx(const LsArgs& a, hipStream_t s) { k_ground_ccd_maxdx<<<grid_for(a.nv, kB), kB, 0, s>>>(a); }
+void launch_ccd(const CcdArgs& a, unsigned grid, hipStream_t s) { k_ccd<<<grid, kB, 0, s>>>(a); }
+void launch_pin_delta(uint32_t nfix, const uint32_t* ids, const float* x, const float* x_tilde, uint32_t* bits, hipStream_t s)
+{
+    if (nfix) k_pin_delta<<<grid_for(nfix, kB), kB, 0, s>>>(nfix, ids, x, x_tilde, bits);
+}
+void launch_reset_status(Status* st, hipStream_t s) { k_reset_status<<<1, 1, 0, s>>>(st); }
+void launch_reset_line_search(Status* st, hipStream_t s) { k_reset_ls<<<1, 1, 0, s>>>(st); }
+
+}  // namespace lcsx

@@ -1,0 +1,382 @@
+// Host-side scene setup. Reference counterparts:
+//   edges / hinges           Solver/MeshOperation/mesh_reader.cpp:219-352
+//   pins                     Solver/Initializer/init_mesh_data.cpp:59-95, 117-240
+//   areas / mass / thickness Solver/Initializer/init_mesh_data.cpp:783-995
+//   offset / d_hat           Solver/Initializer/init_collision_data.cpp:71-174
+//   elements / CSR / offsets Solver/Initializer/init_sim_data.cpp:19-48, 189-480, 760-896
+#include "lcsx_setup.h"
+#include <algorithm>
+#include <cmath>
+#include <map>
+#include <numeric>
+#include "lcsx_physics.h"
+
+namespace lcsx
+{
+
+namespace
+{
+struct EdgeKey
+{
+    uint32_t a, b, f;
+};
+
+// Unique sorted edges and hinges (e0 < e1, opposite vertex of the lower then higher face).
+void topology(const std::vector<int32_t>& tris, std::vector<uint32_t>& E, std::vector<uint32_t>& H)
+{
+    size_t               nt = tris.size() / 3;
+    std::vector<EdgeKey> k(3 * nt);
+    for (size_t f = 0; f < nt; f++)
+    {
+        uint32_t v[3] = {(uint32_t)tris[3 * f], (uint32_t)tris[3 * f + 1], (uint32_t)tris[3 * f + 2]};
+        std::sort(v, v + 3);
+        k[3 * f]     = {v[0], v[1], (uint32_t)f};
+        k[3 * f + 1] = {v[0], v[2], (uint32_t)f};
+        k[3 * f + 2] = {v[1], v[2], (uint32_t)f};
+    }
+    std::sort(k.begin(), k.end(), [](const EdgeKey& l, const EdgeKey& r) {
+        return l.a != r.a ? l.a < r.a : (l.b != r.b ? l.b < r.b : l.f < r.f);
+    });
+    auto same = [&](size_t i, size_t j) { return k[i].a == k[j].a && k[i].b == k[j].b; };
+    E.clear();
+    H.clear();
+    std::vector<uint8_t> first_of_pair(k.size(), 0);
+    for (size_t i = 0; i < k.size(); i++)
+    {
+        bool has_next = i + 1 < k.size() && same(i, i + 1);
+        bool has_prev = i > 0 && same(i, i - 1);
+        if (has_prev || !has_next)
+        {
+            E.push_back(k[i].a);
+            E.push_back(k[i].b);
+        }
+        if (has_next && !has_prev) first_of_pair[i] = 1;
+        if (has_next && has_prev) first_of_pair[i] = 0;  // middle of a non-manifold fan
+    }
+    for (size_t i = 0; i < k.size(); i++)
+    {
+        if (!first_of_pair[i]) continue;
+        const int32_t* c = &tris[3 * k[i].f];
+        const int32_t* d = &tris[3 * k[i + 1].f];
+        uint32_t       s = k[i].a + k[i].b;
+        H.push_back(k[i].a);
+        H.push_back(k[i].b);
+        H.push_back((uint32_t)(c[0] + c[1] + c[2]) - s);
+        H.push_back((uint32_t)(d[0] + d[1] + d[2]) - s);
+    }
+}
+
+f3 at(const std::vector<float>& x, uint32_t i) { return mk(x[3 * i], x[3 * i + 1], x[3 * i + 2]); }
+}  // namespace
+
+void pin_by_method(MeshDesc& m, int method, float r, const lcsx_fixed_anim& anim)
+{
+    size_t n = m.x.size() / 3;
+    float  lo[3] = {3.4e38f, 3.4e38f, 3.4e38f}, hi[3] = {-3.4e38f, -3.4e38f, -3.4e38f};
+    for (size_t i = 0; i < n; i++)
+        for (int d = 0; d < 3; d++)
+        {
+            lo[d] = std::min(lo[d], m.x[3 * i + d]);
+            hi[d] = std::max(hi[d], m.x[3 * i + d]);
+        }
+    float inv[3];
+    for (int d = 0; d < 3; d++) inv[d] = 1.0f / std::max(hi[d] - lo[d], 0.0001f);
+    for (size_t i = 0; i < n; i++)
+    {
+        float x = (m.x[3 * i] - lo[0]) * inv[0], y = (m.x[3 * i + 1] - lo[1]) * inv[1], z = (m.x[3 * i + 2] - lo[2]) * inv[2];
+        bool  L = x < r, R = x > 1.0f - r, F = z < r, B = z > 1.0f - r, U = y > 1.0f - r, D = y < r;
+        bool  sel = false;
+        switch (method)
+        {
+            case 3: sel = L; break;
+            case 4: sel = R; break;
+            case 5: sel = F; break;
+            case 6: sel = B; break;
+            case 7: sel = U; break;
+            case 8: sel = D; break;
+            case 9: sel = L && U; break;
+            case 10: sel = L && D; break;
+            case 11: sel = L && B; break;  // LeftFront: x < r, z > 1 - r (init_mesh_data.cpp:180)
+            case 12: sel = L && F; break;  // LeftBack:  x < r, z < r
+            case 13: sel = R && U; break;
+            case 14: sel = R && D; break;
+            case 15: sel = R && B; break;  // RightFront
+            case 16: sel = R && F; break;  // RightBack
+            case 17: sel = F && U; break;
+            case 18: sel = F && D; break;
+            case 19: sel = B && U; break;
+            case 20: sel = B && D; break;
+            case 21: sel = true; break;
+            default: sel = false;
+        }
+        if (sel)
+        {
+            m.pinned.push_back((uint32_t)i);
+            m.anim.push_back(anim);
+        }
+    }
+}
+
+void animate_target(const lcsx_fixed_anim& a, float t, const float in[3], float out[3])
+{
+    f3 p = mk(in[0], in[1], in[2]);
+    if (a.use_scale) p = mk(a.scale[0] * t * p.x, a.scale[1] * t * p.y, a.scale[2] * t * p.z);
+    if (a.use_rotate)
+    {
+        float ang = t * a.rot_deg_per_s / 180.0f * 3.14159265358979323846f;
+        f3    c   = mk(a.rot_center[0], a.rot_center[1], a.rot_center[2]);
+        f3    rel = p - c;
+        if (ang != 0.0f)
+        {
+            f3    ax = normalize(mk(a.rot_axis[0], a.rot_axis[1], a.rot_axis[2]));
+            float co = std::cos(ang), si = std::sin(ang);
+            f3    tt = (1.0f - co) * ax;
+            f3    c0 = mk(co + tt.x * ax.x, tt.x * ax.y + si * ax.z, tt.x * ax.z - si * ax.y);
+            f3    c1 = mk(tt.y * ax.x - si * ax.z, co + tt.y * ax.y, tt.y * ax.z + si * ax.x);
+            f3    c2 = mk(tt.z * ax.x + si * ax.y, tt.z * ax.y - si * ax.x, co + tt.z * ax.z);
+            rel      = (c0 * rel.x + c1 * rel.y) + c2 * rel.z;
+        }
+        p = c + rel;
+    }
+    if (a.use_translate) p = p + mk(a.translate[0] * t, a.translate[1] * t, a.translate[2] * t);
+    out[0] = p.x;
+    out[1] = p.y;
+    out[2] = p.z;
+}
+
+void Setup::build(const std::vector<MeshDesc>& meshes)
+{
+    *this = Setup();
+    for (size_t mi = 0; mi < meshes.size(); mi++)
+    {
+        const MeshDesc&       m  = meshes[mi];
+        uint32_t              bv = nv;
+        std::vector<uint32_t> E, H;
+        topology(m.tris, E, H);
+        uint32_t mv = (uint32_t)(m.x.size() / 3);
+        rest_x.insert(rest_x.end(), m.x.begin(), m.x.end());
+        vert_mesh.insert(vert_mesh.end(), mv, (uint32_t)mi);
+        is_fixed.insert(is_fixed.end(), mv, 0);
+        for (int32_t t : m.tris) faces.push_back(bv + (uint32_t)t);
+        face_mesh.insert(face_mesh.end(), m.tris.size() / 3, (uint32_t)mi);
+        for (uint32_t e : E) edges.push_back(bv + e);
+        edge_mesh.insert(edge_mesh.end(), E.size() / 2, (uint32_t)mi);
+        for (uint32_t h : H) hinges.push_back(bv + h);
+        for (size_t k = 0; k < m.pinned.size(); k++)
+        {
+            is_fixed[bv + m.pinned[k]] = 1;
+            fixed_verts.push_back(bv + m.pinned[k]);
+            fixed_mesh.push_back((uint32_t)mi);
+            fixed_local.push_back((uint32_t)k);
+            fixed_anim.push_back(m.anim[k]);
+        }
+        nv += mv;
+    }
+    nf = (uint32_t)(faces.size() / 3);
+    ne = (uint32_t)(edges.size() / 2);
+    nh = (uint32_t)(hinges.size() / 4);
+
+    // areas: faces in float, vertices and edges accumulate face/3 in double
+    face_area.resize(nf);
+    std::vector<std::vector<uint32_t>> v_faces(nv);
+    for (uint32_t f = 0; f < nf; f++)
+    {
+        const uint32_t* t = &faces[3 * f];
+        face_area[f]       = length(cross(at(rest_x, t[1]) - at(rest_x, t[0]), at(rest_x, t[2]) - at(rest_x, t[0]))) * 0.5f;
+        for (int j = 0; j < 3; j++) v_faces[t[j]].push_back(f);
+    }
+    {
+        std::map<uint64_t, uint32_t> eid;
+        for (uint32_t e = 0; e < ne; e++) eid[((uint64_t)edges[2 * e] << 32) | edges[2 * e + 1]] = e;
+        std::vector<double> acc(ne, 0.0);
+        for (uint32_t f = 0; f < nf; f++)
+            for (int j = 0; j < 3; j++)
+            {
+                uint32_t a = faces[3 * f + j], b = faces[3 * f + (j + 1) % 3];
+                auto     it = eid.find(((uint64_t)std::min(a, b) << 32) | std::max(a, b));
+                if (it != eid.end()) acc[it->second] += face_area[f] / 3.0;
+            }
+        edge_area.resize(ne);
+        for (uint32_t e = 0; e < ne; e++) edge_area[e] = (float)acc[e];
+    }
+    vert_area.resize(nv);
+    std::vector<float> vvol(nv);
+    for (uint32_t i = 0; i < nv; i++)
+    {
+        double a = 0.0;
+        for (uint32_t f : v_faces[i]) a += face_area[f] / 3.0;
+        vert_area[i] = (float)a;
+        vvol[i]      = vert_area[i] * meshes[vert_mesh[i]].mat.thickness;
+    }
+    std::vector<float> body_vol(meshes.size(), 0.0f), body_mass(meshes.size(), 0.0f);
+    for (uint32_t f = 0; f < nf; f++) body_vol[face_mesh[f]] += face_area[f] * meshes[face_mesh[f]].mat.thickness;
+    for (size_t mi = 0; mi < meshes.size(); mi++)
+        body_mass[mi] = meshes[mi].mat.mass != 0.0f ? meshes[mi].mat.mass : body_vol[mi] * meshes[mi].mat.density;
+    vert_mass.resize(nv);
+    for (uint32_t i = 0; i < nv; i++) vert_mass[i] = (vvol[i] / body_vol[vert_mesh[i]]) * body_mass[vert_mesh[i]];
+
+    // contact offset and d_hat, clamped to 0.9 of the smallest rest edge length / altitude
+    std::vector<float> min_dist(meshes.size(), 1e10f);
+    for (uint32_t e = 0; e < ne; e++)
+        min_dist[edge_mesh[e]] = std::min(min_dist[edge_mesh[e]], length(at(rest_x, edges[2 * e]) - at(rest_x, edges[2 * e + 1])));
+    for (uint32_t f = 0; f < nf; f++)
+    {
+        float rd = 10000.0f;
+        for (int i = 0; i < 3; i++)
+        {
+            f3    v0 = at(rest_x, faces[3 * f + i]), v1 = at(rest_x, faces[3 * f + (i + 1) % 3]), v2 = at(rest_x, faces[3 * f + (i + 2) % 3]);
+            float c0, c1;
+            pe_coeff(v0, v1, v2, c0, c1);
+            rd = std::min(rd, length((c0 * v1 + c1 * v2) - v0));
+        }
+        min_dist[face_mesh[f]] = std::min(min_dist[face_mesh[f]], rd);
+    }
+    offset.resize(nv);
+    dhat.resize(nv);
+    for (uint32_t i = 0; i < nv; i++)
+    {
+        const lcsx_material& mt = meshes[vert_mesh[i]].mat;
+        float                so = 0.5f * mt.thickness, sd = mt.d_hat, safe = 0.9f * min_dist[vert_mesh[i]];
+        if (!(so + mt.d_hat < safe))
+        {
+            if (so > 0.5f * safe) so = 0.5f * safe;
+            if (sd > safe - so) sd = safe - so;
+        }
+        offset[i] = so;
+        dhat[i]   = sd;
+    }
+
+    // culled stretch faces and bending hinges (all-pinned elements dropped)
+    for (uint32_t f = 0; f < nf; f++)
+    {
+        const uint32_t* t = &faces[3 * f];
+        if (is_fixed[t[0]] && is_fixed[t[1]] && is_fixed[t[2]]) continue;
+        sf.insert(sf.end(), t, t + 3);
+        float d[4];
+        dm_inv(at(rest_x, t[0]), at(rest_x, t[1]), at(rest_x, t[2]), d);
+        sf_Dm.insert(sf_Dm.end(), d, d + 4);
+        sf_area.push_back(face_area[f]);
+        const lcsx_material& mt = meshes[face_mesh[f]].mat;
+        double mu = mt.youngs_modulus / (2.0 * (1.0 + mt.poisson_ratio));
+        double la = mt.youngs_modulus * mt.poisson_ratio / ((1.0 + mt.poisson_ratio) * (1.0 - 2.0 * mt.poisson_ratio));
+        sf_mu.push_back((float)(mt.thickness * mu));
+        sf_lambda.push_back((float)(mt.thickness * la));
+    }
+    for (uint32_t h = 0; h < nh; h++)
+    {
+        const uint32_t* q = &hinges[4 * h];
+        if (is_fixed[q[0]] && is_fixed[q[1]] && is_fixed[q[2]] && is_fixed[q[3]]) continue;
+        bh.insert(bh.end(), q, q + 4);
+        f3 p0 = at(rest_x, q[0]), p1 = at(rest_x, q[1]), p2 = at(rest_x, q[2]), p3 = at(rest_x, q[3]);
+        bh_angle.push_back(dihedral_angle(p0, p1, p2, p3));
+        float A1 = length(cross(p1 - p0, p2 - p0)) * 0.5f, A2 = length(cross(p1 - p0, p3 - p0)) * 0.5f;
+        bh_hbar.push_back((A1 + A2) / (3.0f * length(p0 - p1)));
+        bh_k.push_back(meshes[vert_mesh[q[0]]].mat.area_bending_stiffness);
+    }
+    const uint32_t nsf = (uint32_t)(sf.size() / 3), nbh = (uint32_t)(bh.size() / 4);
+
+    // material neighbour lists (sorted), reference CSR layout
+    std::vector<std::vector<uint32_t>> adj(nv);
+    auto add = [&](uint32_t a, uint32_t b) {
+        auto& l = adj[a];
+        if (std::find(l.begin(), l.end(), b) == l.end()) l.push_back(b);
+    };
+    for (uint32_t f = 0; f < nsf; f++)
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++)
+                if (i != j) add(sf[3 * f + i], sf[3 * f + j]);
+    for (uint32_t e = 0; e < nbh; e++)
+        for (int i = 0; i < 4; i++)
+            for (int j = 0; j < 4; j++)
+                if (i != j) add(bh[4 * e + i], bh[4 * e + j]);
+    for (auto& l : adj)
+    {
+        std::sort(l.begin(), l.end());
+        max_degree = std::max<uint32_t>(max_degree, (uint32_t)l.size());
+    }
+    csr.assign(nv + 1, 0);
+    uint32_t cur = nv + 1;
+    for (uint32_t i = 0; i < nv; i++)
+    {
+        csr[i] = cur;
+        cur += (uint32_t)adj[i].size();
+    }
+    csr[nv] = cur;
+    for (uint32_t i = 0; i < nv; i++) csr.insert(csr.end(), adj[i].begin(), adj[i].end());
+
+    // vertex -> element gather lists in element order
+    auto build_lists = [&](const std::vector<uint32_t>& el, int N, std::vector<uint32_t>& ptr, std::vector<uint32_t>& lst) {
+        std::vector<uint32_t> cnt(nv + 1, 0);
+        uint32_t              ne_ = (uint32_t)(el.size() / N);
+        for (uint32_t e = 0; e < ne_; e++)
+            for (int k = 0; k < N; k++) cnt[el[N * e + k] + 1]++;
+        for (uint32_t i = 0; i < nv; i++) cnt[i + 1] += cnt[i];
+        ptr = cnt;
+        lst.assign(cnt[nv], 0);
+        std::vector<uint32_t> fill(cnt.begin(), cnt.end() - 1);
+        for (uint32_t e = 0; e < ne_; e++)
+            for (int k = 0; k < N; k++) lst[fill[el[N * e + k]]++] = (e << 2) | (uint32_t)k;
+    };
+    build_lists(sf, 3, v_sf_ptr, v_sf);
+    build_lists(bh, 4, v_bh_ptr, v_bh);
+
+    // SELL-64 layout
+    nslices = (nv + kSlice - 1) / kSlice;
+    sl_off.assign(nslices + 1, 0);
+    for (uint32_t s = 0; s < nslices; s++)
+    {
+        uint32_t w = 0;
+        for (uint32_t r = s * kSlice; r < std::min(nv, (s + 1) * kSlice); r++) w = std::max<uint32_t>(w, (uint32_t)adj[r].size());
+        sl_off[s + 1] = sl_off[s] + w;
+    }
+    nblocks = sl_off[nslices] * kSlice;
+    sell_col.assign(nblocks, 0);
+    csr_to_sell.assign(csr.size(), 0xffffffffu);
+    for (uint32_t s = 0; s < nslices; s++)
+    {
+        uint32_t w = sl_off[s + 1] - sl_off[s];
+        for (uint32_t l = 0; l < (uint32_t)kSlice; l++)
+        {
+            uint32_t r = s * kSlice + l;
+            for (uint32_t k = 0; k < w; k++)
+            {
+                uint32_t b = (sl_off[s] + k) * kSlice + l;
+                if (r < nv && k < adj[r].size())
+                {
+                    sell_col[b]                  = adj[r][k];
+                    csr_to_sell[csr[r] + k]      = b;
+                }
+                else
+                    sell_col[b] = r < nv ? r : 0;  // padding: zero block against the row itself
+            }
+        }
+    }
+    // contributors per SELL block: faces of the row vertex in face order, then hinges in order
+    std::vector<std::vector<uint32_t>> contrib(nblocks);
+    auto slot_of = [&](uint32_t r, uint32_t c) {
+        const auto& l = adj[r];
+        uint32_t    k = (uint32_t)(std::lower_bound(l.begin(), l.end(), c) - l.begin());
+        return (sl_off[r / kSlice] + k) * kSlice + (r % kSlice);
+    };
+    for (uint32_t r = 0; r < nv; r++)
+    {
+        for (uint32_t p = v_sf_ptr[r]; p < v_sf_ptr[r + 1]; p++)
+        {
+            uint32_t f = v_sf[p] >> 2, a = v_sf[p] & 3;
+            for (uint32_t b = 0; b < 3; b++)
+                if (b != a) contrib[slot_of(r, sf[3 * f + b])].push_back((a << 29) | (b << 27) | f);
+        }
+        for (uint32_t p = v_bh_ptr[r]; p < v_bh_ptr[r + 1]; p++)
+        {
+            uint32_t e = v_bh[p] >> 2, a = v_bh[p] & 3;
+            for (uint32_t b = 0; b < 4; b++)
+                if (b != a) contrib[slot_of(r, bh[4 * e + b])].push_back(0x80000000u | (a << 29) | (b << 27) | e);
+        }
+    }
+    blk_ptr.assign(nblocks + 1, 0);
+    for (uint32_t b = 0; b < nblocks; b++) blk_ptr[b + 1] = blk_ptr[b] + (uint32_t)contrib[b].size();
+    blk_contrib.reserve(blk_ptr[nblocks]);
+    for (auto& c : contrib) blk_contrib.insert(blk_contrib.end(), c.begin(), c.end());
+}
+
+}  // namespace lcsx

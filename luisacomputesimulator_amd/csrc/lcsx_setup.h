@@ -1,0 +1,61 @@
+// Host-side scene setup for the solver: topology, per-vertex quantities, culled elements,
+// the material block-sparse structure (reference CSR + device SELL-64 layout) and the
+// gather lists that let every output block be owned by exactly one thread.
+#pragma once
+#include <stdint.h>
+#include <string>
+#include <vector>
+#include "../../include/lcsx.h"
+
+namespace lcsx
+{
+
+struct MeshDesc
+{
+    std::vector<float>            x;     // world rest positions (3 per vertex)
+    std::vector<int32_t>          tris;  // local indices
+    lcsx_material                 mat;
+    std::vector<uint32_t>         pinned;  // local ids in selection order
+    std::vector<lcsx_fixed_anim>  anim;
+    std::vector<float>            explicit_targets;  // optional (3 per pinned vertex)
+};
+
+constexpr int kSlice = 64;  // SELL slice height (one wavefront)
+
+struct Setup
+{
+    uint32_t nv = 0, nf = 0, ne = 0, nh = 0;
+    std::vector<float>    rest_x;                  // 3 nv
+    std::vector<uint32_t> faces, edges, hinges;    // 3 nf, 2 ne, 4 nh (all elements, for collision)
+    std::vector<uint32_t> vert_mesh, face_mesh, edge_mesh;
+    std::vector<float>    face_area, edge_area, vert_area, vert_mass, offset, dhat;
+    std::vector<uint8_t>  is_fixed;
+    std::vector<uint32_t> fixed_verts;                   // pin order
+    std::vector<uint32_t> fixed_mesh, fixed_local;        // mesh id and index in that mesh's pin list
+    std::vector<lcsx_fixed_anim> fixed_anim;
+    // culled material elements
+    std::vector<uint32_t> sf;                      // 3 per stretch face
+    std::vector<float>    sf_Dm, sf_area, sf_mu, sf_lambda;
+    std::vector<uint32_t> bh;                      // 4 per bending hinge
+    std::vector<float>    bh_angle, bh_hbar, bh_k;
+    // reference CSR of material neighbours: nv+1 prefixes (offset by nv+1), then sorted columns
+    std::vector<uint32_t> csr;
+    // vertex -> (element << 2 | local) gather lists, in element order
+    std::vector<uint32_t> v_sf_ptr, v_sf, v_bh_ptr, v_bh;
+    // SELL-64 layout of the off-diagonal blocks: slice s holds rows [64 s, 64 s + 64), width
+    // sl_off[s+1] - sl_off[s]; block (row r, k-th neighbour) lives at (sl_off[s] + k) * 64 + (r % 64).
+    uint32_t              nslices = 0, nblocks = 0;
+    std::vector<uint32_t> sl_off, sell_col;
+    std::vector<uint32_t> csr_to_sell;             // for each CSR slot (header slots: kInvalid)
+    // per SELL block: contributors (bit 31 hinge, bits 29-30 local a, 27-28 local b, 0-26 element)
+    std::vector<uint32_t> blk_ptr, blk_contrib;
+    uint32_t              max_degree = 0;
+
+    void build(const std::vector<MeshDesc>& meshes);
+};
+
+// Pinned-target animation (reference init_mesh_data.h:56-93, luisa::rotation axis-angle).
+void animate_target(const lcsx_fixed_anim& a, float t, const float in[3], float out[3]);
+void pin_by_method(MeshDesc& m, int method, float range, const lcsx_fixed_anim& anim);
+
+}  // namespace lcsx

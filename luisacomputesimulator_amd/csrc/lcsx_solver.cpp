@@ -1,0 +1,584 @@
+// Newton driver (host) for the MI355X path. Mirrors NewtonSolver::physics_step_GPU
+// (reference Solver/SimulationSolver/newton_solver.cpp:4051-4345) with the host round trips
+// reduced to one status read per Newton iteration plus one per PCG chunk.
+#include "lcsx_solver.h"
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace lcsx
+{
+
+namespace
+{
+constexpr unsigned kPairGridMax = 4096;
+constexpr int      kPcgChunk    = 10;
+
+struct EventPool
+{
+    struct Rec
+    {
+        std::string name;
+        hipEvent_t  a, b;
+    };
+    std::vector<hipEvent_t> free_;
+    std::vector<Rec>        pending;
+    hipEvent_t              get()
+    {
+        if (!free_.empty())
+        {
+            hipEvent_t e = free_.back();
+            free_.pop_back();
+            return e;
+        }
+        hipEvent_t e;
+        LCSX_HIP(hipEventCreate(&e));
+        return e;
+    }
+    ~EventPool()
+    {
+        for (auto e : free_) (void)hipEventDestroy(e);
+        for (auto& r : pending)
+        {
+            (void)hipEventDestroy(r.a);
+            (void)hipEventDestroy(r.b);
+        }
+    }
+};
+thread_local EventPool* g_pool = nullptr;
+hipEvent_t              g_open = nullptr;
+}  // namespace
+
+Solver::~Solver()
+{
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+unsigned Solver::pair_grid(uint32_t cap) const { return std::max(1u, std::min(kPairGridMax, grid_for(cap, 256))); }
+
+void Solver::tic()
+{
+    if (!timing) return;
+    if (!g_pool) g_pool = new EventPool();
+    g_open = g_pool->get();
+    LCSX_HIP(hipEventRecord(g_open, stream));
+}
+void Solver::toc(const char* name)
+{
+    if (!timing || !g_open) return;
+    hipEvent_t b = g_pool->get();
+    LCSX_HIP(hipEventRecord(b, stream));
+    g_pool->pending.push_back({name, g_open, b});
+    g_open = nullptr;
+}
+
+hipEvent_t Solver::ev_get()
+{
+    if (!g_pool) g_pool = new EventPool();
+    return g_pool->get();
+}
+void Solver::ev_put(hipEvent_t e) { g_pool->free_.push_back(e); }
+
+void Solver::read_status()
+{
+    LCSX_HIP(hipMemcpyAsync(&host_status, status.p, sizeof(Status), hipMemcpyDeviceToHost, stream));
+    LCSX_HIP(hipStreamSynchronize(stream));
+    flush_events();
+}
+
+// Converts completed event pairs into per-name time sums (call after a stream synchronize).
+void Solver::flush_events()
+{
+    if (g_pool)
+    {
+        for (auto& r : g_pool->pending)
+        {
+            float ms = 0.0f;
+            if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess)
+            {
+                auto& t = times[r.name];
+                t.first += ms;
+                t.second += 1;
+            }
+            g_pool->free_.push_back(r.a);
+            g_pool->free_.push_back(r.b);
+        }
+        g_pool->pending.clear();
+    }
+}
+
+void Solver::alloc_collision(uint32_t vf, uint32_t ee, uint32_t nc)
+{
+    cap_vf = std::max(vf, 1024u);
+    cap_ee = std::max(ee, 1024u);
+    cap_c  = std::max(nc, 1024u);
+    pairs_vf.alloc(2 * (size_t)cap_vf);
+    pairs_ee.alloc(2 * (size_t)cap_ee);
+    c_idx.alloc(cap_c);
+    c_nk.alloc(cap_c);
+    c_w.alloc(cap_c);
+    c_aux.alloc(cap_c);
+}
+
+void Solver::init()
+{
+    S.build(meshes);
+    setup_ready = true;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+    {
+        set_error("no HIP device available");
+        throw NoDevice();
+    }
+    LCSX_HIP(hipSetDevice(device));
+    if (!stream) LCSX_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    const uint32_t nv = S.nv;
+    for (auto* b : {&x, &v, &x_step, &x_iter, &x_tilde, &dx, &cgB, &r, &z, &p[0], &p[1], &q, &spmv_tmp}) b->alloc(3 * (size_t)nv);
+    diag.alloc(9 * (size_t)nv);
+    minv.alloc(9 * (size_t)nv);
+    sell_val.alloc(9 * (size_t)std::max(S.nblocks, 1u));
+    partials.alloc(grid_for(nv, 256) + 1);
+    upload_state_constants();
+    tree_face.init(3, faces.p, S.nf);
+    tree_edge.init(2, edges.p, S.ne);
+    alloc_collision(24 * nv, 24 * S.ne, 12 * nv);
+    status.alloc(1);
+    pcg_state.alloc(1);
+    LCSX_HIP(hipMemsetAsync(pcg_state.p, 0, sizeof(PcgState), stream));
+    LCSX_HIP(hipMemsetAsync(sell_val.p, 0, sell_val.n * sizeof(float), stream));
+    explicit_targets.assign(3 * S.fixed_verts.size(), 0.0f);
+    has_explicit.assign(meshes.size(), 0);
+    LCSX_HIP(hipStreamSynchronize(stream));
+    inited = true;
+    frame  = 0;
+}
+
+void Solver::upload_state_constants()
+{
+    hipStream_t s = stream;
+    x.upload(S.rest_x, s);
+    rest_x.upload(S.rest_x, s);
+    std::vector<float> zero(3 * (size_t)S.nv, 0.0f);
+    v.upload(zero, s);
+    x_step.upload(S.rest_x, s);
+    offset.upload(S.offset, s);
+    dhat.upload(S.dhat, s);
+    vert_mass.upload(S.vert_mass, s);
+    vert_area.upload(S.vert_area, s);
+    face_area.upload(S.face_area, s);
+    edge_area.upload(S.edge_area, s);
+    is_fixed.upload(S.is_fixed, s);
+    faces.upload(S.faces, s);
+    edges.upload(S.edges, s);
+    sf.upload(S.sf, s);
+    bh.upload(S.bh, s);
+    v_sf_ptr.upload(S.v_sf_ptr, s);
+    v_sf.upload(S.v_sf, s);
+    v_bh_ptr.upload(S.v_bh_ptr, s);
+    v_bh.upload(S.v_bh, s);
+    sl_off.upload(S.sl_off, s);
+    sell_col.upload(S.sell_col, s);
+    blk_ptr.upload(S.blk_ptr, s);
+    blk_contrib.upload(S.blk_contrib, s);
+    fixed_ids.upload(S.fixed_verts, s);
+    targets.alloc(3 * std::max<size_t>(S.fixed_verts.size(), 1));
+    const size_t       nsf = S.sf.size() / 3, nbh = S.bh.size() / 4;
+    std::vector<float4> dm(nsf), cc(nsf), hc(nbh);
+    for (size_t f = 0; f < nsf; f++)
+    {
+        dm[f] = make_float4(S.sf_Dm[4 * f], S.sf_Dm[4 * f + 1], S.sf_Dm[4 * f + 2], S.sf_Dm[4 * f + 3]);
+        cc[f] = make_float4(S.sf_area[f], S.sf_mu[f], S.sf_lambda[f], 0.0f);
+    }
+    for (size_t e = 0; e < nbh; e++) hc[e] = make_float4(S.bh_angle[e], S.bh_hbar[e], S.bh_k[e], 0.0f);
+    sf_Dm.upload(dm, s);
+    sf_c.upload(cc, s);
+    bh_c.upload(hc, s);
+    face_stage.alloc(64 * std::max<size_t>(nsf, 1));
+    hinge_stage.alloc(16 * std::max<size_t>(nbh, 1));
+    LCSX_HIP(hipStreamSynchronize(s));
+}
+
+ContactView Solver::contact_view() const
+{
+    return ContactView{c_idx.p, c_nk.p, c_w.p, c_aux.p, &status.p->n_contacts, cap_c};
+}
+SellView Solver::sell_view() const { return SellView{sl_off.p, sell_col.p, sell_val.p, diag.p, S.nblocks}; }
+
+// ------------------------------------------------------------------ stages
+void Solver::frame_begin()
+{
+    const float t   = (float)frame * prm.implicit_dt;
+    const size_t nf = S.fixed_verts.size();
+    if (nf)
+    {
+        std::vector<float> tg(3 * nf);
+        for (size_t k = 0; k < nf; k++)
+        {
+            if (has_explicit[S.fixed_mesh[k]])
+            {
+                for (int d = 0; d < 3; d++) tg[3 * k + d] = explicit_targets[3 * k + d];
+                continue;
+            }
+            animate_target(S.fixed_anim[k], t, &S.rest_x[3 * (size_t)S.fixed_verts[k]], &tg[3 * k]);
+        }
+        targets.upload(tg, stream);
+        launch_pin_velocity((uint32_t)nf, fixed_ids.p, targets.p, x.p, v.p, prm.implicit_dt, stream);
+    }
+    LCSX_HIP(hipMemcpyAsync(x_step.p, x.p, x.n * sizeof(float), hipMemcpyDeviceToDevice, stream));
+    tic();
+    if (S.nf > 1) tree_face.construct(x_step.p, S.nv, stream);
+    if (S.ne > 1) tree_edge.construct(x_step.p, S.nv, stream);
+    toc("lbvh_build");
+}
+
+void Solver::predict()
+{
+    launch_predict(S.nv, x_step.p, v.p, is_fixed.p, prm.implicit_dt, prm.gravity, x_iter.p, x_tilde.p, x.p, stream);
+}
+
+void Solver::iter_begin()
+{
+    LCSX_HIP(hipMemcpyAsync(x_iter.p, x.p, x.n * sizeof(float), hipMemcpyDeviceToDevice, stream));
+    launch_reset_status(status.p, stream);
+}
+
+void Solver::material()
+{
+    tic();
+    const uint32_t nsf = (uint32_t)(S.sf.size() / 3), nbh = (uint32_t)(S.bh.size() / 4);
+    launch_face_stage(nsf, sf.p, sf_Dm.p, sf_c.p, x.p, face_stage.p, stream);
+    launch_hinge_stage(nbh, bh.p, bh_c.p, prm.bending_scaling, x.p, hinge_stage.p, stream);
+    VertArgs a{};
+    a.nv       = S.nv;
+    a.x        = x.p;
+    a.x_tilde  = x_tilde.p;
+    a.mass     = vert_mass.p;
+    a.fixed    = is_fixed.p;
+    a.area     = vert_area.p;
+    a.offset   = offset.p;
+    a.dhat     = dhat.p;
+    float h    = prm.implicit_dt;
+    a.h2inv    = 1.0f / (h * h);
+    a.kD       = prm.stiffness_dirichlet;
+    a.kappa    = prm.stiffness_collision;
+    a.floor_y  = prm.floor_y;
+    a.use_floor = prm.use_floor;
+    a.v_sf_ptr = v_sf_ptr.p;
+    a.v_sf     = v_sf.p;
+    a.v_bh_ptr = v_bh_ptr.p;
+    a.v_bh     = v_bh.p;
+    a.face_stage  = face_stage.p;
+    a.hinge_stage = hinge_stage.p;
+    a.cgB      = cgB.p;
+    a.diag     = diag.p;
+    launch_vert_assemble(a, stream);
+    launch_offdiag(S.nblocks, blk_ptr.p, blk_contrib.p, face_stage.p, hinge_stage.p, sell_val.p, stream);
+    toc("material");
+}
+
+void Solver::broad_phase(bool ccd)
+{
+    tic();
+    const float* x0 = ccd ? x_iter.p : x.p;
+    Status*      st = status.p;
+    if (S.nf > 1)
+    {
+        tree_face.refit(x0, x.p, offset.p, stream);
+        QueryArgs q{};
+        q.mode      = QueryVerts;
+        q.n_queries = S.nv;
+        q.x0        = x0;
+        q.x1        = x.p;
+        q.offset    = offset.p;
+        q.dhat      = dhat.p;
+        q.pairs     = pairs_vf.p;
+        q.capacity  = cap_vf;
+        q.count     = ccd ? &st->n_vf_ccd : &st->n_vf;
+        q.overflow  = &st->overflow;
+        lbvh_query(tree_face, q, stream);
+    }
+    if (S.ne > 1)
+    {
+        tree_edge.refit(x0, x.p, offset.p, stream);
+        QueryArgs q{};
+        q.mode      = QueryEdges;
+        q.n_queries = S.ne;
+        q.x0        = x0;
+        q.x1        = x.p;
+        q.edges     = edges.p;
+        q.offset    = offset.p;
+        q.dhat      = dhat.p;
+        q.pairs     = pairs_ee.p;
+        q.capacity  = cap_ee;
+        q.count     = ccd ? &st->n_ee_ccd : &st->n_ee;
+        q.overflow  = &st->overflow;
+        lbvh_query(tree_edge, q, stream);
+    }
+    last_bp = ccd ? 2 : 1;
+    toc(ccd ? "broadphase_ccd" : "broadphase_dcd");
+}
+
+void Solver::dcd()
+{
+    tic();
+    ContactBuf out{c_idx.p, c_nk.p, c_w.p, c_aux.p, &status.p->n_contacts, cap_c};
+    DcdArgs    a{};
+    a.x      = x.p;
+    a.faces  = faces.p;
+    a.edges  = edges.p;
+    a.offset = offset.p;
+    a.dhat   = dhat.p;
+    a.kappa  = prm.stiffness_collision;
+    a.out    = out;
+    a.pairs = pairs_vf.p;
+    a.pair_count = &status.p->n_vf;
+    a.pair_capacity = cap_vf;
+    a.area_a = vert_area.p;
+    a.area_b = face_area.p;
+    launch_dcd(a, 0, pair_grid(cap_vf), stream);
+    a.pairs = pairs_ee.p;
+    a.pair_count = &status.p->n_ee;
+    a.pair_capacity = cap_ee;
+    a.area_a = edge_area.p;
+    a.area_b = edge_area.p;
+    launch_dcd(a, 1, pair_grid(cap_ee), stream);
+    toc("dcd");
+}
+
+void Solver::update_contacts()
+{
+    if (prm.use_self_collision)
+    {
+        broad_phase(false);
+        dcd();
+    }
+}
+
+void Solver::contact_assemble()
+{
+    if (!prm.use_self_collision) return;
+    tic();
+    launch_contact_assemble(contact_view(), cgB.p, diag.p, S.nv, pair_grid(cap_c), stream);
+    toc("contact_assemble");
+}
+
+void Solver::pcg()
+{
+    tic();
+    PcgArgs a{};
+    a.n        = S.nv;
+    a.b        = cgB.p;
+    a.diag     = diag.p;
+    a.minv     = minv.p;
+    a.x        = dx.p;
+    a.r        = r.p;
+    a.z        = z.p;
+    a.p_old    = p[0].p;
+    a.p_new    = p[1].p;
+    a.q        = q.p;
+    a.partials = partials.p;
+    a.s        = pcg_state.p;
+    a.rel2     = prm.pcg_rel_tol * prm.pcg_rel_tol;
+    a.abs_tol  = prm.pcg_abs_tol;
+    a.check_every = (uint32_t)std::max(1, prm.pcg_check_every);
+    launch_pcg_init(a, stream);
+    SellView    A  = sell_view();
+    ContactView cv = contact_view();
+    PcgState    hs{};
+    int         issued = 0;
+    int         par    = 0;
+    while (issued < prm.pcg_iter_count)
+    {
+        int chunk = std::min(kPcgChunk, prm.pcg_iter_count - issued);
+        for (int k = 0; k < chunk; k++)
+        {
+            PcgArgs b = a;
+            b.p_old   = p[par].p;
+            b.p_new   = p[1 - par].p;
+            hipEvent_t e0 = nullptr, e1 = nullptr;
+            if (timing_spmv)
+            {
+                e0 = ev_get();
+                LCSX_HIP(hipEventRecord(e0, stream));
+            }
+            launch_pcg_spmv(b, A, stream);
+            if (timing_spmv)
+            {
+                e1 = ev_get();
+                LCSX_HIP(hipEventRecord(e1, stream));
+                spmv_events.push_back({e0, e1});
+            }
+            if (prm.use_self_collision) launch_pcg_contact_spmv(b, cv, pair_grid(cap_c), stream);
+            launch_pcg_finish(b, stream);
+            par = 1 - par;
+        }
+        issued += chunk;
+        LCSX_HIP(hipMemcpyAsync(&hs, pcg_state.p, sizeof(PcgState), hipMemcpyDeviceToHost, stream));
+        LCSX_HIP(hipStreamSynchronize(stream));
+        flush_events();
+        if (hs.converged) break;
+    }
+    toc("pcg");
+    // only launches that did work (the first `iters`; later ones exit on the converged flag)
+    for (size_t k = 0; k < spmv_events.size(); k++)
+    {
+        float ms = 0.0f;
+        if (k < hs.iters && hipEventElapsedTime(&ms, spmv_events[k].first, spmv_events[k].second) == hipSuccess)
+        {
+            auto& t = times["spmv"];
+            t.first += ms;
+            t.second += 1;
+        }
+        ev_put(spmv_events[k].first);
+        ev_put(spmv_events[k].second);
+    }
+    spmv_events.clear();
+    stats.pcg_iters = (int)hs.iters;
+    stats.pcg_iters_total += (int)hs.iters;
+    stats.pcg_rel   = hs.rz0 > 0.0f ? std::sqrt(hs.rz / hs.rz0) : 0.0f;
+    if (hs.nan)
+    {
+        set_error("NaN/Inf in the PCG iteration");
+        throw HipFailure();
+    }
+}
+
+void Solver::ccd()
+{
+    tic();
+    LsArgs l{};
+    l.nv        = S.nv;
+    l.x_iter    = x_iter.p;
+    l.x         = x.p;
+    l.dx        = dx.p;
+    l.fixed     = is_fixed.p;
+    l.offset    = offset.p;
+    l.floor_y   = prm.floor_y;
+    l.use_floor = prm.use_floor && prm.use_ccd_linesearch;
+    l.st        = status.p;
+    launch_ground_ccd_maxdx(l, stream);
+    if (prm.use_ccd_linesearch && prm.use_self_collision)
+    {
+        CcdArgs c{};
+        c.x0       = x_iter.p;
+        c.x1       = x.p;
+        c.faces    = faces.p;
+        c.edges    = edges.p;
+        c.offset   = offset.p;
+        c.toi_bits = &status.p->toi_bits;
+        c.kind     = 0;
+        c.pairs    = pairs_vf.p;
+        c.pair_count    = &status.p->n_vf_ccd;
+        c.pair_capacity = cap_vf;
+        launch_ccd(c, pair_grid(cap_vf), stream);
+        c.kind          = 1;
+        c.pairs         = pairs_ee.p;
+        c.pair_count    = &status.p->n_ee_ccd;
+        c.pair_capacity = cap_ee;
+        launch_ccd(c, pair_grid(cap_ee), stream);
+    }
+    toc("ccd");
+}
+
+bool Solver::grow_if_needed(bool ccd_only)
+{
+    uint32_t vf = ccd_only ? host_status.n_vf_ccd : std::max(host_status.n_vf, host_status.n_vf_ccd);
+    uint32_t ee = ccd_only ? host_status.n_ee_ccd : std::max(host_status.n_ee, host_status.n_ee_ccd);
+    uint32_t nc = ccd_only ? 0 : host_status.n_contacts;
+    if (vf <= cap_vf && ee <= cap_ee && nc <= cap_c) return false;
+    alloc_collision(std::max(cap_vf, vf + vf / 2), std::max(cap_ee, ee + ee / 2), std::max(cap_c, nc + nc / 2));
+    return true;
+}
+
+// NewtonSolver::line_search, newton_solver.cpp:3685-3884 (CCD step bound, no energy backtracking)
+void Solver::line_search()
+{
+    launch_reset_line_search(status.p, stream);
+    launch_apply_dx(S.nv, x_iter.p, dx.p, x.p, nullptr, 1.0f, stream);
+    if (prm.use_ccd_linesearch && prm.use_self_collision) broad_phase(true);
+    ccd();
+    launch_apply_dx(S.nv, x_iter.p, dx.p, x.p, &status.p->toi_bits, 1.0f, stream);
+    launch_pin_delta((uint32_t)S.fixed_verts.size(), fixed_ids.p, x.p, x_tilde.p, &status.p->pin_bits, stream);
+}
+
+bool Solver::newton_iteration()
+{
+    iter_begin();
+    material();
+    update_contacts();
+    contact_assemble();
+    pcg();
+    line_search();
+    read_status();
+    if (host_status.nan)
+    {
+        set_error("NaN/Inf in the Newton direction");
+        throw HipFailure();
+    }
+    if (grow_if_needed(false))
+    {
+        // contact lists were truncated: restore the iterate and redo the iteration
+        LCSX_HIP(hipMemcpyAsync(x.p, x_iter.p, x.n * sizeof(float), hipMemcpyDeviceToDevice, stream));
+        return false;
+    }
+    stats.toi        = ord2f(host_status.toi_bits);
+    stats.max_dx     = __builtin_bit_cast(float, host_status.maxdx_bits);
+    stats.pin_delta  = S.fixed_verts.empty() ? -1e9f : __builtin_bit_cast(float, host_status.pin_bits);
+    stats.num_pairs_vf = (int)host_status.n_vf;
+    stats.num_pairs_ee = (int)host_status.n_ee;
+    stats.num_contacts = (int)host_status.n_contacts;
+    stats.broadphase_overflow |= host_status.overflow ? 1 : 0;
+    if (!(stats.toi > 0.0f && stats.toi <= 1.0f))
+    {
+        set_error("invalid CCD step bound " + std::to_string(stats.toi));
+        throw HipFailure();
+    }
+    return true;
+}
+
+void Solver::frame_end()
+{
+    float h = prm.implicit_dt;
+    launch_frame_end(S.nv, x_step.p, x.p, v.p, h, std::exp(-prm.damping_cloth * h), stream);
+    frame += 1;
+    stats.frame = frame;
+}
+
+void Solver::step()
+{
+    frame_begin();
+    predict();
+    bool converged = false, dirichlet = false;
+    int  it        = 0;
+    stats.pcg_iters_total     = 0;
+    stats.broadphase_overflow = 0;
+    for (it = 0; it < prm.newton_max_iters; it++)
+    {
+        if (converged || (it >= prm.nonlinear_iter_count && dirichlet)) break;
+        while (!newton_iteration())
+        {
+        }
+        if (stats.max_dx < prm.newton_tol * prm.implicit_dt)
+        {
+            converged = dirichlet = true;
+        }
+        else
+        {
+            converged = false;
+            dirichlet = stats.pin_delta < 1e-3f;
+        }
+    }
+    stats.newton_iters = it;
+    stats.converged    = converged ? 1 : 0;
+    frame_end();
+    LCSX_HIP(hipStreamSynchronize(stream));
+}
+
+void Solver::spmv_dx()
+{
+    ContactView cv = contact_view();
+    launch_spmv(sell_view(), prm.use_self_collision ? &cv : nullptr, pair_grid(cap_c), dx.p, spmv_tmp.p, S.nv, stream);
+    LCSX_HIP(hipMemcpyAsync(dx.p, spmv_tmp.p, dx.n * sizeof(float), hipMemcpyDeviceToDevice, stream));
+}
+
+}  // namespace lcsx

@@ -1,0 +1,94 @@
+// The Newton solver context behind the C ABI (host side).
+#pragma once
+#include <map>
+#include <string>
+#include <vector>
+#include "../../include/lcsx.h"
+#include "lcsx_common.h"
+#include "lcsx_internal.h"
+#include "lcsx_kernels.h"
+#include "lcsx_setup.h"
+
+namespace lcsx
+{
+
+struct Solver
+{
+    lcsx_params           prm{};
+    int                   device = 0;
+    hipStream_t           stream = nullptr;
+    std::vector<MeshDesc> meshes;
+    Setup                 S;
+    bool                  inited = false;
+    bool                  setup_ready = false;  // host setup built (readable without a GPU)
+    int                   frame  = 0;
+    lcsx_stats            stats{};
+
+    // state (3 floats per vertex)
+    DBuf<float> x, v, x_step, x_iter, x_tilde, dx, cgB, r, z, p[2], q, spmv_tmp;
+    DBuf<float> diag, minv, sell_val;  // 9 planes
+    DBuf<float> partials;
+    // constants
+    DBuf<float>    rest_x, offset, dhat, vert_mass, vert_area, face_area, edge_area, targets;
+    DBuf<uint8_t>  is_fixed;
+    DBuf<uint32_t> faces, edges, sf, bh, v_sf_ptr, v_sf, v_bh_ptr, v_bh, sl_off, sell_col, blk_ptr, blk_contrib, fixed_ids;
+    DBuf<float4>   sf_Dm, sf_c, bh_c;
+    DBuf<float>    face_stage, hinge_stage;
+    Lbvh           tree_face, tree_edge;
+    // collision buffers
+    DBuf<uint32_t> pairs_vf, pairs_ee;
+    uint32_t       cap_vf = 0, cap_ee = 0, cap_c = 0;
+    DBuf<uint4>    c_idx;
+    DBuf<float4>   c_nk, c_w, c_aux;
+    DBuf<Status>   status;
+    DBuf<PcgState> pcg_state;
+    Status         host_status{};
+    int            last_bp = 0;  // 0 none, 1 DCD, 2 CCD
+    int            pcg_parity = 0;
+    std::vector<float> explicit_targets;  // per fixed vertex (3), when set
+    std::vector<uint8_t> has_explicit;     // per mesh
+
+    // timing
+    bool                         timing = true;
+    std::map<std::string, std::pair<double, int>> times;
+    hipEvent_t                   ev[2] = {nullptr, nullptr};
+
+    ~Solver();
+    void init();
+    void upload_state_constants();
+    void alloc_collision(uint32_t vf, uint32_t ee, uint32_t nc);
+
+    // stages
+    void frame_begin();
+    void predict();
+    void iter_begin();
+    void material();
+    void broad_phase(bool ccd);
+    void dcd();
+    void update_contacts();
+    void contact_assemble();
+    void pcg();
+    void ccd();
+    void line_search();
+    bool newton_iteration();  // returns false if it must be redone (buffer growth)
+    void frame_end();
+    void step();
+    void spmv_dx();
+    void read_status();
+    bool grow_if_needed(bool ccd_only);
+
+    ContactView contact_view() const;
+    SellView    sell_view() const;
+    unsigned    pair_grid(uint32_t cap) const;
+
+    // timing helpers
+    void tic();
+    void toc(const char* name);
+    void       flush_events();
+    hipEvent_t ev_get();
+    void       ev_put(hipEvent_t e);
+    bool       timing_spmv = false;  // hipEvent pair around every SpMV launch (measurement mode)
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> spmv_events;
+};
+
+}  // namespace lcsx

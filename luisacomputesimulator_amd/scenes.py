@@ -1,0 +1,229 @@
+"""Synthetic scene specifications for the cloth Newton step.
+
+A scene is a plain description (meshes, materials, pinned vertices, solver parameters) that
+the solver consumes through :class:`luisacomputesimulator_amd.NewtonSolver`. The field names
+follow the reference's scene JSON (``Application/app_simulation_demo_config.cpp:174-193`` for
+scene keys, ``:363-480`` for ``ClothMaterial`` keys, ``:630-745`` for ``fixed_points``) and the
+material defaults of ``Solver/SimulationCore/physical_material.h:45-63``.
+
+Workloads:
+  * ``square_cloth(33)``     1.0 x 1.0 m cloth, 33 x 33 vertices, two back corners pinned,
+                             dropped under gravity onto the ground plane y = 0.
+  * ``cylinder_tube()``      297 rings x 297 vertices (88,209 vertices) cloth tube of radius
+                             0.11 m around a kinematic cylinder (r = 0.1 m) spinning at 2*pi rad/s.
+  * ``twisting_sheet(513)``  513 x 513 sheet whose opposite edges rotate in opposite directions.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+from typing import Dict, List, Optional
+
+import numpy as np
+
+# FixedPointsType (Solver/Initializer/init_mesh_data.h:14-37)
+FIXED_METHODS = {
+    "None": 0, "FromIndices": 1, "FromFunction": 2, "Left": 3, "Right": 4, "Front": 5, "Back": 6,
+    "Up": 7, "Down": 8, "LeftUp": 9, "LeftDown": 10, "LeftFront": 11, "LeftBack": 12,
+    "RightUp": 13, "RightDown": 14, "RightFront": 15, "RightBack": 16, "FrontUp": 17,
+    "FrontDown": 18, "BackUp": 19, "BackDown": 20, "All": 21,
+}
+
+
+@dataclasses.dataclass
+class Anim:
+    """FixedPointAnimationInfo (Solver/Initializer/init_mesh_data.h:39-93)."""
+    use_translate: bool = False
+    translate: tuple = (0.0, 0.0, 0.0)
+    use_scale: bool = False
+    scale: tuple = (1.0, 1.0, 1.0)
+    use_rotate: bool = False
+    rot_center: tuple = (0.0, 0.0, 0.0)
+    rot_axis: tuple = (1.0, 0.0, 0.0)
+    rot_deg_per_s: float = 0.0
+
+    def pack(self) -> List[float]:
+        return [float(self.use_translate), *map(float, self.translate), float(self.use_scale),
+                *map(float, self.scale), float(self.use_rotate), *map(float, self.rot_center),
+                *map(float, self.rot_axis), float(self.rot_deg_per_s), 0.0]
+
+
+@dataclasses.dataclass
+class Material:
+    """ClothMaterial (Solver/SimulationCore/physical_material.h:45-63)."""
+    thickness: float = 2e-3
+    youngs_modulus: float = 1e5
+    poisson_ratio: float = 0.25
+    area_bending_stiffness: float = 5e-3
+    density: float = 1e3
+    mass: float = 0.0
+    d_hat: float = 2e-3
+
+    def pack(self) -> List[float]:
+        return [self.thickness, self.youngs_modulus, self.poisson_ratio, self.area_bending_stiffness,
+                self.density, self.mass, self.d_hat]
+
+
+@dataclasses.dataclass
+class FixedPoints:
+    method: str = "All"
+    range: float = 0.001
+    anim: Anim = dataclasses.field(default_factory=Anim)
+
+
+@dataclasses.dataclass
+class Mesh:
+    name: str
+    verts: np.ndarray  # (n, 3) float32, world space
+    tris: np.ndarray   # (m, 3) int32
+    material: Material = dataclasses.field(default_factory=Material)
+    fixed: List[FixedPoints] = dataclasses.field(default_factory=list)
+
+
+@dataclasses.dataclass
+class Params:
+    """Scene-level solver parameters (Solver/SimulationCore/scene_params.h:15-182)."""
+    implicit_dt: float = 1.0 / 60.0
+    nonlinear_iter_count: int = 3
+    newton_max_iters: int = 100
+    pcg_iter_count: int = 100
+    pcg_check_every: int = 1
+    pcg_rel_tol: float = 1e-4
+    pcg_abs_tol: float = 1e-10
+    gravity: tuple = (0.0, -9.8, 0.0)
+    use_floor: bool = True
+    use_self_collision: bool = True
+    use_ccd_linesearch: bool = True
+    floor_y: float = 0.0
+    stiffness_collision: float = 1e7
+    stiffness_dirichlet: float = 1e9
+    damping_cloth: float = 2.0
+    bending_scaling: float = 1.0
+    newton_tol: float = 1e-2
+
+    def pack(self) -> List[float]:
+        return [self.implicit_dt, self.nonlinear_iter_count, self.newton_max_iters, self.pcg_iter_count,
+                self.pcg_check_every, self.pcg_rel_tol, self.pcg_abs_tol, *map(float, self.gravity),
+                float(self.use_floor), float(self.use_self_collision), float(self.use_ccd_linesearch),
+                self.floor_y, self.stiffness_collision, self.stiffness_dirichlet, self.damping_cloth,
+                self.bending_scaling, self.newton_tol]
+
+
+@dataclasses.dataclass
+class Scene:
+    name: str
+    meshes: List[Mesh]
+    params: Params
+    frames: int = 1
+
+    @property
+    def num_verts(self) -> int:
+        return sum(len(m.verts) for m in self.meshes)
+
+    @property
+    def num_faces(self) -> int:
+        return sum(len(m.tris) for m in self.meshes)
+
+
+def grid_triangles(rows: int, cols: int, wrap_cols: bool = False) -> np.ndarray:
+    """Triangles of a rows x cols vertex grid (vertex id = r * cols + c) with diagonals
+    alternating cell by cell; ``wrap_cols`` closes the grid around the column direction."""
+    ncols = cols if wrap_cols else cols - 1
+    tris = np.empty(((rows - 1) * ncols * 2, 3), dtype=np.int32)
+    k = 0
+    for r in range(rows - 1):
+        for c in range(ncols):
+            c1 = (c + 1) % cols
+            a, b, cc, d = r * cols + c, r * cols + c1, (r + 1) * cols + c, (r + 1) * cols + c1
+            if (r + c) % 2 == 0:
+                tris[k] = (a, cc, d)
+                tris[k + 1] = (a, d, b)
+            else:
+                tris[k] = (a, cc, b)
+                tris[k + 1] = (b, cc, d)
+            k += 2
+    return tris
+
+
+def square_cloth(n: int = 33, seed: int = 0, size: float = 1.0, height: float = 0.5,
+                 jitter: float = 1e-4) -> Scene:
+    """Square cloth in the xz-plane, two back corners pinned, dropped onto the ground y = 0.
+
+    Material: Young's 1e4 Pa, nu 0.3, thickness 1 mm, 0.2 kg/m^2 areal density (volumetric
+    density 200 kg/m^3 since mass = area * thickness * density), bending stiffness 1e-4,
+    d_hat 1e-3 m; kappa = 1e7 (the reference default, scene_params.h:130).
+    """
+    rng = np.random.default_rng(seed)
+    xs = np.linspace(-0.5 * size, 0.5 * size, n, dtype=np.float64)
+    z, x = np.meshgrid(xs, xs, indexing="ij")
+    y = height + rng.uniform(-jitter, jitter, size=(n, n))
+    verts = np.stack([x, y, z], axis=-1).reshape(-1, 3).astype(np.float32)
+    mat = Material(thickness=1e-3, youngs_modulus=1e4, poisson_ratio=0.3, area_bending_stiffness=1e-4,
+                   density=200.0, d_hat=1e-3)
+    mesh = Mesh("square", verts, grid_triangles(n, n), mat,
+                [FixedPoints("LeftBack", 0.001), FixedPoints("RightBack", 0.001)])
+    return Scene(f"square_cloth_{n}", [mesh], Params(stiffness_collision=1e7), frames=30)
+
+
+def cylinder_tube(rings: int = 297, per_ring: int = 297, radius: float = 0.11, length: float = 1.0,
+                  cyl_radius: float = 0.1, cyl_segments: int = 128, cyl_rings: int = 45,
+                  omega_deg: float = 360.0) -> Scene:
+    """Cloth tube around a kinematic cylinder that spins about the x axis.
+
+    The tube (rings x per_ring vertices, axis along x, centred at the origin) has both end rings
+    pinned and rotating with the cylinder. The cylinder is an all-pinned cloth mesh (its elements
+    are culled as in init_sim_data.cpp:207-240, it only collides). Config-0 material, gravity
+    -9.8 m/s^2, no ground, kappa = 1e9 as the reference's 88K cylinder scene.
+    """
+    spin = Anim(use_rotate=True, rot_center=(0.0, 0.0, 0.0), rot_axis=(1.0, 0.0, 0.0), rot_deg_per_s=omega_deg)
+    xs = np.linspace(-0.5 * length, 0.5 * length, rings)
+    th = 2.0 * np.pi * np.arange(per_ring) / per_ring
+    X, TH = np.meshgrid(xs, th, indexing="ij")
+    tube = np.stack([X, radius * np.cos(TH), radius * np.sin(TH)], -1).reshape(-1, 3).astype(np.float32)
+    mat = Material(thickness=1e-3, youngs_modulus=1e4, poisson_ratio=0.3, area_bending_stiffness=1e-4,
+                   density=200.0, d_hat=1e-3)
+    tube_mesh = Mesh("tube", tube, grid_triangles(rings, per_ring, wrap_cols=True), mat,
+                     [FixedPoints("Left", 0.001, spin), FixedPoints("Right", 0.001, spin)])
+    cx = np.linspace(-0.55 * length, 0.55 * length, cyl_rings)
+    ct = 2.0 * np.pi * np.arange(cyl_segments) / cyl_segments
+    CX, CT = np.meshgrid(cx, ct, indexing="ij")
+    cyl = np.stack([CX, cyl_radius * np.cos(CT), cyl_radius * np.sin(CT)], -1).reshape(-1, 3).astype(np.float32)
+    cyl_mesh = Mesh("cylinder", cyl, grid_triangles(cyl_rings, cyl_segments, wrap_cols=True), mat,
+                    [FixedPoints("All", 0.001, spin)])
+    prm = Params(use_floor=False, stiffness_collision=1e9)
+    return Scene(f"cylinder_tube_{rings}x{per_ring}", [tube_mesh, cyl_mesh], prm, frames=60)
+
+
+def twisting_sheet(n: int = 513, omega_deg: float = 180.0) -> Scene:
+    """n x n sheet of 1 x 1 m whose two opposite edges rotate in opposite directions about the
+    sheet's centre line (x axis), d_hat 5e-4 m, no gravity, no ground."""
+    xs = np.linspace(-0.5, 0.5, n)
+    z, x = np.meshgrid(xs, xs, indexing="ij")
+    verts = np.stack([x, np.zeros_like(x), z], -1).reshape(-1, 3).astype(np.float32)
+    mat = Material(thickness=1e-3, youngs_modulus=1e4, poisson_ratio=0.3, area_bending_stiffness=1e-4,
+                   density=200.0, d_hat=5e-4)
+    left = Anim(use_rotate=True, rot_axis=(1.0, 0.0, 0.0), rot_deg_per_s=omega_deg)
+    right = Anim(use_rotate=True, rot_axis=(1.0, 0.0, 0.0), rot_deg_per_s=-omega_deg)
+    mesh = Mesh("sheet", verts, grid_triangles(n, n), mat,
+                [FixedPoints("Left", 0.001, left), FixedPoints("Right", 0.001, right)])
+    return Scene(f"twisting_sheet_{n}", [mesh], Params(use_floor=False, gravity=(0.0, 0.0, 0.0)), frames=120)
+
+
+def small_two_sheets(n: int = 9, gap: float = 2.5e-3) -> Scene:
+    """Two small stacked sheets a few millimetres apart (self-contact exercise): the lower one is
+    pinned at its four corners, the upper one falls onto it."""
+    xs = np.linspace(-0.05, 0.05, n)
+    z, x = np.meshgrid(xs, xs, indexing="ij")
+    lower = np.stack([x, np.zeros_like(x), z], -1).reshape(-1, 3).astype(np.float32)
+    upper = np.stack([x * 0.8 + 0.003, np.full_like(x, gap), z * 0.8 - 0.002], -1).reshape(-1, 3).astype(np.float32)
+    mat = Material(thickness=1e-3, youngs_modulus=1e4, poisson_ratio=0.3, area_bending_stiffness=1e-4,
+                   density=200.0, d_hat=1e-3)
+    corners = [FixedPoints(m, 0.001) for m in ("LeftBack", "RightBack", "LeftFront", "RightFront")]
+    m0 = Mesh("lower", lower, grid_triangles(n, n), mat, corners)
+    m1 = Mesh("upper", upper, grid_triangles(n, n), mat, [])
+    return Scene("two_sheets", [m0, m1], Params(use_floor=False, stiffness_collision=1e7), frames=10)
+
+
+def by_name(name: str, **kw) -> Scene:
+    table = {"square": square_cloth, "tube": cylinder_tube, "twist": twisting_sheet, "two_sheets": small_two_sheets}
+    return table[name](**kw)

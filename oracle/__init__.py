@@ -1,0 +1,10 @@
+"""CPU oracle for the cloth Newton step: a restatement of the reference algorithm in C++.
+
+TEST INFRASTRUCTURE ONLY. Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s
+``cpu_baseline`` leg may import this package, and only as the checker (or as the timed CPU
+baseline, clearly labelled). The product never imports it.
+
+The restatement follows the reference files cited at the top of each ``orc_*.h`` header.
+See ``oracle/build.py`` for how ``liblcs_oracle.so`` is compiled (g++, -ffp-contract=off).
+"""
+from .pyoracle import Oracle, OracleLbvh, load, morton32, accd, dist2, barrier, bw_face, dm_inv, dihedral  # noqa: F401

@@ -1,0 +1,33 @@
+"""Build recipe for the CPU oracle library (test infrastructure, see oracle/__init__.py).
+
+The reference itself cannot be compiled here: its kernels are LuisaCompute DSL lambdas and
+LuisaCompute (plus Eigen) are fetched from the network at configure time
+(reference ext/CMakeLists.txt:5-52), so there is no oracle/_ref build.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liblcs_oracle.so")
+SOURCES = ["orc_capi.cpp"]
+HEADERS = ["orc_math.h", "orc_lbvh.h", "orc_collision.h", "orc_energy.h", "orc_scene.h"]
+
+
+def build(force: bool = True) -> str:
+    srcs = [os.path.join(HERE, s) for s in SOURCES]
+    deps = srcs + [os.path.join(HERE, h) for h in HEADERS]
+    if not force and os.path.exists(LIB) and all(os.path.getmtime(LIB) >= os.path.getmtime(d) for d in deps):
+        return LIB
+    tmp = LIB + ".tmp"
+    cmd = ["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-ffp-contract=off", "-fno-fast-math",
+           "-o", tmp] + srcs
+    subprocess.run(cmd, check=True, cwd=HERE)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(force="--force" in sys.argv or True))

@@ -1,0 +1,495 @@
+// Oracle C interface (ctypes) over the CPU restatement in orc_*.h.
+//
+// TEST INFRASTRUCTURE ONLY: loaded by tests/, __graft_entry__.smoke() and bench.py's
+// cpu_baseline leg as the checker. The product never links or loads this library.
+#include <cstring>
+#include <string>
+#include "orc_scene.h"
+
+using namespace orc;
+
+namespace
+{
+enum DType
+{
+    DT_U32 = 0,
+    DT_F32 = 1,
+    DT_F64 = 2,
+    DT_U64 = 3,
+    DT_U16 = 4,
+    DT_U8  = 5
+};
+
+struct Handle
+{
+    int           prec = 32;
+    Scene<float>  s32;
+    Scene<double> s64;
+};
+
+template <class T>
+struct View
+{
+    const void* p;
+    int64_t     n;
+    int         dt;
+};
+
+template <class T> constexpr int fdt() { return sizeof(T) == 4 ? DT_F32 : DT_F64; }
+
+// Packs contact records into flat arrays on request.
+template <class T>
+struct ContactCache
+{
+    std::vector<uint32_t> idx, type;
+    std::vector<T>        n, k, w, bary;
+};
+
+template <class T>
+bool lookup(Scene<T>& s, const std::string& name, View<T>& v, ContactCache<T>& cc)
+{
+    auto vec3 = [&](const std::vector<V3<T>>& a) { v = {a.data(), (int64_t)a.size() * 3, fdt<T>()}; return true; };
+    auto m3   = [&](const std::vector<M3<T>>& a) { v = {a.data(), (int64_t)a.size() * 9, fdt<T>()}; return true; };
+    auto u32  = [&](const std::vector<uint32_t>& a) { v = {a.data(), (int64_t)a.size(), DT_U32}; return true; };
+    auto fl   = [&](const std::vector<T>& a) { v = {a.data(), (int64_t)a.size(), fdt<T>()}; return true; };
+    if (name == "x") return vec3(s.x);
+    if (name == "v") return vec3(s.v);
+    if (name == "x_step") return vec3(s.x_step);
+    if (name == "x_iter") return vec3(s.x_iter);
+    if (name == "x_tilde") return vec3(s.x_tilde);
+    if (name == "x_outer") return vec3(s.x_outer);
+    if (name == "v_outer") return vec3(s.v_outer);
+    if (name == "rest_x") return vec3(s.rest_x);
+    if (name == "target") return vec3(s.target);
+    if (name == "dx") return vec3(s.dx);
+    if (name == "cgB") return vec3(s.cgB);
+    if (name == "diag") return m3(s.diag);
+    if (name == "offdiag") return m3(s.offdiag);
+    if (name == "faces") return u32(s.faces);
+    if (name == "edges") return u32(s.edges);
+    if (name == "hinges") return u32(s.hinges);
+    if (name == "csr") return u32(s.csr);
+    if (name == "stretch_faces") return u32(s.sf);
+    if (name == "bending_hinges") return u32(s.bh);
+    if (name == "fixed_verts") return u32(s.fixed_verts);
+    if (name == "face_area") return fl(s.face_area);
+    if (name == "edge_area") return fl(s.edge_area);
+    if (name == "vert_area") return fl(s.vert_area);
+    if (name == "vert_mass") return fl(s.vert_mass);
+    if (name == "offset") return fl(s.offset);
+    if (name == "dhat") return fl(s.dhat);
+    if (name == "sf_Dm") return fl(s.sf_Dm);
+    if (name == "sf_area") return fl(s.sf_area);
+    if (name == "sf_mu") return fl(s.sf_mu);
+    if (name == "sf_lambda") return fl(s.sf_lambda);
+    if (name == "bh_angle") return fl(s.bh_angle);
+    if (name == "bh_hbar") return fl(s.bh_hbar);
+    if (name == "bh_k") return fl(s.bh_k);
+    if (name == "is_fixed") { v = {s.is_fixed.data(), (int64_t)s.is_fixed.size(), DT_U8}; return true; }
+    if (name == "sf_off") { v = {s.sf_off.data(), (int64_t)s.sf_off.size(), DT_U16}; return true; }
+    if (name == "bh_off") { v = {s.bh_off.data(), (int64_t)s.bh_off.size(), DT_U16}; return true; }
+    if (name == "pairs_vf") return u32(s.pairs_vf);
+    if (name == "pairs_ee") return u32(s.pairs_ee);
+    auto tree = [&](Lbvh<T>& t, const std::string& f) {
+        if (f == "morton") { v = {t.morton.data(), (int64_t)t.morton.size(), DT_U64}; return true; }
+        if (f == "morton_sorted") { v = {t.morton_sorted.data(), (int64_t)t.morton_sorted.size(), DT_U64}; return true; }
+        if (f == "sorted_to_orig") return u32(t.sorted_to_orig);
+        if (f == "parent") return u32(t.parent);
+        if (f == "children") return u32(t.children);
+        if (f == "object_idx") return u32(t.object_idx);
+        if (f == "node_aabb") return fl(t.node_aabb);
+        return false;
+    };
+    if (name.rfind("face_tree.", 0) == 0) return tree(s.tree_face, name.substr(10));
+    if (name.rfind("edge_tree.", 0) == 0) return tree(s.tree_edge, name.substr(10));
+    if (name.rfind("contact_", 0) == 0)
+    {
+        cc.idx.clear(); cc.type.clear(); cc.n.clear(); cc.k.clear(); cc.w.clear(); cc.bary.clear();
+        for (auto& c : s.contacts)
+        {
+            for (int k = 0; k < 4; k++) cc.idx.push_back(c.idx[k]);
+            cc.type.push_back(c.type);
+            cc.n.push_back(c.n.x); cc.n.push_back(c.n.y); cc.n.push_back(c.n.z);
+            cc.k.push_back(c.k1); cc.k.push_back(c.k2); cc.k.push_back(c.area_kappa);
+            for (int k = 0; k < 4; k++) cc.w.push_back(c.w[k]);
+            cc.bary.push_back(c.bary0); cc.bary.push_back(c.bary1);
+        }
+        if (name == "contact_idx") return u32(cc.idx);
+        if (name == "contact_type") return u32(cc.type);
+        if (name == "contact_n") return fl(cc.n);
+        if (name == "contact_k") return fl(cc.k);
+        if (name == "contact_w") return fl(cc.w);
+        if (name == "contact_bary") return fl(cc.bary);
+    }
+    return false;
+}
+
+template <class T>
+int run_stage(Scene<T>& s, const std::string& st)
+{
+    if (st == "init") { s.init(); return 0; }
+    if (st == "frame_begin") { s.frame_begin(); return 0; }
+    if (st == "predict") { s.predict(); return 0; }
+    if (st == "iter_begin") { s.x_iter = s.x; return 0; }
+    if (st == "material") { s.material(); return 0; }
+    if (st == "update_contacts") { s.update_contacts(); return 0; }
+    if (st == "broad_phase_dcd") { return s.broad_phase(s.x, s.x) ? 0 : 1; }
+    if (st == "broad_phase_ccd") { return s.broad_phase(s.x_iter, s.x) ? 0 : 1; }
+    if (st == "dcd") { s.dcd(); return 0; }
+    if (st == "contact_assemble") { s.contact_assemble(); return 0; }
+    if (st == "pcg") { s.pcg(); return 0; }
+    if (st == "line_search") { s.line_search(); return 0; }
+    if (st == "newton_iteration") { s.newton_iteration(); return 0; }
+    if (st == "frame_end") { s.frame_end(); return 0; }
+    if (st == "step") { s.step(); return 0; }
+    if (st == "spmv_dx")
+    {
+        std::vector<V3<T>> y(s.nv);
+        s.spmv(s.dx, y);
+        s.dx = y;
+        return 0;
+    }
+    if (st == "ccd")
+    {
+        s.toi = s.ccd(s.x_iter, s.x);
+        return 0;
+    }
+    return -1;
+}
+}  // namespace
+
+extern "C"
+{
+void* orc_new(int precision)
+{
+    Handle* h = new Handle();
+    h->prec   = precision == 64 ? 64 : 32;
+    return h;
+}
+void orc_free(void* p) { delete (Handle*)p; }
+
+// params order: dt, newton_min, newton_max, pcg_max, pcg_check_every, pcg_rel_tol, pcg_abs_tol,
+// gx, gy, gz, use_floor, use_self_collision, use_ccd, floor_y, kappa, k_dirichlet, damping,
+// bending_scaling, newton_tol
+int orc_set_params(void* p, const double* a, int n)
+{
+    if (n < 19) return -1;
+    Params q;
+    q.dt = a[0]; q.newton_min = (int)a[1]; q.newton_max = (int)a[2]; q.pcg_max = (int)a[3];
+    q.pcg_check_every = std::max(1, (int)a[4]); q.pcg_rel_tol = a[5]; q.pcg_abs_tol = a[6];
+    q.gravity[0] = a[7]; q.gravity[1] = a[8]; q.gravity[2] = a[9];
+    q.use_floor = (int)a[10]; q.use_self_collision = (int)a[11]; q.use_ccd = (int)a[12];
+    q.floor_y = a[13]; q.kappa = a[14]; q.k_dirichlet = a[15]; q.damping = a[16];
+    q.bending_scaling = a[17]; q.newton_tol = a[18];
+    Handle* h = (Handle*)p;
+    h->s32.prm = q;
+    h->s64.prm = q;
+    return 0;
+}
+
+// material order: thickness, youngs, poisson, bending, density, mass, d_hat
+int orc_add_mesh(void* p, const float* x, int nv, const int32_t* tris, int nt, const double* mat, int nm)
+{
+    if (nm < 7) return -1;
+    MeshIn m;
+    m.x.assign(x, x + 3 * (size_t)nv);
+    m.tris.resize(3 * (size_t)nt);
+    for (size_t i = 0; i < m.tris.size(); i++) m.tris[i] = (uint32_t)tris[i];
+    m.mat.thickness = mat[0]; m.mat.youngs = mat[1]; m.mat.poisson = mat[2]; m.mat.bending = mat[3];
+    m.mat.density = mat[4]; m.mat.mass = mat[5]; m.mat.d_hat = mat[6];
+    Handle* h = (Handle*)p;
+    h->s32.meshes.push_back(m);
+    h->s64.meshes.push_back(m);
+    return (int)h->s32.meshes.size() - 1;
+}
+
+static Anim anim_from(const double* a, int n)
+{
+    Anim r;
+    if (n >= 17)
+    {
+        r.use_translate = (int)a[0]; r.translate[0] = a[1]; r.translate[1] = a[2]; r.translate[2] = a[3];
+        r.use_scale = (int)a[4]; r.scale[0] = a[5]; r.scale[1] = a[6]; r.scale[2] = a[7];
+        r.use_rotate = (int)a[8]; r.rot_center[0] = a[9]; r.rot_center[1] = a[10]; r.rot_center[2] = a[11];
+        r.rot_axis[0] = a[12]; r.rot_axis[1] = a[13]; r.rot_axis[2] = a[14]; r.rot_deg_per_s = a[15];
+    }
+    return r;
+}
+
+// anim order: use_translate, t[3], use_scale, s[3], use_rotate, center[3], axis[3], deg_per_s, (unused)
+int orc_pin_method(void* p, int mesh, int method, double range, const double* anim, int na)
+{
+    Handle* h = (Handle*)p;
+    if (mesh < 0 || mesh >= (int)h->s32.meshes.size()) return -1;
+    Anim a = anim_from(anim, na);
+    h->s32.pin_method(mesh, method, range, a);
+    h->s64.pin_method(mesh, method, range, a);
+    return (int)h->s32.meshes[mesh].pinned.size();
+}
+
+int orc_stage(void* p, const char* stage)
+{
+    Handle* h = (Handle*)p;
+    return h->prec == 64 ? run_stage(h->s64, stage) : run_stage(h->s32, stage);
+}
+
+// Writes element count and dtype code for a named array; returns 0 if found.
+int orc_info(void* p, const char* name, int64_t* count, int* dtype)
+{
+    Handle* h = (Handle*)p;
+    if (h->prec == 64)
+    {
+        View<double> v; ContactCache<double> cc;
+        if (!lookup(h->s64, name, v, cc)) return -1;
+        *count = v.n; *dtype = v.dt;
+    }
+    else
+    {
+        View<float> v; ContactCache<float> cc;
+        if (!lookup(h->s32, name, v, cc)) return -1;
+        *count = v.n; *dtype = v.dt;
+    }
+    return 0;
+}
+
+static int64_t dt_size(int dt) { return dt == DT_F64 || dt == DT_U64 ? 8 : dt == DT_U16 ? 2 : dt == DT_U8 ? 1 : 4; }
+
+int orc_get(void* p, const char* name, void* out, int64_t bytes)
+{
+    Handle* h = (Handle*)p;
+    auto    cp = [&](const void* src, int64_t n, int dt) {
+        int64_t b = n * dt_size(dt);
+        if (b > bytes) return -2;
+        std::memcpy(out, src, (size_t)b);
+        return 0;
+    };
+    if (h->prec == 64)
+    {
+        View<double> v; ContactCache<double> cc;
+        if (!lookup(h->s64, name, v, cc)) return -1;
+        return cp(v.p, v.n, v.dt);
+    }
+    View<float> v; ContactCache<float> cc;
+    if (!lookup(h->s32, name, v, cc)) return -1;
+    return cp(v.p, v.n, v.dt);
+}
+
+// Overwrites a named state array (same size and dtype as orc_info reports).
+int orc_set(void* p, const char* name, const void* in, int64_t bytes)
+{
+    Handle* h = (Handle*)p;
+    auto    put = [&](auto& s) -> int {
+        using S = std::remove_reference_t<decltype(s)>;
+        View<typename std::remove_reference_t<decltype(s.toi)>> v;
+        ContactCache<typename std::remove_reference_t<decltype(s.toi)>> cc;
+        (void)sizeof(S);
+        if (!lookup(s, name, v, cc)) return -1;
+        int64_t b = v.n * dt_size(v.dt);
+        if (b != bytes) return -2;
+        std::memcpy(const_cast<void*>(v.p), in, (size_t)b);
+        return 0;
+    };
+    return h->prec == 64 ? put(h->s64) : put(h->s32);
+}
+
+// Scalars of the last iteration: toi, alpha, max_dx, pin_delta, pcg_iters, newton_iters, frame,
+// converged, dirichlet_converged, pcg_rel, bp_overflow
+int orc_scalars(void* p, double* out, int n)
+{
+    Handle* h = (Handle*)p;
+    auto    f = [&](auto& s) {
+        double v[11] = {(double)s.toi, (double)s.alpha, (double)s.max_dx, (double)s.pin_delta, (double)s.pcg_iters,
+                        (double)s.newton_iters, (double)s.frame, (double)s.converged, (double)s.dirichlet_converged,
+                        (double)s.pcg_rel, (double)s.bp_overflow};
+        for (int i = 0; i < n && i < 11; i++) out[i] = v[i];
+    };
+    if (h->prec == 64) f(h->s64); else f(h->s32);
+    return 0;
+}
+
+// ---- stateless kernels for golden-vector tests -------------------------------------------
+uint32_t orc_morton32(float x, float y, float z) { return morton32(V3<float>(x, y, z)); }
+
+// raw ACCD toi for one pair; kind 0 = point-triangle (p, t0, t1, t2), 1 = edge-edge.
+double orc_accd(const double* x0, const double* x1, double offset, int kind, int precision, int* iters)
+{
+    if (precision == 64)
+    {
+        V3<double> a[4], b[4];
+        for (int k = 0; k < 4; k++)
+        {
+            a[k] = V3<double>(x0[3 * k], x0[3 * k + 1], x0[3 * k + 2]);
+            b[k] = V3<double>(x1[3 * k], x1[3 * k + 1], x1[3 * k + 2]);
+        }
+        return accd(a, b, offset, kind, iters);
+    }
+    V3<float> a[4], b[4];
+    for (int k = 0; k < 4; k++)
+    {
+        a[k] = V3<float>((float)x0[3 * k], (float)x0[3 * k + 1], (float)x0[3 * k + 2]);
+        b[k] = V3<float>((float)x1[3 * k], (float)x1[3 * k + 1], (float)x1[3 * k + 2]);
+    }
+    return accd(a, b, (float)offset, kind, iters);
+}
+
+// squared distance (kind 0: point-triangle, 1: edge-edge) and barycentrics (4 values)
+double orc_dist2(const double* x, int kind, int precision, double* coeff)
+{
+    if (precision == 64)
+    {
+        V3<double> p[4];
+        for (int k = 0; k < 4; k++) p[k] = V3<double>(x[3 * k], x[3 * k + 1], x[3 * k + 2]);
+        if (kind == 0)
+        {
+            auto c = pt_coeff_unclassified(p[0], p[1], p[2], p[3]);
+            coeff[0] = c.x; coeff[1] = c.y; coeff[2] = c.z; coeff[3] = 0;
+            return pt_dist2(p[0], p[1], p[2], p[3]);
+        }
+        auto c = ee_coeff_unclassified(p[0], p[1], p[2], p[3]);
+        for (int k = 0; k < 4; k++) coeff[k] = c.v[k];
+        return ee_dist2(p[0], p[1], p[2], p[3]);
+    }
+    V3<float> p[4];
+    for (int k = 0; k < 4; k++) p[k] = V3<float>((float)x[3 * k], (float)x[3 * k + 1], (float)x[3 * k + 2]);
+    if (kind == 0)
+    {
+        auto c = pt_coeff_unclassified(p[0], p[1], p[2], p[3]);
+        coeff[0] = c.x; coeff[1] = c.y; coeff[2] = c.z; coeff[3] = 0;
+        return pt_dist2(p[0], p[1], p[2], p[3]);
+    }
+    auto c = ee_coeff_unclassified(p[0], p[1], p[2], p[3]);
+    for (int k = 0; k < 4; k++) coeff[k] = c.v[k];
+    return ee_dist2(p[0], p[1], p[2], p[3]);
+}
+
+// barrier value and derivatives in the requested precision: out = (b, b', b'')
+void orc_barrier(double d, double dh, int precision, double* out)
+{
+    if (precision == 64)
+    {
+        out[0] = barrier(d, dh); out[1] = barrier_d1(d, dh); out[2] = barrier_d2(d, dh);
+        return;
+    }
+    float fd = (float)d, fh = (float)dh;
+    out[0] = barrier(fd, fh); out[1] = barrier_d1(fd, fh); out[2] = barrier_d2(fd, fh);
+}
+
+// Baraff-Witkin face: grad (9), H9 (81, row-major), H6 (36), energy
+void orc_bw_face(const double* x, const double* Dm, double mu, double lmd, double area, int precision,
+                 double* grad, double* H9, double* H6, double* energy)
+{
+    auto run = [&](auto tag) {
+        using T = decltype(tag);
+        V3<T> p[3];
+        for (int k = 0; k < 3; k++) p[k] = V3<T>(T(x[3 * k]), T(x[3 * k + 1]), T(x[3 * k + 2]));
+        T dm[4] = {T(Dm[0]), T(Dm[1]), T(Dm[2]), T(Dm[3])};
+        FaceGH<T> g;
+        bw_face(p[0], p[1], p[2], dm, T(mu), T(lmd), T(area), g);
+        for (int a = 0; a < 3; a++)
+            for (int d = 0; d < 3; d++) grad[3 * a + d] = g.g[a][d];
+        for (int i = 0; i < 9; i++)
+            for (int j = 0; j < 9; j++) H9[9 * i + j] = g.H[i][j];
+        for (int i = 0; i < 6; i++)
+            for (int j = 0; j < 6; j++) H6[6 * i + j] = g.H6[i][j];
+        *energy = bw_energy(p[0], p[1], p[2], dm, T(mu), T(lmd), T(area));
+    };
+    if (precision == 64) run(double()); else run(float());
+}
+
+void orc_dm_inv(const double* x, int precision, double* out)
+{
+    if (precision == 64)
+    {
+        double d[4];
+        dm_inv(V3<double>(x[0], x[1], x[2]), V3<double>(x[3], x[4], x[5]), V3<double>(x[6], x[7], x[8]), d);
+        for (int k = 0; k < 4; k++) out[k] = d[k];
+        return;
+    }
+    float d[4];
+    dm_inv(V3<float>((float)x[0], (float)x[1], (float)x[2]), V3<float>((float)x[3], (float)x[4], (float)x[5]),
+           V3<float>((float)x[6], (float)x[7], (float)x[8]), d);
+    for (int k = 0; k < 4; k++) out[k] = d[k];
+}
+
+// dihedral angle and its gradient (12) for a hinge (p0, p1 edge; p2, p3 opposite)
+double orc_dihedral(const double* x, int precision, double* grad)
+{
+    auto run = [&](auto tag) -> double {
+        using T = decltype(tag);
+        V3<T> p[4];
+        for (int k = 0; k < 4; k++) p[k] = V3<T>(T(x[3 * k]), T(x[3 * k + 1]), T(x[3 * k + 2]));
+        V3<T> g[4];
+        dihedral_grad(p[0], p[1], p[2], p[3], g);
+        for (int k = 0; k < 4; k++)
+            for (int d = 0; d < 3; d++) grad[3 * k + d] = g[k][d];
+        return dihedral_angle(p[0], p[1], p[2], p[3]);
+    };
+    return precision == 64 ? run(double()) : run(float());
+}
+
+// Standalone LBVH over faces (arity 3) or edges (arity 2): build at x, refit on (x0, x1) with
+// per-vertex pad, then query. Outputs go to caller buffers sized for n leaves.
+void* orc_lbvh_new(int type, const uint32_t* prims, uint32_t n)
+{
+    auto* t = new Lbvh<float>();
+    t->init(type, prims, n);
+    return t;
+}
+void orc_lbvh_free(void* t) { delete (Lbvh<float>*)t; }
+int orc_lbvh_construct(void* t, const float* x, uint32_t n_verts)
+{
+    ((Lbvh<float>*)t)->construct((const V3<float>*)x, n_verts);
+    return ((Lbvh<float>*)t)->healthy ? 0 : 1;
+}
+void orc_lbvh_refit(void* t, const float* x0, const float* x1, const float* pad)
+{
+    ((Lbvh<float>*)t)->refit((const V3<float>*)x0, (const V3<float>*)x1, pad);
+}
+int orc_lbvh_get(void* t, const char* name, void* out, int64_t bytes)
+{
+    auto*       b = (Lbvh<float>*)t;
+    std::string f = name;
+    const void* src = nullptr;
+    int64_t     nb  = 0;
+    if (f == "morton") { src = b->morton.data(); nb = b->morton.size() * 8; }
+    else if (f == "morton_sorted") { src = b->morton_sorted.data(); nb = b->morton_sorted.size() * 8; }
+    else if (f == "sorted_to_orig") { src = b->sorted_to_orig.data(); nb = b->sorted_to_orig.size() * 4; }
+    else if (f == "parent") { src = b->parent.data(); nb = b->parent.size() * 4; }
+    else if (f == "children") { src = b->children.data(); nb = b->children.size() * 4; }
+    else if (f == "object_idx") { src = b->object_idx.data(); nb = b->object_idx.size() * 4; }
+    else if (f == "node_aabb") { src = b->node_aabb.data(); nb = b->node_aabb.size() * 4; }
+    else if (f == "global_aabb")
+    {
+        float g[6] = {b->global_lo.x, b->global_lo.y, b->global_lo.z, b->global_hi.x, b->global_hi.y, b->global_hi.z};
+        if (bytes < 24) return -2;
+        std::memcpy(out, g, 24);
+        return 0;
+    }
+    else return -1;
+    if (nb > bytes) return -2;
+    std::memcpy(out, src, (size_t)nb);
+    return 0;
+}
+// Query boxes (6 floats each: lo, hi); edge_filter keeps self < leaf. Returns pair count (or
+// -(count) - 1 if any stack overflowed); pairs written up to cap.
+int64_t orc_lbvh_query(void* t, const float* boxes, uint32_t nq, int edge_filter, uint32_t* pairs, int64_t cap)
+{
+    auto*   b  = (Lbvh<float>*)t;
+    int64_t k  = 0;
+    bool    ok = true;
+    for (uint32_t q = 0; q < nq; q++)
+    {
+        AABB3<float> bx = {V3<float>(boxes[6 * q], boxes[6 * q + 1], boxes[6 * q + 2]),
+                           V3<float>(boxes[6 * q + 3], boxes[6 * q + 4], boxes[6 * q + 5])};
+        ok &= b->query(bx, q, edge_filter != 0, [&](uint32_t a, uint32_t o) {
+            if (k < cap)
+            {
+                pairs[2 * k]     = a;
+                pairs[2 * k + 1] = o;
+            }
+            k++;
+        });
+    }
+    return ok ? k : -k - 1;
+}
+}

@@ -1,0 +1,952 @@
+// Oracle: scene setup and the Newton step of the cloth simulator, restated on the CPU.
+//
+// TEST INFRASTRUCTURE ONLY (see orc_math.h). Restates:
+//   Solver/MeshOperation/mesh_reader.cpp:219-352        edges (sorted unique) and hinges
+//   Solver/Initializer/init_mesh_data.cpp:1-100,117-260 pinned vertices from normalised positions
+//   Solver/Initializer/init_mesh_data.h:56-93           pinned target animation (scale, rotate, translate)
+//   Solver/Initializer/init_mesh_data.cpp:339-1011      areas, per-vertex mass, thickness
+//   Solver/Initializer/init_sim_data.cpp:19-48,189-1156 culled elements, constants, CSR adjacency, offsets
+//   Solver/Initializer/init_collision_data.cpp:8-174    per-vertex contact offset and d_hat clamp
+//   Solver/SimulationSolver/newton_solver.cpp:577-636   predict / update velocity
+//   Solver/SimulationSolver/newton_solver.cpp:705-1172  inertia, ground, stretch, bending and assembly
+//   Solver/CollisionDetector/narrow_phase.cpp:293-1098  VF/EE CCD and DCD
+//   Solver/CollisionDetector/narrow_phase.cpp:2550-2775 per-pair contact gradient/diagonal and SpMV
+//   Solver/LinearSolver/precond_cg.cpp:24-331,626-771   block-Jacobi PCG
+//   Solver/SimulationSolver/newton_solver.cpp:3174-3427,3685-3884,4051-4345 driver and line search
+//   Solver/SimulationSolver/solver_interface.cpp:130-183 per-frame pre/post operations
+#pragma once
+#include <map>
+#include <string>
+#include <vector>
+#include "orc_collision.h"
+#include "orc_energy.h"
+#include "orc_lbvh.h"
+
+namespace orc
+{
+
+struct Material
+{
+    double thickness = 2e-3, youngs = 1e5, poisson = 0.25, bending = 5e-3, density = 1e3, mass = 0.0, d_hat = 2e-3;
+};
+struct Anim
+{
+    int    use_translate = 0, use_scale = 0, use_rotate = 0;
+    double translate[3] = {0, 0, 0}, scale[3] = {1, 1, 1}, rot_center[3] = {0, 0, 0}, rot_axis[3] = {1, 0, 0};
+    double rot_deg_per_s = 0;
+};
+struct MeshIn
+{
+    std::vector<float>    x;     // world-space rest positions
+    std::vector<uint32_t> tris;  // local indices
+    Material              mat;
+    std::vector<uint32_t> pinned;  // local vertex ids in selection order
+    std::vector<Anim>     pin_anim;
+};
+struct Params
+{
+    double dt = 1.0 / 60.0;
+    int    newton_min = 3, newton_max = 100, pcg_max = 100, pcg_check_every = 1;
+    double pcg_rel_tol = 1e-4, pcg_abs_tol = 1e-10;
+    double gravity[3] = {0, -9.8, 0};
+    int    use_floor = 1, use_self_collision = 1, use_ccd = 1;
+    double floor_y = 0.0, kappa = 1e7, k_dirichlet = 1e9, damping = 2.0, bending_scaling = 1.0, newton_tol = 1e-2;
+};
+
+template <class T>
+struct Scene
+{
+    using V = V3<T>;
+    Params              prm;
+    std::vector<MeshIn> meshes;
+
+    // ---- mesh data (a1)
+    uint32_t              nv = 0, nf = 0, ne = 0, nh = 0;
+    std::vector<V>        rest_x;
+    std::vector<uint32_t> faces, edges, hinges;  // 3/2/4 per element
+    std::vector<uint32_t> vert_mesh, face_mesh, edge_mesh, hinge_mesh;
+    std::vector<T>        face_area, edge_area, vert_area, vert_mass;
+    std::vector<uint8_t>  is_fixed;
+    std::vector<T>        offset, dhat;  // per vertex
+    std::vector<uint32_t> fixed_verts;   // in pin order
+    std::vector<Anim>     fixed_anim;
+    std::vector<V>        target;        // pinned targets (all verts; only fixed used)
+    // ---- sim data
+    std::vector<uint32_t> sf;  // stretch faces (culled)
+    std::vector<T>        sf_Dm, sf_area, sf_mu, sf_lambda;
+    std::vector<uint32_t> bh;  // bending hinges (culled)
+    std::vector<T>        bh_angle, bh_hbar, bh_k;
+    std::vector<uint32_t> csr;              // reference layout: n+1 prefixes (offset by n+1), then columns
+    std::vector<uint16_t> sf_off, bh_off;   // offsets_in_adjlist (6 per face, 12 per hinge)
+    // ---- state
+    std::vector<V> x, v, x_step, x_iter, x_tilde, x_outer, v_outer, dx;
+    std::vector<V> cgB;
+    std::vector<M3<T>> diag, offdiag;  // offdiag indexed by CSR slot (row-major, header slots unused)
+    Lbvh<T>            tree_face, tree_edge;
+    // ---- contacts
+    std::vector<uint32_t> pairs_vf, pairs_ee;  // broad-phase candidate lists (q, leaf)
+    bool                  bp_overflow = false;
+    struct Contact
+    {
+        uint32_t idx[4];
+        uint32_t type;  // 4 = VF, 8 = EE
+        V        n;
+        T        k1, k2, area_kappa, bary0, bary1;
+        T        w[4];
+    };
+    std::vector<Contact> contacts;
+    // ---- per-iteration stats
+    T   toi = 1, alpha = 1, max_dx = 0, pin_delta = 0;
+    int pcg_iters = 0, newton_iters = 0, frame = 0;
+    T   pcg_rel = 0;
+    bool converged = false, dirichlet_converged = false;
+    std::vector<int> iters_log;
+
+    // ============================================================ setup
+    static void extract_edges(const std::vector<uint32_t>& tris, std::vector<uint32_t>& E, std::vector<uint32_t>& H)
+    {
+        const size_t nt = tris.size() / 3;
+        struct Tri3
+        {
+            uint32_t a, b, f;
+        };
+        std::vector<Tri3> tmp(3 * nt);
+        for (size_t f = 0; f < nt; f++)
+        {
+            uint32_t s[3] = {tris[3 * f], tris[3 * f + 1], tris[3 * f + 2]};
+            std::sort(s, s + 3);
+            tmp[3 * f + 0] = {s[0], s[1], (uint32_t)f};
+            tmp[3 * f + 1] = {s[0], s[2], (uint32_t)f};
+            tmp[3 * f + 2] = {s[1], s[2], (uint32_t)f};
+        }
+        std::sort(tmp.begin(), tmp.end(), [](const Tri3& l, const Tri3& r) {
+            if (l.a != r.a) return l.a < r.a;
+            if (l.b != r.b) return l.b < r.b;
+            return l.f < r.f;
+        });
+        std::vector<uint8_t> kind(tmp.size(), 0);
+        for (size_t i = 0; i < tmp.size(); i++)
+        {
+            if (i + 1 < tmp.size() && tmp[i + 1].a == tmp[i].a && tmp[i + 1].b == tmp[i].b) kind[i] = 1;
+            if (i > 0 && tmp[i - 1].a == tmp[i].a && tmp[i - 1].b == tmp[i].b) kind[i] = 2;
+        }
+        E.clear();
+        H.clear();
+        for (size_t i = 0; i < tmp.size(); i++)
+        {
+            if (kind[i] == 0 || kind[i] == 2)
+            {
+                E.push_back(tmp[i].a);
+                E.push_back(tmp[i].b);
+            }
+        }
+        for (size_t i = 0; i < tmp.size(); i++)
+        {
+            if (kind[i] == 1)
+            {
+                const uint32_t* cf  = &tris[3 * tmp[i].f];
+                const uint32_t* nf_ = &tris[3 * tmp[i + 1].f];
+                uint32_t        e0 = tmp[i].a, e1 = tmp[i].b;
+                uint32_t        c   = (cf[0] + cf[1] + cf[2]) - (e0 + e1);
+                uint32_t        n   = (nf_[0] + nf_[1] + nf_[2]) - (e0 + e1);
+                H.push_back(e0);
+                H.push_back(e1);
+                H.push_back(c);
+                H.push_back(n);
+            }
+        }
+    }
+
+    // FixedPointsType selection on normalised model positions (init_mesh_data.cpp:59-95,117-240).
+    void pin_method(int mesh, int method, double range, const Anim& an)
+    {
+        MeshIn&  m = meshes[mesh];
+        size_t   n = m.x.size() / 3;
+        float    lo[3] = {3.4e38f, 3.4e38f, 3.4e38f}, hi[3] = {-3.4e38f, -3.4e38f, -3.4e38f};
+        for (size_t i = 0; i < n; i++)
+            for (int d = 0; d < 3; d++)
+            {
+                lo[d] = std::min(lo[d], m.x[3 * i + d]);
+                hi[d] = std::max(hi[d], m.x[3 * i + d]);
+            }
+        float inv[3];
+        for (int d = 0; d < 3; d++) inv[d] = 1.0f / std::max(hi[d] - lo[d], 0.0001f);
+        float r = (float)range;
+        for (size_t i = 0; i < n; i++)
+        {
+            float p[3];
+            for (int d = 0; d < 3; d++) p[d] = (m.x[3 * i + d] - lo[d]) * inv[d];
+            bool s = false;
+            switch (method)
+            {
+                case 0: s = false; break;                                  // None
+                case 3: s = p[0] < r; break;                               // Left
+                case 4: s = p[0] > 1.0f - r; break;                        // Right
+                case 5: s = p[2] < r; break;                               // Front
+                case 6: s = p[2] > 1.0f - r; break;                        // Back
+                case 7: s = p[1] > 1.0f - r; break;                        // Up
+                case 8: s = p[1] < r; break;                               // Down
+                case 9: s = p[0] < r && p[1] > 1.0f - r; break;            // LeftUp
+                case 10: s = p[0] < r && p[1] < r; break;                  // LeftDown
+                case 11: s = p[0] < r && p[2] > 1.0f - r; break;           // LeftFront
+                case 12: s = p[0] < r && p[2] < r; break;                  // LeftBack
+                case 13: s = p[0] > 1.0f - r && p[1] > 1.0f - r; break;    // RightUp
+                case 14: s = p[0] > 1.0f - r && p[1] < r; break;           // RightDown
+                case 15: s = p[0] > 1.0f - r && p[2] > 1.0f - r; break;    // RightFront
+                case 16: s = p[0] > 1.0f - r && p[2] < r; break;           // RightBack
+                case 17: s = p[2] < r && p[1] > 1.0f - r; break;           // FrontUp
+                case 18: s = p[2] < r && p[1] < r; break;                  // FrontDown
+                case 19: s = p[2] > 1.0f - r && p[1] > 1.0f - r; break;    // BackUp
+                case 20: s = p[2] > 1.0f - r && p[1] < r; break;           // BackDown
+                case 21: s = true; break;                                  // All
+                default: s = false;
+            }
+            if (s)
+            {
+                m.pinned.push_back((uint32_t)i);
+                m.pin_anim.push_back(an);
+            }
+        }
+    }
+
+    // init_mesh_data.h:56-93 with luisa::rotation's axis-angle matrix.
+    static V3<float> animate(const Anim& a, float t, V3<float> p)
+    {
+        if (a.use_scale) p = V3<float>((float)a.scale[0] * t * p.x, (float)a.scale[1] * t * p.y, (float)a.scale[2] * t * p.z);
+        if (a.use_rotate)
+        {
+            float     ang = t * (float)a.rot_deg_per_s / 180.0f * 3.14159265358979323846f;
+            V3<float> c((float)a.rot_center[0], (float)a.rot_center[1], (float)a.rot_center[2]);
+            V3<float> rel = p - c;
+            if (ang != 0.0f)
+            {
+                V3<float> ax  = normalize(V3<float>((float)a.rot_axis[0], (float)a.rot_axis[1], (float)a.rot_axis[2]));
+                float     co  = std::cos(ang), si = std::sin(ang);
+                V3<float> tt  = (1.0f - co) * ax;
+                V3<float> c0(co + tt.x * ax.x, tt.x * ax.y + si * ax.z, tt.x * ax.z - si * ax.y);
+                V3<float> c1(tt.y * ax.x - si * ax.z, co + tt.y * ax.y, tt.y * ax.z + si * ax.x);
+                V3<float> c2(tt.z * ax.x + si * ax.y, tt.z * ax.y - si * ax.x, co + tt.z * ax.z);
+                rel = (c0 * rel.x + c1 * rel.y) + c2 * rel.z;
+            }
+            p = c + rel;
+        }
+        if (a.use_translate) p = p + V3<float>((float)a.translate[0] * t, (float)a.translate[1] * t, (float)a.translate[2] * t);
+        return p;
+    }
+
+    void update_targets(float time)
+    {
+        for (size_t k = 0; k < fixed_verts.size(); k++)
+        {
+            uint32_t  vid = fixed_verts[k];
+            V3<float> r((float)rest_x[vid].x, (float)rest_x[vid].y, (float)rest_x[vid].z);
+            V3<float> t = animate(fixed_anim[k], time, r);
+            target[vid] = V(T(t.x), T(t.y), T(t.z));
+        }
+    }
+
+    void init()
+    {
+        nv = nf = ne = nh = 0;
+        rest_x.clear(); faces.clear(); edges.clear(); hinges.clear();
+        vert_mesh.clear(); face_mesh.clear(); edge_mesh.clear(); hinge_mesh.clear();
+        is_fixed.clear(); fixed_verts.clear(); fixed_anim.clear();
+        std::vector<float> rest_f;
+        for (size_t mi = 0; mi < meshes.size(); mi++)
+        {
+            const MeshIn&         m  = meshes[mi];
+            uint32_t              bv = nv;
+            std::vector<uint32_t> E, H;
+            extract_edges(m.tris, E, H);
+            size_t mv = m.x.size() / 3;
+            for (size_t i = 0; i < mv; i++)
+            {
+                rest_f.push_back(m.x[3 * i]); rest_f.push_back(m.x[3 * i + 1]); rest_f.push_back(m.x[3 * i + 2]);
+                vert_mesh.push_back((uint32_t)mi);
+                is_fixed.push_back(0);
+            }
+            for (size_t i = 0; i < m.tris.size(); i++) faces.push_back(bv + m.tris[i]);
+            for (size_t i = 0; i < m.tris.size() / 3; i++) face_mesh.push_back((uint32_t)mi);
+            for (size_t i = 0; i < E.size(); i++) edges.push_back(bv + E[i]);
+            for (size_t i = 0; i < E.size() / 2; i++) edge_mesh.push_back((uint32_t)mi);
+            for (size_t i = 0; i < H.size(); i++) hinges.push_back(bv + H[i]);
+            for (size_t i = 0; i < H.size() / 4; i++) hinge_mesh.push_back((uint32_t)mi);
+            for (size_t k = 0; k < m.pinned.size(); k++)
+            {
+                is_fixed[bv + m.pinned[k]] = 1;
+                fixed_verts.push_back(bv + m.pinned[k]);
+                fixed_anim.push_back(m.pin_anim[k]);
+            }
+            nv += (uint32_t)mv;
+        }
+        nf = (uint32_t)(faces.size() / 3);
+        ne = (uint32_t)(edges.size() / 2);
+        nh = (uint32_t)(hinges.size() / 4);
+        rest_x.resize(nv);
+        std::vector<V3<float>> rxf(nv);
+        for (uint32_t i = 0; i < nv; i++)
+        {
+            rxf[i]    = V3<float>(rest_f[3 * i], rest_f[3 * i + 1], rest_f[3 * i + 2]);
+            rest_x[i] = V(T(rxf[i].x), T(rxf[i].y), T(rxf[i].z));
+        }
+        // ---- areas, masses (init_mesh_data.cpp:783-995), computed in float as the reference
+        std::vector<float> farea(nf), earea(ne, 0.0f), varea(nv), vvol(nv);
+        std::vector<std::vector<uint32_t>> v_faces(nv);
+        for (uint32_t f = 0; f < nf; f++)
+        {
+            const uint32_t* t = &faces[3 * f];
+            farea[f]           = length(cross(rxf[t[1]] - rxf[t[0]], rxf[t[2]] - rxf[t[0]])) * 0.5f;
+            for (int j = 0; j < 3; j++) v_faces[t[j]].push_back(f);
+        }
+        {
+            // edge_adj_faces: faces containing both endpoints, in face order
+            std::map<std::pair<uint32_t, uint32_t>, uint32_t> eid;
+            for (uint32_t e = 0; e < ne; e++) eid[{edges[2 * e], edges[2 * e + 1]}] = e;
+            std::vector<double> acc(ne, 0.0);
+            for (uint32_t f = 0; f < nf; f++)
+            {
+                const uint32_t* t = &faces[3 * f];
+                for (int j = 0; j < 3; j++)
+                {
+                    uint32_t a = t[j], b = t[(j + 1) % 3];
+                    auto     it = eid.find({std::min(a, b), std::max(a, b)});
+                    if (it != eid.end()) acc[it->second] += farea[f] / 3.0;
+                }
+            }
+            for (uint32_t e = 0; e < ne; e++) earea[e] = (float)acc[e];
+        }
+        for (uint32_t i = 0; i < nv; i++)
+        {
+            double a = 0.0;
+            for (uint32_t f : v_faces[i]) a += farea[f] / 3.0;
+            varea[i] = (float)a;
+            vvol[i]  = varea[i] * (float)meshes[vert_mesh[i]].mat.thickness;
+        }
+        std::vector<float> body_vol(meshes.size(), 0.0f), body_mass(meshes.size(), 0.0f);
+        for (size_t mi = 0; mi < meshes.size(); mi++)
+        {
+            float s = 0.0f;
+            for (uint32_t f = 0; f < nf; f++)
+                if (face_mesh[f] == mi) s += farea[f] * (float)meshes[mi].mat.thickness;
+            body_vol[mi]  = s;
+            body_mass[mi] = meshes[mi].mat.mass != 0.0 ? (float)meshes[mi].mat.mass : s * (float)meshes[mi].mat.density;
+        }
+        face_area.resize(nf); edge_area.resize(ne); vert_area.resize(nv); vert_mass.resize(nv);
+        for (uint32_t f = 0; f < nf; f++) face_area[f] = T(farea[f]);
+        for (uint32_t e = 0; e < ne; e++) edge_area[e] = T(earea[e]);
+        for (uint32_t i = 0; i < nv; i++)
+        {
+            vert_area[i]  = T(varea[i]);
+            float w       = vvol[i] / body_vol[vert_mesh[i]];
+            vert_mass[i]  = T(w * body_mass[vert_mesh[i]]);
+        }
+        // ---- contact offset / d_hat (init_collision_data.cpp:71-174)
+        std::vector<float> min_dist(meshes.size(), 1e10f);
+        for (uint32_t e = 0; e < ne; e++)
+        {
+            float l = length(rxf[edges[2 * e]] - rxf[edges[2 * e + 1]]);
+            min_dist[edge_mesh[e]] = std::min(min_dist[edge_mesh[e]], l);
+        }
+        for (uint32_t f = 0; f < nf; f++)
+        {
+            const uint32_t* t = &faces[3 * f];
+            float           rd = 10000.0f;
+            for (int i = 0; i < 3; i++)
+            {
+                V3<float> v0 = rxf[t[i]], v1 = rxf[t[(i + 1) % 3]], v2 = rxf[t[(i + 2) % 3]];
+                float     c0, c1;
+                pe_coeff(v0, v1, v2, c0, c1);
+                rd = std::min(rd, length((c0 * v1 + c1 * v2) - v0));
+            }
+            min_dist[face_mesh[f]] = std::min(min_dist[face_mesh[f]], rd);
+        }
+        offset.resize(nv); dhat.resize(nv);
+        std::vector<float> moff(meshes.size()), mdh(meshes.size());
+        for (size_t mi = 0; mi < meshes.size(); mi++)
+        {
+            float so = 0.5f * (float)meshes[mi].mat.thickness, sd = (float)meshes[mi].mat.d_hat;
+            float dh = sd;
+            float safe = 0.9f * min_dist[mi];
+            if (!(so + dh < safe))
+            {
+                if (so > 0.5f * safe) so = 0.5f * safe;
+                if (sd > safe - so) sd = safe - so;
+            }
+            moff[mi] = so;
+            mdh[mi]  = sd;
+        }
+        for (uint32_t i = 0; i < nv; i++)
+        {
+            offset[i] = T(moff[vert_mesh[i]]);
+            dhat[i]   = T(mdh[vert_mesh[i]]);
+        }
+        // ---- culled elements and their constants (init_sim_data.cpp:189-480)
+        sf.clear(); sf_Dm.clear(); sf_area.clear(); sf_mu.clear(); sf_lambda.clear();
+        for (uint32_t f = 0; f < nf; f++)
+        {
+            const uint32_t* t = &faces[3 * f];
+            if (is_fixed[t[0]] && is_fixed[t[1]] && is_fixed[t[2]]) continue;
+            sf.push_back(t[0]); sf.push_back(t[1]); sf.push_back(t[2]);
+            float d[4];
+            dm_inv(rxf[t[0]], rxf[t[1]], rxf[t[2]], d);
+            for (int k = 0; k < 4; k++) sf_Dm.push_back(T(d[k]));
+            sf_area.push_back(T(length(cross(rxf[t[1]] - rxf[t[0]], rxf[t[2]] - rxf[t[0]])) * 0.5f));
+            const Material& mt = meshes[face_mesh[f]].mat;
+            double mu = (float)mt.youngs / (2.0 * (1.0 + (float)mt.poisson));
+            double la = (float)mt.youngs * (float)mt.poisson / ((1.0 + (float)mt.poisson) * (1.0 - 2.0 * (float)mt.poisson));
+            mu = (float)mt.thickness * mu;
+            la = (float)mt.thickness * la;
+            sf_mu.push_back(T((float)mu));
+            sf_lambda.push_back(T((float)la));
+        }
+        bh.clear(); bh_angle.clear(); bh_hbar.clear(); bh_k.clear();
+        for (uint32_t h = 0; h < nh; h++)
+        {
+            const uint32_t* e = &hinges[4 * h];
+            if (is_fixed[e[0]] && is_fixed[e[1]] && is_fixed[e[2]] && is_fixed[e[3]]) continue;
+            for (int k = 0; k < 4; k++) bh.push_back(e[k]);
+            V3<float> p0 = rxf[e[0]], p1 = rxf[e[1]], p2 = rxf[e[2]], p3 = rxf[e[3]];
+            float     ang = dihedral_angle(p0, p1, p2, p3);
+            float     A1  = length(cross(p1 - p0, p2 - p0)) * 0.5f;
+            float     A2  = length(cross(p1 - p0, p3 - p0)) * 0.5f;
+            float     L0  = length(p0 - p1);
+            bh_angle.push_back(T(ang));
+            bh_hbar.push_back(T((A1 + A2) / (3.0f * L0)));
+            bh_k.push_back(T((float)meshes[hinge_mesh[h]].mat.bending));
+        }
+        // ---- CSR adjacency of material force vertices (init_sim_data.cpp:760-896)
+        std::vector<std::vector<uint32_t>> adj(nv);
+        auto ins = [&](uint32_t a, uint32_t b) {
+            auto& l = adj[a];
+            if (std::find(l.begin(), l.end(), b) == l.end()) l.push_back(b);
+        };
+        for (size_t f = 0; f < sf.size() / 3; f++)
+            for (int i = 0; i < 3; i++)
+                for (int j = 0; j < 3; j++)
+                    if (i != j) ins(sf[3 * f + i], sf[3 * f + j]);
+        for (size_t h = 0; h < bh.size() / 4; h++)
+            for (int i = 0; i < 4; i++)
+                for (int j = 0; j < 4; j++)
+                    if (i != j) ins(bh[4 * h + i], bh[4 * h + j]);
+        for (auto& l : adj) std::sort(l.begin(), l.end());
+        csr.assign(nv + 1, 0);
+        uint32_t cur = nv + 1;
+        for (uint32_t i = 0; i < nv; i++)
+        {
+            csr[i] = cur;
+            cur += (uint32_t)adj[i].size();
+        }
+        csr[nv] = cur;
+        for (uint32_t i = 0; i < nv; i++)
+            for (uint32_t c : adj[i]) csr.push_back(c);
+        auto offs = [&](const uint32_t* el, int N, std::vector<uint16_t>& out) {
+            for (int i = 0; i < N; i++)
+                for (int j = 0; j < N; j++)
+                    if (i != j)
+                    {
+                        const auto& l = adj[el[i]];
+                        out.push_back((uint16_t)(std::find(l.begin(), l.end(), el[j]) - l.begin()));
+                    }
+        };
+        sf_off.clear(); bh_off.clear();
+        for (size_t f = 0; f < sf.size() / 3; f++) offs(&sf[3 * f], 3, sf_off);
+        for (size_t h = 0; h < bh.size() / 4; h++) offs(&bh[4 * h], 4, bh_off);
+        // ---- state (restart_system, solver_interface.cpp:184-205)
+        x.assign(rest_x.begin(), rest_x.end());
+        x_outer = rest_x;
+        v_outer.assign(nv, V());
+        v.assign(nv, V());
+        x_step = x_iter = x_tilde = dx = x;
+        target = rest_x;
+        cgB.assign(nv, V());
+        diag.assign(nv, M3<T>());
+        offdiag.assign(csr.size(), M3<T>());
+        tree_face.init(TreeFace, faces.data(), nf);
+        tree_edge.init(TreeEdge, edges.data(), ne);
+        frame = 0;
+    }
+
+    // ============================================================ per-frame pieces
+    // solver_interface.cpp:130-162 (pinned velocity from target) + app_simulation.cpp:114-123
+    void frame_begin()
+    {
+        update_targets(float(frame) * float(prm.dt));
+        for (uint32_t vid : fixed_verts) v_outer[vid] = (target[vid] - x_outer[vid]) / T(float(prm.dt));
+        for (uint32_t i = 0; i < nv; i++)
+        {
+            x_step[i] = x_outer[i];
+            x[i]      = x_outer[i];
+            v[i]      = v_outer[i];
+        }
+        if (nf > 1) tree_face.construct(x_step.data(), nv);
+        if (ne > 1) tree_edge.construct(x_step.data(), nv);
+    }
+
+    // newton_solver.cpp:577-608
+    void predict()
+    {
+        const T   h = T(float(prm.dt));
+        const V   g(T(float(prm.gravity[0])), T(float(prm.gravity[1])), T(float(prm.gravity[2])));
+        for (uint32_t i = 0; i < nv; i++)
+        {
+            V vp = v[i] + h * g;
+            if (is_fixed[i]) vp = v[i];
+            x_iter[i]  = x_step[i];
+            x_tilde[i] = x_step[i] + h * vp;
+            x[i]       = x_step[i];
+        }
+    }
+
+    // newton_solver.cpp:819-950, 1053-1172 and the assembly template 705-814.
+    void material()
+    {
+        const T h = T(float(prm.dt)), h2inv = T(1) / (h * h), kD = T(float(prm.k_dirichlet));
+        for (uint32_t i = 0; i < nv; i++)
+        {
+            T     m    = vert_mass[i];
+            V     grad = ((-m) * h2inv) * (x[i] - x_tilde[i]);
+            M3<T> hess = M3<T>::diag((T(1) * m) * h2inv);
+            if (is_fixed[i])
+            {
+                grad = grad + kD * grad;
+                hess = hess + kD * hess;
+            }
+            cgB[i]  = grad;
+            diag[i] = hess;
+        }
+        if (prm.use_floor)
+        {
+            const T kap = T(float(prm.kappa)), fy = T(float(prm.floor_y));
+            for (uint32_t i = 0; i < nv; i++)
+            {
+                if (is_fixed[i]) continue;
+                T dh = dhat[i], th = offset[i], dist = x[i].y - fy;
+                if (dist - th < dh)
+                {
+                    T stiff = kap * vert_area[i];
+                    T k1    = stiff * barrier_d1(dist - th, dh);
+                    T k2    = stiff * barrier_d2(dist - th, dh);
+                    cgB[i].y        = cgB[i].y - k1;
+                    diag[i].m[1][1] = diag[i].m[1][1] + k2;
+                }
+            }
+        }
+        std::fill(offdiag.begin(), offdiag.end(), M3<T>());
+        // element evaluation into staging, then per-vertex gather in element order (owner computes)
+        const size_t          nsf = sf.size() / 3, nbh = bh.size() / 4;
+        std::vector<FaceGH<T>> fgh(nsf);
+        for (size_t f = 0; f < nsf; f++)
+        {
+            const uint32_t* t = &sf[3 * f];
+            bw_face(x[t[0]], x[t[1]], x[t[2]], &sf_Dm[4 * f], sf_mu[f], sf_lambda[f], sf_area[f], fgh[f]);
+        }
+        struct HingeGH
+        {
+            V     g[4];
+            T     stiff;
+            T     coef;  // stiff * (theta - rest)
+        };
+        std::vector<HingeGH> hgh(nbh);
+        const T scaling = T(float(prm.bending_scaling));
+        for (size_t e = 0; e < nbh; e++)
+        {
+            const uint32_t* q = &bh[4 * e];
+            V               g[4];
+            T               ang = dihedral_angle(x[q[0]], x[q[1]], x[q[2]], x[q[3]]);
+            dihedral_grad(x[q[0]], x[q[1]], x[q[2]], x[q[3]], g);
+            T d     = ang - bh_angle[e];
+            T stiff = (bh_k[e] * scaling) * bh_hbar[e];
+            for (int k = 0; k < 4; k++) hgh[e].g[k] = g[k];
+            hgh[e].stiff = stiff;
+            hgh[e].coef  = stiff * d;
+        }
+        std::vector<std::vector<uint32_t>> v_sf(nv), v_bh(nv);
+        for (size_t f = 0; f < nsf; f++)
+            for (int j = 0; j < 3; j++) v_sf[sf[3 * f + j]].push_back((uint32_t)f);
+        for (size_t e = 0; e < nbh; e++)
+            for (int j = 0; j < 4; j++) v_bh[bh[4 * e + j]].push_back((uint32_t)e);
+        for (uint32_t i = 0; i < nv; i++)
+        {
+            const uint32_t base = csr[i];
+            for (uint32_t f : v_sf[i])
+            {
+                int a = 0;
+                for (int k = 0; k < 3; k++)
+                    if (sf[3 * f + k] == i) a = k;
+                const FaceGH<T>& G = fgh[f];
+                cgB[i]             = cgB[i] + (-G.g[a]);
+                M3<T> Hd;
+                for (int r = 0; r < 3; r++)
+                    for (int c = 0; c < 3; c++) Hd.m[r][c] = G.H[3 * a + r][3 * a + c];
+                diag[i] = diag[i] + Hd;
+                int ii  = 0;
+                for (int b = 0; b < 3; b++)
+                {
+                    if (b == a) continue;
+                    M3<T> Hb;
+                    for (int r = 0; r < 3; r++)
+                        for (int c = 0; c < 3; c++) Hb.m[r][c] = G.H[3 * a + r][3 * b + c];
+                    uint32_t slot = base + sf_off[6 * f + 2 * a + ii];
+                    offdiag[slot] = offdiag[slot] + Hb;
+                    ii++;
+                }
+            }
+            for (uint32_t e : v_bh[i])
+            {
+                int a = 0;
+                for (int k = 0; k < 4; k++)
+                    if (bh[4 * e + k] == i) a = k;
+                const HingeGH& G = hgh[e];
+                cgB[i]           = cgB[i] + (-(G.coef * G.g[a]));
+                diag[i]          = diag[i] + G.stiff * outer(G.g[a], G.g[a]);
+                int ii           = 0;
+                for (int b = 0; b < 4; b++)
+                {
+                    if (b == a) continue;
+                    uint32_t slot = base + bh_off[12 * e + 3 * a + ii];
+                    offdiag[slot] = offdiag[slot] + G.stiff * outer(G.g[a], G.g[b]);
+                    ii++;
+                }
+            }
+        }
+    }
+
+    // ---- broad phase (newton_solver.cpp:3174-3235)
+    bool broad_phase(const std::vector<V>& xb, const std::vector<V>& xe)
+    {
+        pairs_vf.clear();
+        pairs_ee.clear();
+        bool ok = true;
+        if (nf > 1)
+        {
+            tree_face.refit(xb.data(), xe.data(), offset.data());
+            for (uint32_t i = 0; i < nv; i++)
+            {
+                AABB3<T> b = {vmin(xb[i], xe[i]), vmax(xb[i], xe[i])};
+                b          = aabb_pad(b, offset[i] + dhat[i]);
+                ok &= tree_face.query(b, i, false, [&](uint32_t q, uint32_t o) { pairs_vf.push_back(q); pairs_vf.push_back(o); });
+            }
+        }
+        if (ne > 1)
+        {
+            tree_edge.refit(xb.data(), xe.data(), offset.data());
+            for (uint32_t e = 0; e < ne; e++)
+            {
+                uint32_t a = edges[2 * e], c = edges[2 * e + 1];
+                AABB3<T> b = {vmin(vmin(xb[a], xb[c]), vmin(xe[a], xe[c])), vmax(vmax(xb[a], xb[c]), vmax(xe[a], xe[c]))};
+                b          = aabb_pad(b, offset[a] + dhat[a]);
+                ok &= tree_edge.query(b, e, true, [&](uint32_t q, uint32_t o) { pairs_ee.push_back(q); pairs_ee.push_back(o); });
+            }
+        }
+        bp_overflow = !ok;
+        return ok;
+    }
+
+    // ---- DCD narrow phase (narrow_phase.cpp:771-1098), barrier energy type
+    void dcd()
+    {
+        contacts.clear();
+        const T kap = T(float(prm.kappa));
+        for (size_t p = 0; p < pairs_vf.size() / 2; p++)
+        {
+            uint32_t        vid = pairs_vf[2 * p], fid = pairs_vf[2 * p + 1];
+            const uint32_t* f   = &faces[3 * fid];
+            if (vid == f[0] || vid == f[1] || vid == f[2]) continue;
+            V     P = x[vid], t0 = x[f[0]], t1 = x[f[1]], t2 = x[f[2]];
+            V3<T> b = pt_coeff_unclassified(P, t0, t1, t2);
+            V     X = (b.x * (P - t0) + b.y * (P - t1)) + b.z * (P - t2);
+            T     d2 = dot(X, X);
+            T     dh = (dhat[vid] + dhat[f[0]]) * T(0.5f);
+            T     th = offset[vid] + offset[f[0]];
+            if (d2 < (th + dh) * (th + dh))
+            {
+                T       d  = std::sqrt(d2);
+                Contact c;
+                c.idx[0] = vid; c.idx[1] = f[0]; c.idx[2] = f[1]; c.idx[3] = f[2];
+                c.type       = 4;
+                c.n          = X / d;
+                T aa         = T(0.5f) * (vert_area[vid] + face_area[fid]);
+                c.k1         = ((-aa) * kap) * barrier_d1(d - th, dh);
+                c.k2         = (aa * kap) * barrier_d2(d - th, dh);
+                c.area_kappa = aa * kap;
+                c.bary0      = b.x;
+                c.bary1      = b.y;
+                c.w[0]       = T(1);
+                c.w[1]       = -b.x;
+                c.w[2]       = -b.y;
+                c.w[3]       = (b.x + b.y) - T(1);
+                contacts.push_back(c);
+            }
+        }
+        for (size_t p = 0; p < pairs_ee.size() / 2; p++)
+        {
+            uint32_t        l = pairs_ee[2 * p], r = pairs_ee[2 * p + 1];
+            const uint32_t* A = &edges[2 * l];
+            const uint32_t* B = &edges[2 * r];
+            if (A[0] == B[0] || A[0] == B[1] || A[1] == B[0] || A[1] == B[1]) continue;
+            V     a0 = x[A[0]], a1 = x[A[1]], b0 = x[B[0]], b1 = x[B[1]];
+            C4<T> c4 = ee_coeff_unclassified(a0, a1, b0, b1);
+            V     x0 = c4.v[0] * a0 + c4.v[1] * a1;
+            V     x1 = c4.v[2] * b0 + c4.v[3] * b1;
+            V     X  = x0 - x1;
+            T     d2 = dot(X, X);
+            T     dh = (dhat[A[0]] + dhat[B[0]]) * T(0.5f);
+            T     th = offset[A[0]] + offset[B[0]];
+            if (d2 < (th + dh) * (th + dh))
+            {
+                T       d  = std::sqrt(d2);
+                Contact c;
+                c.idx[0] = A[0]; c.idx[1] = A[1]; c.idx[2] = B[0]; c.idx[3] = B[1];
+                c.type       = 8;
+                c.n          = X / d;
+                T aa         = T(0.5f) * (edge_area[l] + edge_area[r]);
+                c.k1         = ((-aa) * kap) * barrier_d1(d - th, dh);
+                c.k2         = (aa * kap) * barrier_d2(d - th, dh);
+                c.area_kappa = aa * kap;
+                c.bary0      = c4.v[0];
+                c.bary1      = c4.v[2];
+                c.w[0]       = c4.v[0];
+                c.w[1]       = T(1) - c4.v[0];
+                c.w[2]       = -c4.v[2];
+                c.w[3]       = c4.v[2] - T(1);
+                contacts.push_back(c);
+            }
+        }
+    }
+
+    // narrow_phase.cpp:2550-2676: gradient and diagonal blocks of the contacts (soft bodies)
+    void contact_assemble()
+    {
+        for (const Contact& c : contacts)
+        {
+            M3<T> nn = outer(c.n, c.n);
+            for (int k = 0; k < 4; k++)
+            {
+                uint32_t i = c.idx[k];
+                cgB[i]     = cgB[i] + (c.k1 * c.w[k]) * c.n;
+                diag[i]    = diag[i] + ((c.k2 * c.w[k]) * c.w[k]) * nn;
+            }
+        }
+    }
+
+    // y = A p: diagonal, material off-diagonal slots in row order, then contacts per pair
+    // (the per-pair SpMV form of narrow_phase.cpp:2730-2775).
+    void spmv(const std::vector<V>& p, std::vector<V>& y) const
+    {
+        for (uint32_t i = 0; i < nv; i++)
+        {
+            V s = diag[i] * p[i];
+            for (uint32_t k = csr[i]; k < csr[i + 1]; k++) s = s + offdiag[k] * p[csr[k]];
+            y[i] = s;
+        }
+        for (const Contact& c : contacts)
+        {
+            M3<T> xx = c.k2 * outer(c.n, c.n);
+            for (int j = 0; j < 4; j++)
+            {
+                V o;
+                for (int jj = 0; jj < 4; jj++)
+                    if (j != jj) o = o + ((c.w[j] * c.w[jj]) * xx) * p[c.idx[jj]];
+                y[c.idx[j]] = y[c.idx[j]] + o;
+            }
+        }
+    }
+
+    // Blocked pairwise sum: 256-wide blocks reduced as a binary tree, then the block sums the same way.
+    static T tree_sum(std::vector<T>& a)
+    {
+        size_t n = a.size();
+        if (n == 0) return T(0);
+        while (n > 1)
+        {
+            size_t h = (n + 1) / 2;
+            for (size_t i = 0; i + h < n; i++) a[i] = a[i] + a[i + h];
+            n = h;
+        }
+        return a[0];
+    }
+
+    // precond_cg.cpp:626-771 with a relative stopping rule rz_k <= tol^2 rz_0 or rz_k < abs_tol
+    // checked every pcg_check_every iterations (the reference checks rz < 1e-10 every 25).
+    void pcg()
+    {
+        std::vector<M3<T>> Minv(nv);
+        for (uint32_t i = 0; i < nv; i++) Minv[i] = inverse(diag[i]);
+        std::vector<V> r(cgB), z(nv), p(nv, V()), q(nv, V());
+        dx.assign(nv, V());
+        std::vector<T> tmp(nv), tmp2(nv);
+        T rz_old = 0, rz0 = 0;
+        int it = 0;
+        const T rel2 = T(prm.pcg_rel_tol * prm.pcg_rel_tol), absl = T(prm.pcg_abs_tol);
+        pcg_rel = 0;
+        for (it = 0; it < prm.pcg_max; it++)
+        {
+            for (uint32_t i = 0; i < nv; i++)
+            {
+                z[i]    = Minv[i] * r[i];
+                tmp[i]  = dot(r[i], z[i]);
+                tmp2[i] = dot(r[i], r[i]);
+            }
+            T rz = tree_sum(tmp);
+            (void)tree_sum(tmp2);
+            if (it == 0) rz0 = rz;
+            pcg_rel = rz0 > T(0) ? std::sqrt(rz / rz0) : T(0);
+            if (it % prm.pcg_check_every == 0)
+            {
+                if (!(rz == rz)) break;
+                if (rz < absl || rz <= rel2 * rz0) break;
+            }
+            T beta = rz_old < T(1e-10f) ? T(0) : rz / rz_old;
+            rz_old = rz;
+            for (uint32_t i = 0; i < nv; i++) p[i] = z[i] + beta * p[i];
+            spmv(p, q);
+            for (uint32_t i = 0; i < nv; i++) tmp[i] = dot(p[i], q[i]);
+            T pq    = tree_sum(tmp);
+            T alpha = pq < T(1e-10f) ? T(0) : rz / pq;
+            for (uint32_t i = 0; i < nv; i++)
+            {
+                dx[i] = dx[i] + alpha * p[i];
+                r[i]  = r[i] - alpha * q[i];
+            }
+        }
+        pcg_iters = it;
+    }
+
+    // newton_solver.cpp:952-1000 and narrow_phase.cpp:293-593: global alpha = min(toi / 1.25) over
+    // ground, VF and EE (1 if no impact).
+    T ccd(const std::vector<V>& xb, const std::vector<V>& xe)
+    {
+        T t = T(1);
+        if (prm.use_floor)
+        {
+            const T fy = T(float(prm.floor_y));
+            for (uint32_t i = 0; i < nv; i++)
+            {
+                if (is_fixed[i]) continue;
+                T off = offset[i], cy = xe[i].y;
+                if (cy - off < fy)
+                {
+                    T iy = xb[i].y;
+                    t    = std::min(t, ((iy - off) - fy) / (iy - cy) / T(kLineSearchMaxT));
+                }
+            }
+        }
+        if (!prm.use_self_collision) return t;
+        for (size_t p = 0; p < pairs_vf.size() / 2; p++)
+        {
+            uint32_t        vid = pairs_vf[2 * p], fid = pairs_vf[2 * p + 1];
+            const uint32_t* f   = &faces[3 * fid];
+            if (vid == f[0] || vid == f[1] || vid == f[2]) continue;
+            V a0[4] = {xb[vid], xb[f[0]], xb[f[1]], xb[f[2]]};
+            V a1[4] = {xe[vid], xe[f[0]], xe[f[1]], xe[f[2]]};
+            T toi   = accd(a0, a1, offset[vid] + offset[f[0]], 0);
+            if (toi != T(kLineSearchMaxT)) t = std::min(t, toi / T(kLineSearchMaxT));
+        }
+        for (size_t p = 0; p < pairs_ee.size() / 2; p++)
+        {
+            uint32_t        l = pairs_ee[2 * p], r = pairs_ee[2 * p + 1];
+            const uint32_t* A = &edges[2 * l];
+            const uint32_t* B = &edges[2 * r];
+            if (A[0] == B[0] || A[0] == B[1] || A[1] == B[0] || A[1] == B[1]) continue;
+            V a0[4] = {xb[A[0]], xb[A[1]], xb[B[0]], xb[B[1]]};
+            V a1[4] = {xe[A[0]], xe[A[1]], xe[B[0]], xe[B[1]]};
+            T toi   = accd(a0, a1, offset[A[0]] + offset[B[0]], 1);
+            if (toi != T(kLineSearchMaxT)) t = std::min(t, toi / T(kLineSearchMaxT));
+        }
+        return t;
+    }
+
+    void apply_dx(T a)
+    {
+        for (uint32_t i = 0; i < nv; i++) x[i] = x_iter[i] + a * dx[i];
+    }
+
+    // newton_solver.cpp:3685-3884 (CCD line search, no energy backtracking)
+    void line_search()
+    {
+        apply_dx(T(1));
+        toi = T(1);
+        if (prm.use_ccd)
+        {
+            if (prm.use_self_collision) broad_phase(x_iter, x);
+            else { pairs_vf.clear(); pairs_ee.clear(); }
+            toi = ccd(x_iter, x);
+        }
+        alpha  = toi;
+        max_dx = 0;
+        for (uint32_t i = 0; i < nv; i++) max_dx = std::max(max_dx, length(dx[i]));
+        if (max_dx < T(float(prm.newton_tol)) * T(float(prm.dt)))
+        {
+            apply_dx(alpha);
+            converged = dirichlet_converged = true;
+            return;
+        }
+        converged = false;
+        apply_dx(alpha);
+        pin_delta = T(-1e9f);
+        for (uint32_t i = 0; i < nv; i++)
+            if (is_fixed[i]) pin_delta = std::max(pin_delta, length(x[i] - x_tilde[i]));
+        dirichlet_converged = pin_delta < T(1e-3f);
+    }
+
+    void update_contacts()
+    {
+        if (prm.use_self_collision)
+        {
+            broad_phase(x, x);
+            dcd();
+        }
+        else
+        {
+            pairs_vf.clear();
+            pairs_ee.clear();
+            contacts.clear();
+        }
+    }
+
+    void newton_iteration()
+    {
+        x_iter = x;
+        material();
+        update_contacts();
+        contact_assemble();
+        pcg();
+        line_search();
+        iters_log.push_back(pcg_iters);
+    }
+
+    // newton_solver.cpp:4051-4345
+    void step()
+    {
+        frame_begin();
+        predict();
+        converged = dirichlet_converged = false;
+        int it;
+        for (it = 0; it < prm.newton_max; it++)
+        {
+            if (converged || (it >= prm.newton_min && dirichlet_converged)) break;
+            newton_iteration();
+        }
+        newton_iters = it;
+        frame_end();
+    }
+
+    // newton_solver.cpp:610-636 + solver_interface.cpp:163-183
+    void frame_end()
+    {
+        const T h = T(float(prm.dt));
+        const T damp = std::exp(-T(float(prm.damping)) * h);
+        for (uint32_t i = 0; i < nv; i++)
+        {
+            V vel     = (x[i] - x_step[i]) / h;
+            vel       = vel * damp;
+            v[i]      = vel;
+            x_step[i] = x[i];
+            x_outer[i] = x[i];
+            v_outer[i] = v[i];
+        }
+        frame += 1;
+    }
+};
+
+}  // namespace orc

@@ -1,0 +1,215 @@
+"""ctypes wrapper of liblcs_oracle.so (test infrastructure, see oracle/__init__.py)."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB: Optional[C.CDLL] = None
+_DT = {0: np.uint32, 1: np.float32, 2: np.float64, 3: np.uint64, 4: np.uint16, 5: np.uint8}
+
+
+def load() -> C.CDLL:
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    path = os.path.join(_HERE, "liblcs_oracle.so")
+    if not os.path.exists(path):
+        from . import build as _b
+        _b.build()
+    lib = C.CDLL(path)
+    vp, dp, fp, ip = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_float), C.POINTER(C.c_int)
+    lib.orc_new.restype = vp
+    lib.orc_new.argtypes = [C.c_int]
+    lib.orc_free.argtypes = [vp]
+    lib.orc_set_params.argtypes = [vp, dp, C.c_int]
+    lib.orc_add_mesh.argtypes = [vp, fp, C.c_int, C.POINTER(C.c_int32), C.c_int, dp, C.c_int]
+    lib.orc_pin_method.argtypes = [vp, C.c_int, C.c_int, C.c_double, dp, C.c_int]
+    lib.orc_stage.argtypes = [vp, C.c_char_p]
+    lib.orc_info.argtypes = [vp, C.c_char_p, C.POINTER(C.c_int64), ip]
+    lib.orc_get.argtypes = [vp, C.c_char_p, vp, C.c_int64]
+    lib.orc_set.argtypes = [vp, C.c_char_p, vp, C.c_int64]
+    lib.orc_scalars.argtypes = [vp, dp, C.c_int]
+    lib.orc_morton32.restype = C.c_uint32
+    lib.orc_morton32.argtypes = [C.c_float, C.c_float, C.c_float]
+    lib.orc_accd.restype = C.c_double
+    lib.orc_accd.argtypes = [dp, dp, C.c_double, C.c_int, C.c_int, ip]
+    lib.orc_dist2.restype = C.c_double
+    lib.orc_dist2.argtypes = [dp, C.c_int, C.c_int, dp]
+    lib.orc_barrier.argtypes = [C.c_double, C.c_double, C.c_int, dp]
+    lib.orc_bw_face.argtypes = [dp, dp, C.c_double, C.c_double, C.c_double, C.c_int, dp, dp, dp, dp]
+    lib.orc_dm_inv.argtypes = [dp, C.c_int, dp]
+    lib.orc_dihedral.restype = C.c_double
+    lib.orc_dihedral.argtypes = [dp, C.c_int, dp]
+    lib.orc_lbvh_new.restype = vp
+    lib.orc_lbvh_new.argtypes = [C.c_int, C.POINTER(C.c_uint32), C.c_uint32]
+    lib.orc_lbvh_free.argtypes = [vp]
+    lib.orc_lbvh_construct.argtypes = [vp, fp, C.c_uint32]
+    lib.orc_lbvh_refit.argtypes = [vp, fp, fp, fp]
+    lib.orc_lbvh_get.argtypes = [vp, C.c_char_p, vp, C.c_int64]
+    lib.orc_lbvh_query.restype = C.c_int64
+    lib.orc_lbvh_query.argtypes = [vp, fp, C.c_uint32, C.c_int, C.POINTER(C.c_uint32), C.c_int64]
+    _LIB = lib
+    return lib
+
+
+def _d(a) -> C.POINTER(C.c_double):
+    return np.ascontiguousarray(a, dtype=np.float64).ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Oracle:
+    """One scene in the CPU restatement (FP32 mirror or FP64 twin)."""
+
+    SCALARS = ("toi", "alpha", "max_dx", "pin_delta", "pcg_iters", "newton_iters", "frame", "converged",
+               "dirichlet_converged", "pcg_rel", "bp_overflow")
+
+    def __init__(self, scene, precision: int = 32):
+        self.lib = load()
+        self.h = self.lib.orc_new(precision)
+        self.precision = precision
+        p = np.asarray(scene.params.pack(), dtype=np.float64)
+        assert self.lib.orc_set_params(self.h, _d(p), len(p)) == 0
+        for mi, m in enumerate(scene.meshes):
+            x = np.ascontiguousarray(m.verts, dtype=np.float32)
+            t = np.ascontiguousarray(m.tris, dtype=np.int32)
+            mat = np.asarray(m.material.pack(), dtype=np.float64)
+            self._keep = (x, t, mat)
+            r = self.lib.orc_add_mesh(self.h, x.ctypes.data_as(C.POINTER(C.c_float)), len(x),
+                                      t.ctypes.data_as(C.POINTER(C.c_int32)), len(t), _d(mat), len(mat))
+            assert r == mi
+            from luisacomputesimulator_amd.scenes import FIXED_METHODS
+            for fp in m.fixed:
+                an = np.asarray(fp.anim.pack(), dtype=np.float64)
+                self.lib.orc_pin_method(self.h, mi, FIXED_METHODS[fp.method], fp.range, _d(an), len(an))
+        self.stage("init")
+
+    def __del__(self):
+        try:
+            self.lib.orc_free(self.h)
+        except Exception:
+            pass
+
+    def stage(self, name: str) -> int:
+        r = self.lib.orc_stage(self.h, name.encode())
+        if r < 0:
+            raise ValueError(f"unknown oracle stage {name}")
+        return r
+
+    def get(self, name: str) -> np.ndarray:
+        n, dt = C.c_int64(), C.c_int()
+        if self.lib.orc_info(self.h, name.encode(), C.byref(n), C.byref(dt)) != 0:
+            raise KeyError(name)
+        a = np.empty(n.value, dtype=_DT[dt.value])
+        assert self.lib.orc_get(self.h, name.encode(), a.ctypes.data_as(C.c_void_p), a.nbytes) == 0
+        return a
+
+    def set(self, name: str, value) -> None:
+        n, dt = C.c_int64(), C.c_int()
+        if self.lib.orc_info(self.h, name.encode(), C.byref(n), C.byref(dt)) != 0:
+            raise KeyError(name)
+        a = np.ascontiguousarray(np.asarray(value).reshape(-1), dtype=_DT[dt.value])
+        assert self.lib.orc_set(self.h, name.encode(), a.ctypes.data_as(C.c_void_p), a.nbytes) == 0
+
+    def scalars(self) -> dict:
+        out = np.zeros(len(self.SCALARS), dtype=np.float64)
+        self.lib.orc_scalars(self.h, out.ctypes.data_as(C.POINTER(C.c_double)), len(out))
+        return dict(zip(self.SCALARS, out.tolist()))
+
+    def step(self, frames: int = 1) -> None:
+        for _ in range(frames):
+            self.stage("step")
+
+
+class OracleLbvh:
+    """Standalone FP32 LBVH (faces: arity 3, edges: arity 2)."""
+
+    def __init__(self, tree_type: int, prims: np.ndarray):
+        self.lib = load()
+        self.prims = np.ascontiguousarray(prims, dtype=np.uint32).reshape(-1)
+        self.n = len(self.prims) // (3 if tree_type == 0 else 2)
+        self.h = self.lib.orc_lbvh_new(tree_type, self.prims.ctypes.data_as(C.POINTER(C.c_uint32)), self.n)
+
+    def __del__(self):
+        try:
+            self.lib.orc_lbvh_free(self.h)
+        except Exception:
+            pass
+
+    def construct(self, x: np.ndarray) -> bool:
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        return self.lib.orc_lbvh_construct(self.h, x.ctypes.data_as(C.POINTER(C.c_float)), len(x)) == 0
+
+    def refit(self, x0, x1, pad) -> None:
+        x0 = np.ascontiguousarray(x0, dtype=np.float32)
+        x1 = np.ascontiguousarray(x1, dtype=np.float32)
+        pad = np.ascontiguousarray(pad, dtype=np.float32)
+        fp = C.POINTER(C.c_float)
+        self.lib.orc_lbvh_refit(self.h, x0.ctypes.data_as(fp), x1.ctypes.data_as(fp), pad.ctypes.data_as(fp))
+
+    def get(self, name: str) -> np.ndarray:
+        sizes = {"morton": (self.n, np.uint64), "morton_sorted": (self.n, np.uint64),
+                 "sorted_to_orig": (self.n, np.uint32), "parent": (2 * self.n - 1, np.uint32),
+                 "children": (2 * (2 * self.n - 1), np.uint32), "object_idx": (2 * self.n - 1, np.uint32),
+                 "node_aabb": (6 * (2 * self.n - 1), np.float32), "global_aabb": (6, np.float32)}
+        n, dt = sizes[name]
+        a = np.empty(n, dtype=dt)
+        assert self.lib.orc_lbvh_get(self.h, name.encode(), a.ctypes.data_as(C.c_void_p), a.nbytes) == 0
+        return a
+
+    def query(self, boxes: np.ndarray, edge_filter: bool):
+        boxes = np.ascontiguousarray(boxes, dtype=np.float32).reshape(-1, 6)
+        cap = max(16, 64 * len(boxes))
+        while True:
+            out = np.empty(2 * cap, dtype=np.uint32)
+            k = self.lib.orc_lbvh_query(self.h, boxes.ctypes.data_as(C.POINTER(C.c_float)), len(boxes),
+                                        int(edge_filter), out.ctypes.data_as(C.POINTER(C.c_uint32)), cap)
+            ok = k >= 0
+            k = k if ok else -k - 1
+            if k <= cap:
+                return out[: 2 * k].reshape(-1, 2), ok
+            cap = int(k)
+
+
+def morton32(x: float, y: float, z: float) -> int:
+    return int(load().orc_morton32(x, y, z))
+
+
+def accd(x0, x1, offset: float, kind: int, precision: int = 32):
+    it = C.c_int()
+    t = load().orc_accd(_d(x0), _d(x1), offset, kind, precision, C.byref(it))
+    return t, it.value
+
+
+def dist2(x, kind: int, precision: int = 32):
+    coeff = np.zeros(4)
+    d = load().orc_dist2(_d(x), kind, precision, coeff.ctypes.data_as(C.POINTER(C.c_double)))
+    return d, coeff
+
+
+def barrier(d: float, dh: float, precision: int = 64):
+    out = np.zeros(3)
+    load().orc_barrier(d, dh, precision, out.ctypes.data_as(C.POINTER(C.c_double)))
+    return out
+
+
+def bw_face(x, Dm, mu, lmd, area, precision: int = 64):
+    g, H9, H6, e = np.zeros(9), np.zeros(81), np.zeros(36), np.zeros(1)
+    p = C.POINTER(C.c_double)
+    load().orc_bw_face(_d(x), _d(Dm), mu, lmd, area, precision, g.ctypes.data_as(p), H9.ctypes.data_as(p),
+                       H6.ctypes.data_as(p), e.ctypes.data_as(p))
+    return g, H9.reshape(9, 9), H6.reshape(6, 6), float(e[0])
+
+
+def dm_inv(x, precision: int = 64):
+    out = np.zeros(4)
+    load().orc_dm_inv(_d(x), precision, out.ctypes.data_as(C.POINTER(C.c_double)))
+    return out
+
+
+def dihedral(x, precision: int = 64):
+    g = np.zeros(12)
+    a = load().orc_dihedral(_d(x), precision, g.ctypes.data_as(C.POINTER(C.c_double)))
+    return a, g.reshape(4, 3)

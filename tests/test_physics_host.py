@@ -1,0 +1,23 @@
+"""The product's per-element physics source, compiled for the host, equals the oracle bit for bit.
+
+tests/cpp/physics_parity.cpp includes luisacomputesimulator_amd/csrc/lcsx_physics.h (the code the
+HIP kernels run) and the oracle headers, and compares FP32 results on random inputs. Both are built
+without floating-point contraction, so any difference is a restatement error, not rounding.
+"""
+import os
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_device_math_matches_oracle_on_host(tmp_path):
+    src = os.path.join(ROOT, "tests", "cpp", "physics_parity.cpp")
+    exe = str(tmp_path / "physics_parity")
+    cmd = ["/opt/rocm/bin/hipcc", "-x", "hip", "--offload-arch=gfx950", "-O2", "-std=c++17", "-ffp-contract=off",
+           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", src, "-o", exe]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    out = subprocess.run([exe], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0 and out.stdout.startswith("OK"), out.stdout + out.stderr
