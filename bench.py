@@ -24,7 +24,7 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-BASELINE_METRIC = "Newton steps/s @88K-vert cloth, mixed vs FP32; SpMV HBM GB/s; scene-steps/s"
+METRIC = "Newton steps/s @88K-vert cloth, mixed vs FP32; SpMV HBM GB/s; scene-steps/s"
 HBM_PEAK_GBS = 8000.0
 
 
@@ -67,14 +67,12 @@ def main():
     ap.add_argument("--rings", type=int, default=297)
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: the workload is a single scene on one MI355X and this build "
-              "has no multi-GPU path (see DESIGN.md for how it would shard). Run with --gpus 1.",
+        print("bench.py measures one GPU: the workload is a single scene on one MI355X and the solver "
+              "has no multi-GPU path. Run with --gpus 1.",
               file=sys.stderr)
         sys.exit(2)
 
-    import torch
-
-    from luisacomputesimulator_amd import NewtonSolver, scenes
+    from luisacomputesimulator_amd import NewtonSolver, hiprt, scenes
 
     sc = scenes.cylinder_tube(rings=args.rings, per_ring=args.rings)
     g = NewtonSolver.from_scene(sc)
@@ -84,7 +82,7 @@ def main():
     for _ in range(args.warmup):
         g.physics_step_gpu(1)
     g.kernel_times(reset=True)
-    torch.cuda.synchronize()
+    hiprt.synchronize()
     newton = pcg = 0
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -92,7 +90,7 @@ def main():
         st = g.stats()
         newton += st["newton_iters"]
         pcg += st["pcg_iters_total"]
-    torch.cuda.synchronize()
+    hiprt.synchronize()
     elapsed = time.perf_counter() - t0
     kt = g.kernel_times()
     spmv_ms, spmv_n = kt.get("spmv", (float("nan"), 0))
@@ -100,7 +98,7 @@ def main():
     achieved = nbytes / (spmv_ms * 1e-3) / 1e9 if spmv_n else None
     st = g.stats()
     line = {
-        "metric": BASELINE_METRIC,
+        "metric": METRIC,
         "value": newton / elapsed,
         "unit": "Newton steps/s",
         "n_gpus": 1,

@@ -308,10 +308,11 @@ __global__ __launch_bounds__(kBlock) void k_refit(uint32_t n, int arity, const u
     uint32_t cur    = ni + l;
     NodeBox  box    = {make_float4(lo.x, lo.y, lo.z, 0.0f), make_float4(hi.x, hi.y, hi.z, 0.0f)};
     nodes[cur]      = box;
-    while (true)
+    // a tree of n leaves has depth < n; the bound only guards against a corrupted parent array
+    for (uint32_t depth = 0; depth < n; depth++)
     {
         uint32_t p = parent[cur];
-        if (p == kInvalid) break;
+        if (p == kInvalid || p >= ni) break;
         __threadfence();
         if (atomicAdd(&counters[p], 1u) == 0u) break;
         __threadfence();
@@ -369,8 +370,16 @@ __global__ __launch_bounds__(kQueryBlock) void k_query(QueryArgs a, const NodeBo
     const uint32_t tid = threadIdx.x;
     int            sp  = 0;
     stack[sp++][tid]   = 0u;
+    // each inner node is visited at most once per query in a well-formed tree; the cap (as the
+    // reference's 100000-iteration guard) only bounds a corrupted one
+    uint32_t visits = 0;
     while (sp > 0)
     {
+        if (++visits > n_inner + 1u)
+        {
+            if (a.overflow) atomicOr(a.overflow, 2u);
+            break;
+        }
         uint32_t nd  = stack[--sp][tid];
         uint2    ch  = children[nd];
         bool     brk = false;

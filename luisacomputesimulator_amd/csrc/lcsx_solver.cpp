@@ -145,6 +145,8 @@ void Solver::init()
     status.alloc(1);
     pcg_state.alloc(1);
     LCSX_HIP(hipMemsetAsync(pcg_state.p, 0, sizeof(PcgState), stream));
+    LCSX_HIP(hipMemsetAsync(status.p, 0, sizeof(Status), stream));
+    host_status = Status{};
     LCSX_HIP(hipMemsetAsync(sell_val.p, 0, sell_val.n * sizeof(float), stream));
     explicit_targets.assign(3 * S.fixed_verts.size(), 0.0f);
     has_explicit.assign(meshes.size(), 0);

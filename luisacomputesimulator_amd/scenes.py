@@ -115,6 +115,8 @@ class Scene:
     meshes: List[Mesh]
     params: Params
     frames: int = 1
+    # per-vertex initial velocity (all meshes concatenated), None = at rest
+    initial_velocity: Optional[np.ndarray] = None
 
     @property
     def num_verts(self) -> int:
@@ -141,6 +143,28 @@ def grid_triangles(rows: int, cols: int, wrap_cols: bool = False) -> np.ndarray:
             else:
                 tris[k] = (a, cc, b)
                 tris[k + 1] = (b, cc, d)
+            k += 2
+    return tris
+
+
+def staggered_ring_triangles(rings: int, per_ring: int) -> np.ndarray:
+    """Triangles of a closed tube whose odd rings are turned by half a vertex spacing (vertex
+    id = r * per_ring + c): every triangle is isosceles with its apex on the neighbouring ring,
+    so no vertex lies closer to a non-incident triangle than the smaller of the ring spacing and
+    the triangle's slant altitude."""
+    tris = np.empty(((rings - 1) * per_ring * 2, 3), dtype=np.int32)
+    k = 0
+    for r in range(rings - 1):
+        for c in range(per_ring):
+            c1 = (c + 1) % per_ring
+            a, b = r * per_ring + c, r * per_ring + c1
+            u, w = (r + 1) * per_ring + c, (r + 1) * per_ring + c1
+            if r % 2 == 0:  # ring r+1 is turned by +1/2: apex u sits between a and b
+                tris[k] = (a, u, b)
+                tris[k + 1] = (b, u, w)
+            else:           # ring r is turned by +1/2: apex w sits between a and b
+                tris[k] = (a, w, b)
+                tris[k + 1] = (a, u, w)
             k += 2
     return tris
 
@@ -172,17 +196,21 @@ def cylinder_tube(rings: int = 297, per_ring: int = 297, radius: float = 0.11, l
 
     The tube (rings x per_ring vertices, axis along x, centred at the origin) has both end rings
     pinned and rotating with the cylinder. The cylinder is an all-pinned cloth mesh (its elements
-    are culled as in init_sim_data.cpp:207-240, it only collides). Config-0 material, gravity
-    -9.8 m/s^2, no ground, kappa = 1e9 as the reference's 88K cylinder scene.
+    are culled as in init_sim_data.cpp:207-240, it only collides). The square cloth's material,
+    gravity -9.8 m/s^2, no ground, kappa = 1e9 as the reference's 88K cylinder scene. Both meshes
+    start in the steady spin, v = omega x r.
     """
     spin = Anim(use_rotate=True, rot_center=(0.0, 0.0, 0.0), rot_axis=(1.0, 0.0, 0.0), rot_deg_per_s=omega_deg)
     xs = np.linspace(-0.5 * length, 0.5 * length, rings)
-    th = 2.0 * np.pi * np.arange(per_ring) / per_ring
-    X, TH = np.meshgrid(xs, th, indexing="ij")
-    tube = np.stack([X, radius * np.cos(TH), radius * np.sin(TH)], -1).reshape(-1, 3).astype(np.float32)
+    # odd rings turned by half a spacing (staggered triangulation): with 3.4 x 2.3 mm cells split
+    # on a diagonal the cell altitude (1.9 mm) would sit inside the 2 mm contact band (two 0.5 mm
+    # offsets + d_hat) and every cell would start in contact; the staggered altitude is 2.2 mm
+    th = 2.0 * np.pi * (np.arange(per_ring)[None, :] + 0.5 * (np.arange(rings)[:, None] % 2)) / per_ring
+    X = np.broadcast_to(xs[:, None], th.shape)
+    tube = np.stack([X, radius * np.cos(th), radius * np.sin(th)], -1).reshape(-1, 3).astype(np.float32)
     mat = Material(thickness=1e-3, youngs_modulus=1e4, poisson_ratio=0.3, area_bending_stiffness=1e-4,
                    density=200.0, d_hat=1e-3)
-    tube_mesh = Mesh("tube", tube, grid_triangles(rings, per_ring, wrap_cols=True), mat,
+    tube_mesh = Mesh("tube", tube, staggered_ring_triangles(rings, per_ring), mat,
                      [FixedPoints("Left", 0.001, spin), FixedPoints("Right", 0.001, spin)])
     cx = np.linspace(-0.55 * length, 0.55 * length, cyl_rings)
     ct = 2.0 * np.pi * np.arange(cyl_segments) / cyl_segments
@@ -191,7 +219,12 @@ def cylinder_tube(rings: int = 297, per_ring: int = 297, radius: float = 0.11, l
     cyl_mesh = Mesh("cylinder", cyl, grid_triangles(cyl_rings, cyl_segments, wrap_cols=True), mat,
                     [FixedPoints("All", 0.001, spin)])
     prm = Params(use_floor=False, stiffness_collision=1e9)
-    return Scene(f"cylinder_tube_{rings}x{per_ring}", [tube_mesh, cyl_mesh], prm, frames=60)
+    # Both meshes start in the steady spin (v = omega x r): started from rest, the pinned end rings
+    # shear the first free ring by omega * r * dt in one frame, far more than the ring spacing.
+    omega = np.deg2rad(omega_deg) * np.array([1.0, 0.0, 0.0])
+    x_all = np.concatenate([tube, cyl]).astype(np.float64)
+    v0 = np.cross(omega[None, :], x_all).astype(np.float32)
+    return Scene(f"cylinder_tube_{rings}x{per_ring}", [tube_mesh, cyl_mesh], prm, frames=60, initial_velocity=v0)
 
 
 def twisting_sheet(n: int = 513, omega_deg: float = 180.0) -> Scene:

@@ -307,6 +307,19 @@ int orc_scalars(void* p, double* out, int n)
     return 0;
 }
 
+// The pair that bounded the step in the last CCD (kind -1 ground/none, 0 VF, 1 EE), for diagnosis.
+int orc_ccd_argmin(void* p, int64_t* out3)
+{
+    Handle* h = (Handle*)p;
+    auto    f = [&](auto& s) {
+        out3[0] = s.ccd_arg.kind;
+        out3[1] = s.ccd_arg.a;
+        out3[2] = s.ccd_arg.b;
+    };
+    if (h->prec == 64) f(h->s64); else f(h->s32);
+    return 0;
+}
+
 // ---- stateless kernels for golden-vector tests -------------------------------------------
 uint32_t orc_morton32(float x, float y, float z) { return morton32(V3<float>(x, y, z)); }
 

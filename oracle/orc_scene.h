@@ -97,6 +97,11 @@ struct Scene
     std::vector<Contact> contacts;
     // ---- per-iteration stats
     T   toi = 1, alpha = 1, max_dx = 0, pin_delta = 0;
+    struct CcdArg
+    {
+        int      kind;  // -1 ground / none, 0 VF, 1 EE
+        uint32_t a, b;  // (vertex, face) or (edge, edge)
+    } ccd_arg{-1, 0, 0};
     int pcg_iters = 0, newton_iters = 0, frame = 0;
     T   pcg_rel = 0;
     bool converged = false, dirichlet_converged = false;
@@ -817,6 +822,7 @@ struct Scene
     T ccd(const std::vector<V>& xb, const std::vector<V>& xe)
     {
         T t = T(1);
+        ccd_arg = {-1, 0, 0};
         if (prm.use_floor)
         {
             const T fy = T(float(prm.floor_y));
@@ -840,7 +846,11 @@ struct Scene
             V a0[4] = {xb[vid], xb[f[0]], xb[f[1]], xb[f[2]]};
             V a1[4] = {xe[vid], xe[f[0]], xe[f[1]], xe[f[2]]};
             T toi   = accd(a0, a1, offset[vid] + offset[f[0]], 0);
-            if (toi != T(kLineSearchMaxT)) t = std::min(t, toi / T(kLineSearchMaxT));
+            if (toi != T(kLineSearchMaxT) && toi / T(kLineSearchMaxT) < t)
+            {
+                t        = toi / T(kLineSearchMaxT);
+                ccd_arg  = {0, vid, fid};
+            }
         }
         for (size_t p = 0; p < pairs_ee.size() / 2; p++)
         {
@@ -851,7 +861,11 @@ struct Scene
             V a0[4] = {xb[A[0]], xb[A[1]], xb[B[0]], xb[B[1]]};
             V a1[4] = {xe[A[0]], xe[A[1]], xe[B[0]], xe[B[1]]};
             T toi   = accd(a0, a1, offset[A[0]] + offset[B[0]], 1);
-            if (toi != T(kLineSearchMaxT)) t = std::min(t, toi / T(kLineSearchMaxT));
+            if (toi != T(kLineSearchMaxT) && toi / T(kLineSearchMaxT) < t)
+            {
+                t        = toi / T(kLineSearchMaxT);
+                ccd_arg  = {1, l, r};
+            }
         }
         return t;
     }

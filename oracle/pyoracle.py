@@ -33,6 +33,7 @@ def load() -> C.CDLL:
     lib.orc_get.argtypes = [vp, C.c_char_p, vp, C.c_int64]
     lib.orc_set.argtypes = [vp, C.c_char_p, vp, C.c_int64]
     lib.orc_scalars.argtypes = [vp, dp, C.c_int]
+    lib.orc_ccd_argmin.argtypes = [vp, C.POINTER(C.c_int64)]
     lib.orc_morton32.restype = C.c_uint32
     lib.orc_morton32.argtypes = [C.c_float, C.c_float, C.c_float]
     lib.orc_accd.restype = C.c_double
@@ -85,6 +86,8 @@ class Oracle:
                 an = np.asarray(fp.anim.pack(), dtype=np.float64)
                 self.lib.orc_pin_method(self.h, mi, FIXED_METHODS[fp.method], fp.range, _d(an), len(an))
         self.stage("init")
+        if getattr(scene, "initial_velocity", None) is not None:
+            self.set("v_outer", scene.initial_velocity)
 
     def __del__(self):
         try:
@@ -121,6 +124,12 @@ class Oracle:
     def step(self, frames: int = 1) -> None:
         for _ in range(frames):
             self.stage("step")
+
+    def ccd_argmin(self) -> tuple:
+        """(kind, a, b) of the pair that bounded the last CCD: kind -1 none/ground, 0 VF, 1 EE."""
+        out = np.zeros(3, dtype=np.int64)
+        self.lib.orc_ccd_argmin(self.h, out.ctypes.data_as(C.POINTER(C.c_int64)))
+        return tuple(int(v) for v in out)
 
 
 class OracleLbvh:

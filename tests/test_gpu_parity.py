@@ -62,15 +62,14 @@ def test_lbvh_reference_eight_leaf_tree(require_gpu):
     # a scene whose edges are exactly these degenerate segments is not a triangle mesh, so use
     # the standalone LBVH entry points on device memory
     import ctypes as C
-    import torch
-    from luisacomputesimulator_amd import load_library, _check
+    from luisacomputesimulator_amd import _check, hiprt, load_library
     lib = load_library()
-    dx = torch.tensor(x.reshape(-1), device="cuda")
-    de = torch.tensor(edges.reshape(-1).astype(np.int64), device="cuda").to(torch.int32)
+    dx = hiprt.DeviceArray(x.reshape(-1).astype(np.float32))
+    de = hiprt.DeviceArray(edges.reshape(-1).astype(np.uint32))
     h = C.c_void_p()
-    _check(lib.lcsx_lbvh_create(2, C.c_void_p(de.data_ptr()), 8, C.byref(h)))
-    _check(lib.lcsx_lbvh_construct(h, C.c_void_p(dx.data_ptr()), 16, None))
-    torch.cuda.synchronize()
+    _check(lib.lcsx_lbvh_create(2, de.ptr, 8, C.byref(h)))
+    _check(lib.lcsx_lbvh_construct(h, dx.ptr, 16, None))
+    hiprt.synchronize()
     ms = np.empty(8, dtype=np.uint64)
     _check(lib.lcsx_lbvh_read(h, b"morton_sorted", ms.ctypes.data_as(C.c_void_p), ms.nbytes))
     ch = np.empty(30, dtype=np.uint32)
@@ -158,7 +157,7 @@ def test_spmv_and_pcg(require_gpu):
     assert np.linalg.norm(dxg - dxo) <= 2e-3 * np.linalg.norm(dxo)
 
 
-@pytest.mark.parametrize("name,frames", [("square33", 6), ("two_sheets", 6)])
+@pytest.mark.parametrize("name,frames", [("square33", 6), ("tube_small", 3)])
 def test_frames_track_oracle(require_gpu, name, frames):
     sc = SCENES[name]()
     g, o = make(sc)
