@@ -1,19 +1,21 @@
 #!/usr/bin/env python3
 """Benchmark: Newton steps/s of the cloth Newton step on one MI355X.
 
-Workload: the 297 x 297-ring cloth tube (88,209 vertices, 175,824 triangles) around a spinning
-kinematic cylinder (5,760 vertices), dt = 1/60 s, barrier contact with kappa = 1e9, FP32.
-A bench "step" is one simulated frame (each frame runs >= 3 Newton iterations); `value` is
-Newton iterations per second over the timed frames, with all inputs resident on the GPU.
+Workload: the 297 x 297-vertex cloth tube (88,209 vertices, 175,824 triangles, r = 0.11 m,
+length 1 m) around a kinematic cylinder (r = 0.1 m, 5,760 vertices) spinning at 2*pi rad/s,
+tube ends pinned to the spin, both started in the steady spin; dt = 1/60 s, gravity, self
+contact with d_hat = 1e-3 m and kappa = 1e9, FP32. A bench "step" is one simulated frame (each
+frame runs >= 3 Newton iterations, each with <= 100 PCG iterations); `value` is Newton
+iterations per second over the timed frames, with every input resident on the GPU.
 
 The JSON line carries:
   roofline      the PCG block-SpMV (dominant kernel): algorithmic bytes per launch
-                40 B per off-diagonal block (36 B values + 4 B column) + 88 B per row
-                (36 B diagonal block, z and p_old of the row 24 B, p_new and q writes 24 B,
-                4 B slice offset amortised) divided by the launch's average duration measured
+                40 B per off-diagonal block (36 B values + 4 B column) + 84 B per row
+                (36 B diagonal block, z and p_old of the row 24 B, p_new and q writes 24 B)
+                + 4 B per 64-row slice, divided by the launch's average duration measured
                 with hipEvents on the solver stream; peak = 8 TB/s HBM3E.
-  cpu_baseline  the CPU oracle (C++ restatement of the reference path, 1 thread) timed on one
-                Newton iteration of the same workload on this host.
+  cpu_baseline  the CPU oracle (C++ restatement of the reference path, 1 thread) timed on
+                frame 0 of the same workload on this host.
 """
 import argparse
 import json
@@ -29,7 +31,8 @@ HBM_PEAK_GBS = 8000.0
 
 
 def spmv_bytes(n_rows: int, n_blocks: int) -> int:
-    return 40 * n_blocks + 88 * n_rows
+    """40 B per off-diagonal block + 84 B per row + 4 B per 64-row slice offset."""
+    return 40 * n_blocks + 84 * n_rows + 4 * ((n_rows + 63) // 64)
 
 
 def traffic_from_profiles():
@@ -45,23 +48,23 @@ def traffic_from_profiles():
 
 
 def cpu_baseline(scene):
+    """The CPU oracle (FP32, one thread) on frame 0 of the same scene: Newton iterations per second."""
     import oracle
 
     o = oracle.Oracle(scene, 32)
-    o.stage("frame_begin")
-    o.stage("predict")
     t0 = time.perf_counter()
-    o.stage("newton_iteration")
+    o.step()
     dt = time.perf_counter() - t0
-    return {"value": 1.0 / dt, "unit": "Newton steps/s", "cores": 1, "kind": "port",
-            "sample": f"first Newton iteration of frame 0 on the same {scene.num_verts}-vertex scene "
-                      f"(C++ oracle, FP32, single thread, {dt:.1f} s)"}
+    its = int(o.scalars()["newton_iters"])
+    return {"value": its / dt, "unit": "Newton steps/s", "cores": 1, "kind": "port",
+            "sample": f"frame 0 of the same {scene.num_verts}-vertex scene ({its} Newton iterations, "
+                      f"{dt:.1f} s; C++ oracle, FP32, single thread)"}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--rings", type=int, default=297)
@@ -79,17 +82,25 @@ def main():
     g.set_timing(2)
     n_rows = g.num_verts
     n_blocks = len(g.read("csr")) - (n_rows + 1)
-    for _ in range(args.warmup):
+    def progress(tag, f, st, t):
+        print(f"[bench] {tag} frame {f}: {st['newton_iters']} Newton, {st['pcg_iters_total']} PCG iterations, "
+              f"{st['num_contacts']} contacts, {t:.2f} s", file=sys.stderr, flush=True)
+
+    for f in range(args.warmup):
+        tw = time.perf_counter()
         g.physics_step_gpu(1)
+        progress("warmup", f, g.stats(), time.perf_counter() - tw)
     g.kernel_times(reset=True)
     hiprt.synchronize()
     newton = pcg = 0
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for f in range(args.steps):
+        tf = time.perf_counter()
         g.physics_step_gpu(1)
         st = g.stats()
         newton += st["newton_iters"]
         pcg += st["pcg_iters_total"]
+        progress("timed", f, st, time.perf_counter() - tf)
     hiprt.synchronize()
     elapsed = time.perf_counter() - t0
     kt = g.kernel_times()

@@ -76,7 +76,7 @@ typedef struct lcsx_params
     int   newton_max_iters;     /* hard cap (reference: 100)                                 */
     int   pcg_iter_count;       /* PCG iteration cap (reference default 100)                 */
     int   pcg_check_every;      /* PCG convergence test interval (reference: 25)             */
-    float pcg_rel_tol;          /* stop when r.z <= tol^2 * r0.z0                            */
+    float pcg_rel_tol;          /* also stop when r.z <= tol^2 * r0.z0 (0: off, reference)   */
     float pcg_abs_tol;          /* ... or r.z < abs_tol (reference: 1e-10)                   */
     float gravity[3];
     int   use_floor;
@@ -119,7 +119,7 @@ int lcsx_device_count(void);
  * (reference defaults). device selects the HIP device (ordinal). */
 lcsx_status lcsx_solver_create(const lcsx_params* params, int device, lcsx_solver** out);
 lcsx_status lcsx_solver_destroy(lcsx_solver* s);
-/* Default parameters (scene_params.h defaults, with pcg_rel_tol 1e-4 and check every iteration). */
+/* Default parameters (scene_params.h defaults; PCG stops on r.z < 1e-10 tested every 25 iterations). */
 void lcsx_params_default(lcsx_params* p);
 void lcsx_material_default(lcsx_material* m);
 

@@ -192,8 +192,8 @@ class NewtonSolver:
             for fp in m.fixed:
                 s.add_fixed_points(mid, fp.method, fp.range, fp.anim)
         s.init_solver()
-        if scene.initial_velocity is not None:
-            s.set_state(v=scene.initial_velocity)
+        if scene.initial_position is not None or scene.initial_velocity is not None:
+            s.set_state(scene.initial_position, scene.initial_velocity)
         return s
 
     def __del__(self):

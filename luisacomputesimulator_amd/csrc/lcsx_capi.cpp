@@ -282,8 +282,8 @@ void lcsx_params_default(lcsx_params* p)
     p->nonlinear_iter_count = 3;
     p->newton_max_iters     = 100;
     p->pcg_iter_count       = 100;
-    p->pcg_check_every      = 1;
-    p->pcg_rel_tol          = 1e-4f;
+    p->pcg_check_every      = 25;
+    p->pcg_rel_tol          = 0.0f;
     p->pcg_abs_tol          = 1e-10f;
     p->gravity[1]           = -9.8f;
     p->use_floor            = 1;

@@ -391,7 +391,8 @@ void Solver::pcg()
     int         par    = 0;
     while (issued < prm.pcg_iter_count)
     {
-        int chunk = std::min(kPcgChunk, prm.pcg_iter_count - issued);
+        // one host check per convergence-test interval (at least kPcgChunk launches)
+        int chunk = std::min(std::max(kPcgChunk, a.check_every > 1 ? (int)a.check_every : 0), prm.pcg_iter_count - issued);
         for (int k = 0; k < chunk; k++)
         {
             PcgArgs b = a;

@@ -46,8 +46,8 @@ struct MeshIn
 struct Params
 {
     double dt = 1.0 / 60.0;
-    int    newton_min = 3, newton_max = 100, pcg_max = 100, pcg_check_every = 1;
-    double pcg_rel_tol = 1e-4, pcg_abs_tol = 1e-10;
+    int    newton_min = 3, newton_max = 100, pcg_max = 100, pcg_check_every = 25;
+    double pcg_rel_tol = 0.0, pcg_abs_tol = 1e-10;
     double gravity[3] = {0, -9.8, 0};
     int    use_floor = 1, use_self_collision = 1, use_ccd = 1;
     double floor_y = 0.0, kappa = 1e7, k_dirichlet = 1e9, damping = 2.0, bending_scaling = 1.0, newton_tol = 1e-2;
@@ -771,8 +771,9 @@ struct Scene
         return a[0];
     }
 
-    // precond_cg.cpp:626-771 with a relative stopping rule rz_k <= tol^2 rz_0 or rz_k < abs_tol
-    // checked every pcg_check_every iterations (the reference checks rz < 1e-10 every 25).
+    // precond_cg.cpp:626-771: stop when rz_k < abs_tol (the reference: 1e-10), tested every
+    // pcg_check_every iterations (the reference: 25, L694-726); an optional relative test
+    // rz_k <= tol^2 rz_0 (pcg_rel_tol > 0) is tested at the same iterations.
     void pcg()
     {
         std::vector<M3<T>> Minv(nv);

@@ -86,8 +86,12 @@ class Oracle:
                 an = np.asarray(fp.anim.pack(), dtype=np.float64)
                 self.lib.orc_pin_method(self.h, mi, FIXED_METHODS[fp.method], fp.range, _d(an), len(an))
         self.stage("init")
+        if getattr(scene, "initial_position", None) is not None:
+            self.set("x_outer", scene.initial_position)
+            self.set("x", scene.initial_position)
         if getattr(scene, "initial_velocity", None) is not None:
             self.set("v_outer", scene.initial_velocity)
+            self.set("v", scene.initial_velocity)
 
     def __del__(self):
         try:
