@@ -36,15 +36,34 @@ def spmv_bytes(n_rows: int, n_blocks: int) -> int:
 
 
 def traffic_from_profiles():
-    """HBM bytes per SpMV launch from the committed rocprofv3 counter summary, if present."""
-    path = os.path.join(ROOT, "profiles", "spmv_tube88k_counters.json")
+    """HBM bytes per SpMV launch from the committed rocprofv3 counter summary, if present
+    (the line `hbm_bytes_per_launch: N` of profiles/k_spmv_pcg_tube88k_counters.md)."""
+    path = os.path.join(ROOT, "profiles", "k_spmv_pcg_tube88k_counters.md")
     if not os.path.exists(path):
         return None
-    try:
-        with open(path) as f:
-            return json.load(f).get("hbm_bytes_per_launch")
-    except Exception:
-        return None
+    with open(path) as f:
+        for line in f:
+            if line.strip().startswith("hbm_bytes_per_launch:"):
+                return float(line.split(":", 1)[1].strip())
+    return None
+
+
+def profile_pcg(g, copy_mib: int = 256) -> None:
+    """Counter-run workload (rocprofv3 --pmc): one device-to-device copy of `copy_mib` MiB (the
+    calibration kernel of known size) followed by one PCG solve (up to 100 SpMV launches) on the
+    system of the last Newton iteration."""
+    import numpy as np
+
+    from luisacomputesimulator_amd import hiprt
+
+    src = hiprt.DeviceArray(np.zeros(copy_mib << 18, dtype=np.float32))
+    dst = hiprt.DeviceArray(np.zeros(copy_mib << 18, dtype=np.float32))
+    hiprt.copy_device(dst, src)
+    hiprt.synchronize()
+    g.stage("pcg")
+    hiprt.synchronize()
+    st = g.stats()
+    print(f"[bench] profile-pcg: copied {copy_mib} MiB, PCG {st['pcg_iters']} iterations", file=sys.stderr, flush=True)
 
 
 def cpu_baseline(scene):
@@ -68,6 +87,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--rings", type=int, default=297)
+    ap.add_argument("--profile-pcg", action="store_true",
+                    help="after the warm-up frames run one calibration copy and one PCG solve, then exit "
+                         "(workload for rocprofv3 counter runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the workload is a single scene on one MI355X and the solver "
@@ -90,6 +112,9 @@ def main():
         tw = time.perf_counter()
         g.physics_step_gpu(1)
         progress("warmup", f, g.stats(), time.perf_counter() - tw)
+    if args.profile_pcg:
+        profile_pcg(g)
+        return
     g.kernel_times(reset=True)
     hiprt.synchronize()
     newton = pcg = 0

@@ -173,10 +173,20 @@ int         lcsx_solver_num_verts(lcsx_solver* s);
  *   "spmv_dx"          dx <- A dx (the system matrix applied to the dx buffer)
  */
 lcsx_status lcsx_solver_run_stage(lcsx_solver* s, const char* stage);
-/* Named solver buffers, converted to a canonical host layout (see DESIGN.md for the list):
- * float arrays are FP32, index arrays uint32, morton keys uint64. Off-diagonal blocks are
- * returned in the reference's CSR slot order (row-major 3x3 per block). count_out receives the
- * element count; with out == NULL only the count is reported. */
+/* Named solver buffers, converted to a canonical host layout: float arrays are FP32, index
+ * arrays uint32, morton keys uint64. Off-diagonal blocks are returned in the reference's CSR slot
+ * order (row-major 3x3 per block). count_out receives the element count; with out == NULL only
+ * the count is reported. Names:
+ *   state      x, v, x_step, x_iter, x_tilde, dx, cgB, diag, offdiag, toi
+ *   setup      rest_x, faces, edges, hinges, csr, stretch_faces, bending_hinges, fixed_verts,
+ *              is_fixed, face_area, edge_area, vert_area, vert_mass, offset, dhat, sf_Dm, sf_area,
+ *              sf_mu, sf_lambda, bh_angle, bh_hbar, bh_k (readable after a failed init for want
+ *              of a device, since the host setup is built first)
+ *   trees      face_tree.* / edge_tree.* with morton, morton_sorted, sorted_to_orig, parent,
+ *              children, object_idx, node_aabb
+ *   contacts   pairs_vf, pairs_ee (last broad phase), contact_idx, contact_type, contact_n,
+ *              contact_k (k1, k2, kappa * area), contact_w, contact_bary
+ * lcsx_solver_write accepts x, v, x_step, x_iter, x_tilde, dx and cgB. */
 lcsx_status lcsx_solver_read(lcsx_solver* s, const char* name, void* out, size_t bytes, size_t* count_out);
 lcsx_status lcsx_solver_write(lcsx_solver* s, const char* name, const void* in, size_t bytes);
 /* Average device time (ms) of each kernel group over the calls since the last reset

@@ -423,7 +423,7 @@ lcsx_status lcsx_solver_get_stats(lcsx_solver* s, lcsx_stats* out)
     *out = s->s.stats;
     return LCSX_OK;
 }
-int lcsx_solver_num_verts(lcsx_solver* s) { return s && s->s.inited ? (int)s->s.S.nv : 0; }
+int lcsx_solver_num_verts(lcsx_solver* s) { return s && (s->s.inited || s->s.setup_ready) ? (int)s->s.S.nv : 0; }
 
 lcsx_status lcsx_solver_run_stage(lcsx_solver* s, const char* stage)
 {

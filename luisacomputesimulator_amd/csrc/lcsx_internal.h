@@ -76,6 +76,9 @@ struct Lbvh
     DBuf<uint32_t>    parent, object_idx, counters, gbox;
     DBuf<uint2>       children;
     DBuf<NodeBox>     nodes;
+    // per inner node: both child boxes with the child codes in the .w lanes (inner child: its
+    // node id; leaf child: 0x80000000 | element id), so a traversal step is one 64 B load
+    DBuf<float4>      packed;
     uint32_t*         sorted_to_orig = nullptr;  // points into vals[?] after construct
 
     void init(int arity_, const uint32_t* device_prims, uint32_t n_leaves);

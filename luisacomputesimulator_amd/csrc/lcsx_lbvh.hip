@@ -278,7 +278,7 @@ __global__ __launch_bounds__(kBlock) void k_refit(uint32_t n, int arity, const u
                                                    const float* __restrict__ x0, const float* __restrict__ x1,
                                                    const float* __restrict__ pad, const uint32_t* __restrict__ parent,
                                                    const uint2* __restrict__ children, NodeBox* nodes,
-                                                   uint32_t* counters)
+                                                   float4* __restrict__ packed, uint32_t* counters)
 {
     uint32_t l = blockIdx.x * kBlock + threadIdx.x;
     if (l >= n) return;
@@ -322,23 +322,40 @@ __global__ __launch_bounds__(kBlock) void k_refit(uint32_t n, int arity, const u
         U.lo     = make_float4(smin(A.lo.x, B.lo.x), smin(A.lo.y, B.lo.y), smin(A.lo.z, B.lo.z), 0.0f);
         U.hi     = make_float4(smax(A.hi.x, B.hi.x), smax(A.hi.y, B.hi.y), smax(A.hi.z, B.hi.z), 0.0f);
         nodes[p] = U;
-        cur      = p;
+        // child codes: leaves keep (element, element) in children[]
+        uint32_t ca = c.x >= ni ? (0x80000000u | children[c.x].x) : c.x;
+        uint32_t cb = c.y >= ni ? (0x80000000u | children[c.y].x) : c.y;
+        float4*  P  = packed + 4 * (size_t)p;
+        P[0] = make_float4(A.lo.x, A.lo.y, A.lo.z, __uint_as_float(ca));
+        P[1] = make_float4(A.hi.x, A.hi.y, A.hi.z, 0.0f);
+        P[2] = make_float4(B.lo.x, B.lo.y, B.lo.z, __uint_as_float(cb));
+        P[3] = make_float4(B.hi.x, B.hi.y, B.hi.z, 0.0f);
+        cur  = p;
     }
+}
+
+__global__ void k_fill_u32(uint32_t* a, uint32_t n, uint32_t v)
+{
+    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) a[i] = v;
 }
 
 // ---------------------------------------------------------------- traversal
 constexpr int kQueryBlock = 128;
 constexpr int kStack      = 32;
 
-LCSX_D bool overlap(const NodeBox& b, f3 lo, f3 hi)
+// query_template2, lbvh.cpp:293-360: depth-first with a 32-entry stack staged in LDS; a full
+// stack ends the query (the reference's silent truncation), reported through *overflow.
+LCSX_D bool overlap4(float4 lo4, float4 hi4, f3 lo, f3 hi)
 {
-    return (b.lo.x < hi.x && b.lo.y < hi.y && b.lo.z < hi.z) && (b.hi.x > lo.x && b.hi.y > lo.y && b.hi.z > lo.z);
+    return (lo4.x < hi.x && lo4.y < hi.y && lo4.z < hi.z) && (hi4.x > lo.x && hi4.y > lo.y && hi4.z > lo.z);
 }
 
 // query_template2, lbvh.cpp:293-360: depth-first with a 32-entry stack staged in LDS; a full
-// stack ends the query (the reference's silent truncation), reported through *overflow.
-__global__ __launch_bounds__(kQueryBlock) void k_query(QueryArgs a, const NodeBox* __restrict__ nodes,
-                                                        const uint2* __restrict__ children, uint32_t n_inner)
+// stack ends the query (the reference's silent truncation), reported through *overflow. Each
+// step loads one packed inner node (both child boxes and codes); children are tested and inner
+// hits pushed in the reference's order, so pairs and overflow behave as the reference's.
+__global__ __launch_bounds__(kQueryBlock) void k_query(QueryArgs a, const float4* __restrict__ packed, uint32_t n_inner)
 {
     __shared__ uint32_t stack[kStack][kQueryBlock];
     uint32_t            q = blockIdx.x * kQueryBlock + threadIdx.x;
@@ -380,17 +397,22 @@ __global__ __launch_bounds__(kQueryBlock) void k_query(QueryArgs a, const NodeBo
             if (a.overflow) atomicOr(a.overflow, 2u);
             break;
         }
-        uint32_t nd  = stack[--sp][tid];
-        uint2    ch  = children[nd];
-        bool     brk = false;
+        uint32_t      nd = stack[--sp][tid];
+        const float4* P  = packed + 4 * (size_t)nd;
+        float4        r0 = P[0], r1 = P[1], r2 = P[2], r3 = P[3];
+        bool          brk = false;
+#pragma unroll
         for (int ii = 0; ii < 2; ii++)
         {
-            uint32_t c    = ii == 0 ? ch.x : ch.y;
-            bool     hit  = overlap(nodes[c], lo, hi);
-            bool     leaf = c >= n_inner;
-            uint32_t obj  = 0;
-            if (hit && leaf) obj = children[c].x;  // leaves store (orig, orig)
-            bool emit = hit && leaf && (!filt || q < obj);
+            float4   blo  = ii == 0 ? r0 : r2;
+            float4   bhi  = ii == 0 ? r1 : r3;
+            uint32_t code = __float_as_uint(blo.w);
+            bool     hit  = overlap4(blo, bhi, lo, hi);
+            bool     leaf = (code & 0x80000000u) != 0u;
+            uint32_t obj  = code & 0x7fffffffu;
+            // a full stack at the first child ends the query before the second is tested
+            hit           = hit && !brk;
+            bool     emit = hit && leaf && (!filt || q < obj);
             uint32_t slot = wave_append(a.count, emit);
             if (emit && slot < a.capacity)
             {
@@ -400,12 +422,11 @@ __global__ __launch_bounds__(kQueryBlock) void k_query(QueryArgs a, const NodeBo
             if (hit && !leaf)
             {
                 if (sp < kStack)
-                    stack[sp++][tid] = c;
+                    stack[sp++][tid] = code;
                 else
                 {
                     brk = true;
                     if (a.overflow) atomicOr(a.overflow, 1u);
-                    break;
                 }
             }
         }
@@ -429,6 +450,7 @@ void Lbvh::init(int arity_, const uint32_t* device_prims, uint32_t n_leaves)
     object_idx.alloc(n_nodes());
     children.alloc(n_nodes());
     nodes.alloc(n_nodes());
+    packed.alloc(4 * (size_t)(n_inner() ? n_inner() : 1));
     counters.alloc(n_inner() ? n_inner() : 1);
     gbox.alloc(6);
     sorted_to_orig = vals[0].p;
@@ -461,16 +483,17 @@ void Lbvh::construct(const float* x, uint32_t n_verts_for_reduce, hipStream_t s)
 void Lbvh::refit(const float* x0, const float* x1, const float* pad, hipStream_t s)
 {
     if (n == 0) return;
-    LCSX_HIP(hipMemsetAsync(counters.p, 0, counters.n * sizeof(uint32_t), s));
+    k_fill_u32<<<grid_for((uint32_t)counters.n, kBlock), kBlock, 0, s>>>(counters.p, (uint32_t)counters.n, 0u);
     k_refit<<<grid_for(n, kBlock), kBlock, 0, s>>>(n, arity, prims, sorted_to_orig, x0, x1, pad, parent.p, children.p,
-                                                   nodes.p, counters.p);
+                                                   nodes.p, packed.p, counters.p);
     LCSX_HIP(hipGetLastError());
 }
 
 void lbvh_query(const Lbvh& t, const QueryArgs& a, hipStream_t s)
 {
     if (t.n == 0 || a.n_queries == 0) return;
-    k_query<<<grid_for(a.n_queries, kQueryBlock), kQueryBlock, 0, s>>>(a, t.nodes.p, t.children.p, t.n_inner());
+    if (t.n < 2) return;  // a single leaf has no inner node to start from (the solver never queries one)
+    k_query<<<grid_for(a.n_queries, kQueryBlock), kQueryBlock, 0, s>>>(a, t.packed.p, t.n_inner());
     LCSX_HIP(hipGetLastError());
 }
 

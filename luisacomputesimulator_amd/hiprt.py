@@ -35,6 +35,13 @@ def _ok(code: int, what: str) -> None:
         raise RuntimeError(f"{what} failed: {_hip().hipGetErrorString(code).decode()}")
 
 
+def copy_device(dst: "DeviceArray", src: "DeviceArray") -> None:
+    """Device-to-device copy of src into dst (same size)."""
+    if dst.nbytes != src.nbytes:
+        raise ValueError("copy_device: sizes differ")
+    _ok(_hip().hipMemcpy(dst.ptr, src.ptr, src.nbytes, 3), "hipMemcpy")
+
+
 def synchronize() -> None:
     _ok(_hip().hipDeviceSynchronize(), "hipDeviceSynchronize")
 

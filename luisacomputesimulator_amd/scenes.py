@@ -265,6 +265,110 @@ def small_two_sheets(n: int = 9, gap: float = 2.5e-3) -> Scene:
     return Scene("two_sheets", [m0, m1], Params(use_floor=False, stiffness_collision=1e7), frames=10)
 
 
+def read_obj(path: str):
+    """Vertices (float32) and fan-triangulated faces of an OBJ file, as the C++ loader reads them."""
+    verts, tris = [], []
+    with open(path) as f:
+        for line in f:
+            tok = line.split()
+            if not tok:
+                continue
+            if tok[0] == "v":
+                verts.append([np.float32(t) for t in tok[1:4]])
+            elif tok[0] == "f":
+                poly = []
+                for t in tok[1:]:
+                    idx = int(t.split("/")[0])
+                    poly.append((len(verts) + idx) if idx < 0 else idx - 1)
+                for k in range(1, len(poly) - 1):
+                    tris.append([poly[0], poly[k], poly[k + 1]])
+    return np.asarray(verts, dtype=np.float32).reshape(-1, 3), np.asarray(tris, dtype=np.int32).reshape(-1, 3)
+
+
+def model_transform(x: np.ndarray, translation=None, rotation=None, scale=None) -> np.ndarray:
+    """translate * Rx * Ry * Rz * scale applied to (n, 3) float32 points, in float32."""
+    f = np.float32
+    tr = np.asarray(translation if translation is not None else (0, 0, 0), dtype=np.float32)[:3]
+    rot = np.asarray(rotation if rotation is not None else (0, 0, 0), dtype=np.float32)[:3]
+    if scale is None:
+        sc = np.ones(3, dtype=np.float32)
+    elif isinstance(scale, (int, float)):
+        sc = np.full(3, scale, dtype=np.float32)
+    else:
+        sc = np.asarray(scale, dtype=np.float32)[:3]
+    if not (tr.any() or rot.any() or (sc != 1).any()):
+        return x
+    cx, sx, cy, sy, cz, sz = (f(np.cos(rot[0])), f(np.sin(rot[0])), f(np.cos(rot[1])), f(np.sin(rot[1])),
+                              f(np.cos(rot[2])), f(np.sin(rot[2])))
+    p0, p1, p2 = x[:, 0] * sc[0], x[:, 1] * sc[1], x[:, 2] * sc[2]
+    x1, y1, z1 = cz * p0 - sz * p1, sz * p0 + cz * p1, p2
+    x2, y2, z2 = cy * x1 + sy * z1, y1, -sy * x1 + cy * z1
+    y3, z3 = cx * y2 - sx * z2, sx * y2 + cx * z2
+    return np.stack([x2 + tr[0], y3 + tr[1], z3 + tr[2]], -1).astype(np.float32)
+
+
+def from_json(path: str, mesh_dir: str) -> Scene:
+    """A scene from the reference's scene JSON (Application/app_simulation_demo_config.cpp:174-745):
+    scene keys, cloth shells with their OBJ mesh, model transform, material and fixed points.
+    The model transform is translate * Rx * Ry * Rz * scale (Core/affine_position.h:41-98),
+    evaluated in float32 in the C loader's order."""
+    import json
+    import os
+
+    with open(path) as f:
+        js = json.load(f)
+    prm = Params()
+    for key, attr in (("implicit_dt", "implicit_dt"), ("stiffness_collision", "stiffness_collision"),
+                      ("stiffness_dirichlet", "stiffness_dirichlet")):
+        if isinstance(js.get(key), (int, float)) and not isinstance(js.get(key), bool):
+            setattr(prm, attr, float(js[key]))
+    for key in ("nonlinear_iter_count", "pcg_iter_count"):
+        if isinstance(js.get(key), (int, float)) and not isinstance(js.get(key), bool):
+            setattr(prm, key, int(js[key]))
+    for key in ("use_floor", "use_self_collision", "use_ccd_linesearch"):
+        if isinstance(js.get(key), bool):
+            setattr(prm, key, js[key])
+    if isinstance(js.get("gravity"), list):
+        prm.gravity = tuple(float(g) for g in js["gravity"][:3])
+    meshes = []
+    for sh in js.get("shells", []):
+        name = sh["model_name"]
+        cand = os.path.join(mesh_dir, name)
+        if not os.path.exists(cand):
+            cand = os.path.join(mesh_dir, os.path.basename(name))
+        verts, tris = read_obj(cand)
+        verts = model_transform(verts, sh.get("translation"), sh.get("rotation"), sh.get("scale"))
+        mat = Material()
+        m = sh.get("material")
+        if isinstance(m, list) and m:
+            m = m[0]
+        if isinstance(m, dict):
+            for key in ("thickness", "youngs_modulus", "poisson_ratio", "area_bending_stiffness", "mass",
+                        "density", "d_hat"):
+                if isinstance(m.get(key), (int, float)):
+                    setattr(mat, key, float(m[key]))
+        fixed = []
+        for fp in sh.get("fixed_points", []):
+            fi = fp.get("fixed_info", {})
+            an = Anim(use_translate=bool(fi.get("use_translate", False)), use_rotate=bool(fi.get("use_rotate", False)),
+                      use_scale=bool(fi.get("use_scale", False)))
+            if "translate" in fi:
+                an.translate = tuple(fi["translate"][:3])
+            if "rotCenter" in fi:
+                an.rot_center = tuple(fi["rotCenter"][:3])
+            if "rotAxis" in fi:
+                an.rot_axis = tuple(fi["rotAxis"][:3])
+            if "rotAngVelDeg" in fi:
+                an.rot_deg_per_s = float(fi["rotAngVelDeg"])
+            if "scale" in fi:
+                an.scale = tuple(fi["scale"][:3])
+                an.use_scale = True
+            rng = fp.get("range", [0.001])
+            fixed.append(FixedPoints(fp.get("method", "All"), float(rng[0]) if rng else 0.001, an))
+        meshes.append(Mesh(os.path.splitext(os.path.basename(name))[0], verts, tris, mat, fixed))
+    return Scene(os.path.splitext(os.path.basename(path))[0], meshes, prm, frames=1)
+
+
 def by_name(name: str, **kw) -> Scene:
     table = {"square": square_cloth, "tube": cylinder_tube, "twist": twisting_sheet, "two_sheets": small_two_sheets}
     return table[name](**kw)

@@ -244,3 +244,29 @@ def test_newton_iteration_in_contact_vs_fp64(require_gpu):
     err_o32 = np.abs(o.get("x").reshape(-1, 3) - x64).max() / scale
     print(f"[gpu-parity] contact iteration error vs FP64: gpu {err_gpu:.3e}, FP32 oracle {err_o32:.3e}", flush=True)
     assert err_gpu <= 3.0 * err_o32 + 1e-6
+
+
+def test_reference_cylinder_7k_scene_frames(require_gpu):
+    """The reference's own 7K rotating-cylinder scene (cloth_rotation_cylinder_7K.json, loaded by the
+    C scene loader) for 4 frames. PCG stops at the scene's 50-iteration cap, so results depend on
+    summation order: the FP32 oracle drifts from its FP64 twin by ~1e-5 by frame 3. Bar per frame:
+    the GPU's error against the FP64 twin is at most three times the FP32 oracle's + 1e-6."""
+    import os
+    here = os.path.dirname(os.path.abspath(__file__))
+    sdir = os.path.join(here, "golden", "scenes")
+    path = os.path.join(sdir, "cloth_rotation_cylinder_7K.json")
+    sc = scenes.from_json(path, sdir)
+    g = NewtonSolver()
+    g.load_scene_json(path, sdir)
+    g.init_solver()
+    o, o64 = oracle.Oracle(sc, 32), oracle.Oracle(sc, 64)
+    for f in range(4):
+        g.physics_step_gpu(1)
+        o.step()
+        o64.step()
+        x64 = o64.get("x").reshape(-1, 3)
+        scale = np.abs(x64).max()
+        err_gpu = np.abs(g.get_simulation_results() - x64).max() / scale
+        err_o32 = np.abs(o.get("x").reshape(-1, 3) - x64).max() / scale
+        print(f"[gpu-parity] cylinder7K frame {f}: gpu {err_gpu:.3e}, FP32 oracle {err_o32:.3e}, {g.stats()}", flush=True)
+        assert err_gpu <= 3.0 * err_o32 + 1e-6

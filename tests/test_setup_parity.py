@@ -46,7 +46,7 @@ def test_setup_bit_exact(name):
 
 
 def test_reference_sizes_square_cloth():
-    # SURVEY-derived counts for the 33 x 33 grid: 3,136 edges, 3,008 hinges, 10,368 material blocks
+    # closed-form counts for the 33 x 33 grid: 3,136 edges, 3,008 hinges, 10,368 material blocks
     sc = scenes.square_cloth(33)
     s = product_setup(sc)
     assert len(s.read("edges")) // 2 == 3136
