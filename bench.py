@@ -9,11 +9,12 @@ frame runs >= 3 Newton iterations, each with <= 100 PCG iterations); `value` is 
 iterations per second over the timed frames, with every input resident on the GPU.
 
 The JSON line carries:
-  roofline      the PCG block-SpMV (dominant kernel): algorithmic bytes per launch
-                40 B per off-diagonal block (36 B values + 4 B column) + 84 B per row
-                (36 B diagonal block, z and p_old of the row 24 B, p_new and q writes 24 B)
-                + 4 B per 64-row slice, divided by the launch's average duration measured
-                with hipEvents on the solver stream; peak = 8 TB/s HBM3E.
+  roofline      the PCG block SpMV k_spmv_pcg (material blocks and contact slots): algorithmic
+                bytes per launch = 40 B per off-diagonal block (36 B values + 4 B column) +
+                84 B per row (36 B diagonal block, z and p_old of the row 24 B, p_new and q
+                writes 24 B) + 4 B per 64-row slice + 24 B per contact slot,
+                divided by the launch's average duration from hipEvents attached to the
+                launches (hipExtLaunchKernelGGL) on the solver stream; peak = 8 TB/s HBM3E.
   cpu_baseline  the CPU oracle (C++ restatement of the reference path, 1 thread) timed on
                 frame 0 of the same workload on this host.
 """
@@ -30,9 +31,11 @@ METRIC = "Newton steps/s @88K-vert cloth, mixed vs FP32; SpMV HBM GB/s; scene-st
 HBM_PEAK_GBS = 8000.0
 
 
-def spmv_bytes(n_rows: int, n_blocks: int) -> int:
-    """40 B per off-diagonal block + 84 B per row + 4 B per 64-row slice offset."""
-    return 40 * n_blocks + 84 * n_rows + 4 * ((n_rows + 63) // 64)
+def spmv_bytes(n_rows: int, n_blocks: int, slots: float = 0.0) -> float:
+    """Algorithmic bytes of one k_spmv_pcg launch: 40 B per off-diagonal block + 84 B per row +
+    4 B per 64-row slice offset, and 24 B per contact slot (16 B coefficient, 4 B row, the
+    contact's 4 B t_c)."""
+    return 40 * n_blocks + 84 * n_rows + 4 * ((n_rows + 63) // 64) + 24 * slots
 
 
 def traffic_from_profiles():
@@ -130,7 +133,8 @@ def main():
     elapsed = time.perf_counter() - t0
     kt = g.kernel_times()
     spmv_ms, spmv_n = kt.get("spmv", (float("nan"), 0))
-    nbytes = spmv_bytes(n_rows, n_blocks)
+    slots = kt.get("spmv_slots", (0.0, 0))[0]
+    nbytes = spmv_bytes(n_rows, n_blocks, slots)
     achieved = nbytes / (spmv_ms * 1e-3) / 1e9 if spmv_n else None
     st = g.stats()
     line = {
@@ -157,8 +161,9 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": traffic_from_profiles(),
                      "kernel": "k_spmv_pcg", "bytes_per_launch": nbytes, "avg_launch_ms": spmv_ms,
+                     "contact_slots_per_launch": slots,
                      "launches": spmv_n},
-        "kernel_ms": {k: round(v[0], 4) for k, v in kt.items()},
+        "kernel_ms": {k: round(v[0], 4) for k, v in kt.items() if k != "spmv_slots"},
     }
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(sc)

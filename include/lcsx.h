@@ -192,8 +192,9 @@ lcsx_status lcsx_solver_write(lcsx_solver* s, const char* name, const void* in, 
 /* Average device time (ms) of each kernel group over the calls since the last reset
  * (hipEvent timing on the solver's stream); names are written as "name=ms;..." into buf. */
 lcsx_status lcsx_solver_kernel_times(lcsx_solver* s, char* buf, size_t buf_len, int reset);
-/* Timing level: 0 off, 1 stage groups (default), 2 also one event pair around every PCG SpMV
- * launch (reported as "spmv"; only launches before convergence are counted). */
+/* Timing level: 0 off, 1 stage groups (default), 2 also the device time of every PCG SpMV launch
+ * (events attached to the launch; reported as "spmv", only launches before convergence are
+ * counted) and the contact slots those launches processed ("spmv_slots", slots per launch). */
 lcsx_status lcsx_solver_set_timing(lcsx_solver* s, int level);
 
 /* ----------------------------------------------- standalone LBVH (device pointers)

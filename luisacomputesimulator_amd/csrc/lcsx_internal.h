@@ -79,6 +79,11 @@ struct Lbvh
     // per inner node: both child boxes with the child codes in the .w lanes (inner child: its
     // node id; leaf child: 0x80000000 | element id), so a traversal step is one 64 B load
     DBuf<float4>      packed;
+    // level-ordered refit: node heights, inner nodes sorted by height, level offsets
+    DBuf<uint32_t>        height, order, level_hist, level_ptr, level_cursor;
+    std::vector<uint32_t> level_ptr_h;
+    uint32_t              levels = 0, tail_level = 0;  // highest height; first height refit in one workgroup
+    bool                  use_levels = false;          // false: one-pass ticket refit (very tall trees)
     uint32_t*         sorted_to_orig = nullptr;  // points into vals[?] after construct
 
     void init(int arity_, const uint32_t* device_prims, uint32_t n_leaves);

@@ -41,6 +41,16 @@ struct ContactView
     const uint32_t* count;
     uint32_t        capacity;
 };
+// Per-vertex contact slots: entries list[ptr[i] .. ptr[i+1]) are (contact << 2 | slot); ptr null
+// means no contacts.
+struct VertContacts
+{
+    const uint32_t* ptr;
+    const uint32_t* list;
+    const uint32_t* row;   // per slot: its vertex
+    const float4*   coef;  // per slot: (k2 w_j n, contact id bits)
+    float*          tc;    // per contact: sum_j w_j n.p_j of the vector being multiplied
+};
 struct ContactBuf
 {
     uint4*    idx;
@@ -71,6 +81,7 @@ struct VertArgs
     const float*    hinge_stage;
     float*          cgB;
     float*          diag;
+    float*          diag_mat;  // optional copy of the material diagonal (the SpMV's; contacts go to diag only)
 };
 
 struct DcdArgs
@@ -147,16 +158,23 @@ void launch_vert_assemble(const VertArgs& a, hipStream_t s);
 void launch_offdiag(uint32_t nblocks, const uint32_t* ptr, const uint32_t* con, const float* fs, const float* hs, float* val,
                     hipStream_t s);
 void launch_dcd(const DcdArgs& a, int kind, unsigned grid, hipStream_t s);
-void launch_contact_assemble(const ContactView& c, float* cgB, float* diag, uint32_t nv, unsigned grid, hipStream_t s);
+// Per-vertex contact lists (cnt: nv + 1 scratch counters, ptr: nv + 1, list: 4 per contact).
+void launch_contact_adjacency(const ContactView& c, uint32_t nv, unsigned grid, uint32_t* cnt, uint32_t* ptr,
+                              uint32_t* list, uint32_t* row, float4* coef, hipStream_t s);
+void launch_contact_assemble(const ContactView& c, const VertContacts& vc, float* cgB, float* diag, uint32_t nv,
+                             hipStream_t s);
 void launch_pcg_init(const PcgArgs& a, hipStream_t s);
-// one PCG iteration = spmv (material part, forms p = z + beta p on the fly), contact part, finish
-// (alpha, x/r/z update, beta and the convergence flag)
-void launch_pcg_spmv(const PcgArgs& a, const SellView& A, hipStream_t s);
-void launch_pcg_contact_spmv(const PcgArgs& a, const ContactView& c, unsigned contact_grid, hipStream_t s);
-void launch_pcg_finish(const PcgArgs& a, hipStream_t s);
+// One PCG iteration = up to three launches: t_c per contact (only with contacts), the SpMV
+// (p = z + beta p formed on the fly, material blocks and contact slots, p.q and alpha), then the
+// update (x, r, z, r.z, beta, convergence flag). With start/stop events the SpMV is launched with
+// hipExtLaunchKernelGGL, which timestamps the kernel's own execution.
+void launch_pcg_contact_t(const PcgArgs& a, const ContactView& c, float* tc, unsigned grid, hipStream_t s);
+void launch_pcg_spmv(const PcgArgs& a, const SellView& A, const VertContacts& vc, hipStream_t s,
+                     hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
+void launch_pcg_update(const PcgArgs& a, hipStream_t s);
 // y = A p (diagonal + material blocks + contacts), for the spmv stage
-void launch_spmv(const SellView& A, const ContactView* c, unsigned contact_grid, const float* p, float* y, uint32_t n,
-                 hipStream_t s);
+void launch_spmv(const SellView& A, const ContactView& c, const VertContacts& vc, unsigned contact_grid, const float* p,
+                 float* y, uint32_t n, hipStream_t s);
 void launch_apply_dx(uint32_t nv, const float* x_iter, const float* dx, float* x, const uint32_t* alpha_key,
                      float alpha_const, hipStream_t s);
 void launch_ground_ccd_maxdx(const LsArgs& a, hipStream_t s);

@@ -1,6 +1,7 @@
 // Newton-step kernels for gfx950: material evaluation and owner-computes assembly, DCD/CCD
 // narrow phase, contact assembly, SELL-64 block SpMV, block-Jacobi PCG and line search.
 // Reference counterparts are cited per kernel (paths relative to the reference root).
+#include <hip/hip_ext.h>
 #include "lcsx_kernels.h"
 #include "lcsx_physics.h"
 
@@ -205,6 +206,7 @@ __global__ __launch_bounds__(kB) void k_vert_assemble(VertArgs a)
     }
     st3(a.cgB, i, grad);
     st_planes(a.diag, a.nv, i, H);
+    if (a.diag_mat) st_planes(a.diag_mat, a.nv, i, H);
 }
 
 // Off-diagonal blocks: one thread per SELL block sums its contributors in order
@@ -332,30 +334,170 @@ __global__ __launch_bounds__(kB) void k_dcd_ee(DcdArgs a)
     }
 }
 
-// fn_perPair_assemble_gradient_hessian, narrow_phase.cpp:2550-2676 (soft bodies)
-__global__ __launch_bounds__(kB) void k_contact_assemble(ContactView c, float* __restrict__ cgB, float* __restrict__ diag,
-                                                          uint32_t nv)
+__global__ void k_fill_u32(uint32_t* a, uint32_t n, uint32_t v)
+{
+    uint32_t i = blockIdx.x * kB + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+// ---------------------------------------------------------------- contact adjacency
+// Per-vertex lists of (contact << 2 | slot), built once per Newton iteration after the DCD, so
+// that contact terms are gathered by the rows that own them instead of scattered with atomics.
+__global__ __launch_bounds__(kB) void k_vc_count(ContactView c, uint32_t* __restrict__ cnt)
 {
     uint32_t n = min(*c.count, c.capacity);
     for (uint32_t k = blockIdx.x * kB + threadIdx.x; k < n; k += gridDim.x * kB)
     {
-        uint4  id = c.idx[k];
-        float4 nk = c.nk[k], w4 = c.w[k], ax = c.aux[k];
-        f3     nn  = mk(nk.x, nk.y, nk.z);
-        float  nv3[3] = {nk.x, nk.y, nk.z};
+        uint4 id = c.idx[k];
+        atomicAdd(&cnt[id.x & 0x7fffffffu], 1u);
+        atomicAdd(&cnt[id.y & 0x7fffffffu], 1u);
+        atomicAdd(&cnt[id.z & 0x7fffffffu], 1u);
+        atomicAdd(&cnt[id.w & 0x7fffffffu], 1u);
+    }
+}
+
+// Exclusive scan of n counts into out[0..n] (out[n] = total), one block of 1024 threads: each
+// thread scans a contiguous chunk, chunk totals are scanned in LDS.
+__global__ __launch_bounds__(1024) void k_scan_excl(const uint32_t* __restrict__ in, uint32_t* __restrict__ out, uint32_t n)
+{
+    __shared__ uint32_t part[1024];
+    const uint32_t t   = threadIdx.x;
+    const uint32_t per = (n + 1023u) / 1024u;
+    const uint32_t b   = min(n, t * per), e = min(n, b + per);
+    uint32_t       s   = 0;
+    for (uint32_t i = b; i < e; i++) s += in[i];
+    part[t] = s;
+    __syncthreads();
+    for (uint32_t off = 1; off < 1024u; off <<= 1)
+    {
+        uint32_t v = t >= off ? part[t - off] : 0u;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    uint32_t run = part[t] - s;
+    for (uint32_t i = b; i < e; i++)
+    {
+        uint32_t x = in[i];
+        out[i]     = run;
+        run += x;
+    }
+    if (t == 1023u) out[n] = part[1023];
+}
+
+__global__ __launch_bounds__(kB) void k_vc_fill(ContactView c, const uint32_t* __restrict__ ptr, uint32_t* __restrict__ cursor,
+                                                 uint32_t* __restrict__ list, uint32_t* __restrict__ row)
+{
+    uint32_t n = min(*c.count, c.capacity);
+    for (uint32_t k = blockIdx.x * kB + threadIdx.x; k < n; k += gridDim.x * kB)
+    {
+        uint4    id     = c.idx[k];
         uint32_t ids[4] = {id.x & 0x7fffffffu, id.y & 0x7fffffffu, id.z & 0x7fffffffu, id.w & 0x7fffffffu};
-        float    w[4]   = {w4.x, w4.y, w4.z, w4.w};
-        for (int j = 0; j < 4; j++)
+        for (uint32_t j = 0; j < 4; j++)
         {
-            f3 g = (ax.x * w[j]) * nn;
-            atomicAdd(&cgB[3 * (size_t)ids[j]], g.x);
-            atomicAdd(&cgB[3 * (size_t)ids[j] + 1], g.y);
-            atomicAdd(&cgB[3 * (size_t)ids[j] + 2], g.z);
-            float s = (nk.w * w[j]) * w[j];
-            for (int r = 0; r < 3; r++)
-                for (int cc = 0; cc < 3; cc++) atomicAdd(&diag[(3 * r + cc) * (size_t)nv + ids[j]], s * (nv3[r] * nv3[cc]));
+            uint32_t v   = ids[j];
+            uint32_t pos = ptr[v] + atomicAdd(&cursor[v], 1u);
+            list[pos]    = (k << 2) | j;
+            row[pos]     = v;
         }
     }
+}
+
+// fn_perPair_assemble_gradient_hessian, narrow_phase.cpp:2550-2676 (soft bodies), gathered per
+// vertex: cgB_i += k1 w_j n and diag_i += k2 w_j^2 n n^T over the vertex's contact slots.
+__global__ __launch_bounds__(kB) void k_contact_assemble(ContactView c, VertContacts vc, float* __restrict__ cgB,
+                                                          float* __restrict__ diag, uint32_t nv)
+{
+    uint32_t i = blockIdx.x * kB + threadIdx.x;
+    if (i >= nv) return;
+    uint32_t e0 = vc.ptr[i], e1 = vc.ptr[i + 1];
+    if (e0 == e1) return;
+    f3    g = mk(0.0f, 0.0f, 0.0f);
+    float D[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) D[k] = 0.0f;
+    for (uint32_t e = e0; e < e1; e++)
+    {
+        uint32_t ent = vc.list[e], k = ent >> 2, j = ent & 3u;
+        float4   nk = c.nk[k], w4 = c.w[k], ax = c.aux[k];
+        float    w[4]   = {w4.x, w4.y, w4.z, w4.w};
+        float    nv3[3] = {nk.x, nk.y, nk.z};
+        g               = g + (ax.x * w[j]) * mk(nk.x, nk.y, nk.z);
+        float s         = (nk.w * w[j]) * w[j];
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+#pragma unroll
+            for (int cc = 0; cc < 3; cc++) D[3 * r + cc] = D[3 * r + cc] + s * (nv3[r] * nv3[cc]);
+    }
+    st3(cgB, i, ld3(cgB, i) + g);
+#pragma unroll
+    for (int k = 0; k < 9; k++) diag[k * (size_t)nv + i] = diag[k * (size_t)nv + i] + D[k];
+}
+
+// Per-slot coefficient a = k2 w_j n (the contact's id in .w), in list order. With the contact
+// diagonal blocks left out of the SpMV's diagonal, the contact part of (A p)_i
+// (narrow_phase.cpp:2730-2775) is sum over the row's slots of a t_c, t_c = sum_jj w_jj n.p_jj.
+__global__ __launch_bounds__(kB) void k_vc_coeffs(ContactView c, const uint32_t* __restrict__ ptr, uint32_t nv,
+                                                   const uint32_t* __restrict__ list, float4* __restrict__ coef)
+{
+    uint32_t total = ptr[nv];
+    for (uint32_t e = blockIdx.x * kB + threadIdx.x; e < total; e += gridDim.x * kB)
+    {
+        uint32_t ent = list[e], k = ent >> 2, j = ent & 3u;
+        float4   nk = c.nk[k], w4 = c.w[k];
+        float    w[4] = {w4.x, w4.y, w4.z, w4.w};
+        f3       a    = (nk.w * w[j]) * mk(nk.x, nk.y, nk.z);
+        coef[e]       = make_float4(a.x, a.y, a.z, __uint_as_float(k));
+    }
+}
+
+// t_c = sum_j w_j n.p_j for every contact, with p given by pfun.
+template <class PF>
+LCSX_D void contact_t(const ContactView& c, float* __restrict__ tc, PF pfun)
+{
+    uint32_t n = min(*c.count, c.capacity);
+    for (uint32_t k = blockIdx.x * kB + threadIdx.x; k < n; k += gridDim.x * kB)
+    {
+        uint4    id     = c.idx[k];
+        float4   nk     = c.nk[k], w4 = c.w[k];
+        f3       nn     = mk(nk.x, nk.y, nk.z);
+        float    t      = w4.x * dot(nn, pfun(id.x & 0x7fffffffu));
+        t               = t + w4.y * dot(nn, pfun(id.y & 0x7fffffffu));
+        t               = t + w4.z * dot(nn, pfun(id.z & 0x7fffffffu));
+        tc[k]           = t + w4.w * dot(nn, pfun(id.w & 0x7fffffffu));
+    }
+}
+
+// Contact slots of the block's rows [r0, r1): slots are contiguous in row order, so the whole
+// block walks them in strides of kB (coalesced, balanced over threads) and accumulates a t_c per
+// row in LDS. Called by every thread of the block.
+LCSX_D void contact_block(const VertContacts& vc, uint32_t r0, uint32_t r1, float* acc)
+{
+    for (uint32_t k = threadIdx.x; k < 3 * kB; k += kB) acc[k] = 0.0f;
+    __syncthreads();
+    const uint32_t s0 = vc.ptr[r0], s1 = vc.ptr[r1];
+    for (uint32_t e = s0 + threadIdx.x; e < s1; e += kB)
+    {
+        float4   C   = vc.coef[e];
+        uint32_t row = vc.row[e] - r0;
+        float    t   = vc.tc[__float_as_uint(C.w)];
+        atomicAdd(&acc[3 * row], C.x * t);
+        atomicAdd(&acc[3 * row + 1], C.y * t);
+        atomicAdd(&acc[3 * row + 2], C.z * t);
+    }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(kB) void k_contact_t_pcg(PcgArgs a, ContactView c, float* __restrict__ tc)
+{
+    if (a.s->converged) return;
+    const float beta = a.s->beta;
+    contact_t(c, tc, [&](uint32_t v) { return ld3(a.z, v) + beta * ld3(a.p_old, v); });
+}
+
+__global__ __launch_bounds__(kB) void k_contact_t_plain(ContactView c, const float* __restrict__ p, float* __restrict__ tc)
+{
+    contact_t(c, tc, [&](uint32_t v) { return ld3(p, v); });
 }
 
 // ---------------------------------------------------------------- PCG
@@ -407,87 +549,79 @@ __global__ __launch_bounds__(kB) void k_pcg_init(PcgArgs a)
     }
 }
 
-// q = A p_new with p_new = z + beta p_old formed on the fly for every gathered column.
-__global__ __launch_bounds__(kB) void k_spmv_pcg(PcgArgs a, SellView A)
-{
-    if (a.s->converged) return;
-    uint32_t i = blockIdx.x * kB + threadIdx.x;
-    if (i >= a.n) return;
-    float beta = a.s->beta;
-    f3    pi   = ld3(a.z, i) + beta * ld3(a.p_old, i);
-    st3(a.p_new, i, pi);
-    f3       y  = mul(ld_planes(A.diag, a.n, i), pi);
-    uint32_t s  = i >> 6, l = i & 63u;
-    uint32_t k0 = A.sl_off[s], k1 = A.sl_off[s + 1];
-    for (uint32_t k = k0; k < k1; k++)
-    {
-        size_t   b  = (size_t)k * 64 + l;
-        uint32_t c  = A.col[b];
-        f3       pc = ld3(a.z, c) + beta * ld3(a.p_old, c);
-        m3       M;
-#pragma unroll
-        for (int cc = 0; cc < 9; cc++) M.m[cc] = A.val[cc * (size_t)A.nblocks + b];
-        y = y + mul(M, pc);
-    }
-    st3(a.q, i, y);
-}
-
-// y = D p + sum_k A_k p[col_k] (material part only), one thread per row.
-__global__ __launch_bounds__(kB) void k_spmv_plain(uint32_t n, SellView A, const float* __restrict__ p, float* __restrict__ y)
-{
-    uint32_t i = blockIdx.x * kB + threadIdx.x;
-    if (i >= n) return;
-    f3       acc = mul(ld_planes(A.diag, n, i), ld3(p, i));
-    uint32_t s = i >> 6, l = i & 63u;
-    for (uint32_t k = A.sl_off[s]; k < A.sl_off[s + 1]; k++)
-    {
-        size_t b = (size_t)k * 64 + l;
-        m3     M;
-#pragma unroll
-        for (int cc = 0; cc < 9; cc++) M.m[cc] = A.val[cc * (size_t)A.nblocks + b];
-        acc = acc + mul(M, ld3(p, A.col[b]));
-    }
-    st3(y, i, acc);
-}
-
-// Contact part of q = A p (narrow_phase.cpp:2730-2775, per-pair form): for each contact
-// q_j += k2 w_j n (sum_{jj != j} w_jj n.p_jj).
-__global__ __launch_bounds__(kB) void k_contact_spmv(ContactView c, const float* __restrict__ p, float* __restrict__ q,
-                                                      const uint32_t* __restrict__ converged)
-{
-    if (converged && *converged) return;
-    uint32_t n = min(*c.count, c.capacity);
-    for (uint32_t k = blockIdx.x * kB + threadIdx.x; k < n; k += gridDim.x * kB)
-    {
-        uint4    id     = c.idx[k];
-        float4   nk     = c.nk[k], w4 = c.w[k];
-        uint32_t ids[4] = {id.x & 0x7fffffffu, id.y & 0x7fffffffu, id.z & 0x7fffffffu, id.w & 0x7fffffffu};
-        float    w[4]   = {w4.x, w4.y, w4.z, w4.w};
-        f3       nn     = mk(nk.x, nk.y, nk.z);
-        float    t[4];
-        for (int j = 0; j < 4; j++) t[j] = w[j] * dot(nn, ld3(p, ids[j]));
-        for (int j = 0; j < 4; j++)
-        {
-            float s = 0.0f;
-            for (int jj = 0; jj < 4; jj++)
-                if (jj != j) s = s + t[jj];
-            f3 o = ((nk.w * w[j]) * s) * nn;
-            atomicAdd(&q[3 * (size_t)ids[j]], o.x);
-            atomicAdd(&q[3 * (size_t)ids[j] + 1], o.y);
-            atomicAdd(&q[3 * (size_t)ids[j] + 2], o.z);
-        }
-    }
-}
-
-__global__ __launch_bounds__(kB) void k_pcg_alpha(PcgArgs a)
+// q = A p_new with p_new = z + beta p_old formed on the fly for every gathered column: diagonal
+// block, SELL-64 material blocks, then the row's contact slots; p.q is reduced in the same launch
+// and the last block sets alpha = r.z / p.q (precond_cg.cpp:24-31 safe divide).
+__global__ __launch_bounds__(kB) void k_spmv_pcg(PcgArgs a, SellView A, VertContacts vc)
 {
     __shared__ float sh[4];
+    __shared__ float acc[3 * kB];
     if (a.s->converged) return;
-    uint32_t i   = blockIdx.x * kB + threadIdx.x;
-    float    loc = i < a.n ? dot(ld3(a.p_new, i), ld3(a.q, i)) : 0.0f;
-    float    tot;
+    uint32_t    i    = blockIdx.x * kB + threadIdx.x;
+    const float beta = a.s->beta;
+    f3          pi   = mk(0.0f, 0.0f, 0.0f), y = mk(0.0f, 0.0f, 0.0f);
+    if (i < a.n)
+    {
+        pi = ld3(a.z, i) + beta * ld3(a.p_old, i);
+        st3(a.p_new, i, pi);
+        y           = mul(ld_planes(A.diag, a.n, i), pi);
+        uint32_t s  = i >> 6, l = i & 63u;
+        uint32_t k0 = A.sl_off[s], k1 = A.sl_off[s + 1];
+        for (uint32_t k = k0; k < k1; k++)
+        {
+            size_t   b  = (size_t)k * 64 + l;
+            uint32_t cl = A.col[b];
+            f3       pc = ld3(a.z, cl) + beta * ld3(a.p_old, cl);
+            m3       M;
+#pragma unroll
+            for (int cc = 0; cc < 9; cc++) M.m[cc] = A.val[cc * (size_t)A.nblocks + b];
+            y = y + mul(M, pc);
+        }
+    }
+    if (vc.ptr)
+    {
+        uint32_t r0 = blockIdx.x * kB;
+        contact_block(vc, r0, min(r0 + kB, a.n), acc);
+        if (i < a.n) y = y + mk(acc[3 * threadIdx.x], acc[3 * threadIdx.x + 1], acc[3 * threadIdx.x + 2]);
+    }
+    float loc = 0.0f;
+    if (i < a.n)
+    {
+        st3(a.q, i, y);
+        loc = dot(pi, y);
+    }
+    float tot;
     if (last_block_sum(loc, a.partials, &a.s->ticket, &tot, sh) && threadIdx.x == 0)
         a.s->alpha = tot < 1e-10f ? 0.0f : a.s->rz / tot;
+}
+
+// y = A p (diagonal, material blocks and contacts), one thread per row.
+__global__ __launch_bounds__(kB) void k_spmv_plain(uint32_t n, SellView A, VertContacts vc, const float* __restrict__ p,
+                                                    float* __restrict__ y)
+{
+    __shared__ float acc[3 * kB];
+    uint32_t         i   = blockIdx.x * kB + threadIdx.x;
+    f3               out = mk(0.0f, 0.0f, 0.0f);
+    if (i < n)
+    {
+        out        = mul(ld_planes(A.diag, n, i), ld3(p, i));
+        uint32_t s = i >> 6, l = i & 63u;
+        for (uint32_t k = A.sl_off[s]; k < A.sl_off[s + 1]; k++)
+        {
+            size_t b = (size_t)k * 64 + l;
+            m3     M;
+#pragma unroll
+            for (int cc = 0; cc < 9; cc++) M.m[cc] = A.val[cc * (size_t)A.nblocks + b];
+            out = out + mul(M, ld3(p, A.col[b]));
+        }
+    }
+    if (vc.ptr)
+    {
+        uint32_t r0 = blockIdx.x * kB;
+        contact_block(vc, r0, min(r0 + kB, n), acc);
+        if (i < n) out = out + mk(acc[3 * threadIdx.x], acc[3 * threadIdx.x + 1], acc[3 * threadIdx.x + 2]);
+    }
+    if (i < n) st3(y, i, out);
 }
 
 __global__ __launch_bounds__(kB) void k_pcg_update(PcgArgs a)
@@ -684,30 +818,40 @@ void launch_dcd(const DcdArgs& a, int kind, unsigned grid, hipStream_t s)
     if (kind == 0) k_dcd_vf<<<grid, kB, 0, s>>>(a);
     else k_dcd_ee<<<grid, kB, 0, s>>>(a);
 }
-void launch_contact_assemble(const ContactView& c, float* cgB, float* diag, uint32_t nv, unsigned grid, hipStream_t s)
+void launch_contact_adjacency(const ContactView& c, uint32_t nv, unsigned grid, uint32_t* cnt, uint32_t* ptr,
+                              uint32_t* list, uint32_t* row, float4* coef, hipStream_t s)
 {
-    k_contact_assemble<<<grid, kB, 0, s>>>(c, cgB, diag, nv);
+    k_fill_u32<<<grid_for(nv + 1, kB), kB, 0, s>>>(cnt, nv + 1, 0u);
+    k_vc_count<<<grid, kB, 0, s>>>(c, cnt);
+    k_scan_excl<<<1, 1024, 0, s>>>(cnt, ptr, nv);
+    k_fill_u32<<<grid_for(nv + 1, kB), kB, 0, s>>>(cnt, nv + 1, 0u);
+    k_vc_fill<<<grid, kB, 0, s>>>(c, ptr, cnt, list, row);
+    k_vc_coeffs<<<4 * grid, kB, 0, s>>>(c, ptr, nv, list, coef);
+}
+void launch_contact_assemble(const ContactView& c, const VertContacts& vc, float* cgB, float* diag, uint32_t nv,
+                             hipStream_t s)
+{
+    k_contact_assemble<<<grid_for(nv, kB), kB, 0, s>>>(c, vc, cgB, diag, nv);
 }
 void launch_pcg_init(const PcgArgs& a, hipStream_t s) { k_pcg_init<<<grid_for(a.n, kB), kB, 0, s>>>(a); }
-void launch_pcg_spmv(const PcgArgs& a, const SellView& A, hipStream_t s)
+void launch_pcg_contact_t(const PcgArgs& a, const ContactView& c, float* tc, unsigned grid, hipStream_t s)
 {
-    k_spmv_pcg<<<grid_for(a.n, kB), kB, 0, s>>>(a, A);
+    k_contact_t_pcg<<<grid, kB, 0, s>>>(a, c, tc);
 }
-void launch_pcg_contact_spmv(const PcgArgs& a, const ContactView& c, unsigned contact_grid, hipStream_t s)
+void launch_pcg_spmv(const PcgArgs& a, const SellView& A, const VertContacts& vc, hipStream_t s, hipEvent_t start,
+                     hipEvent_t stop)
 {
-    k_contact_spmv<<<contact_grid, kB, 0, s>>>(c, a.p_new, a.q, &a.s->converged);
+    if (start)
+        hipExtLaunchKernelGGL(k_spmv_pcg, dim3(grid_for(a.n, kB)), dim3(kB), 0, s, start, stop, 0, a, A, vc);
+    else
+        k_spmv_pcg<<<grid_for(a.n, kB), kB, 0, s>>>(a, A, vc);
 }
-void launch_pcg_finish(const PcgArgs& a, hipStream_t s)
+void launch_pcg_update(const PcgArgs& a, hipStream_t s) { k_pcg_update<<<grid_for(a.n, kB), kB, 0, s>>>(a); }
+void launch_spmv(const SellView& A, const ContactView& c, const VertContacts& vc, unsigned contact_grid, const float* p,
+                 float* y, uint32_t n, hipStream_t s)
 {
-    unsigned g = grid_for(a.n, kB);
-    k_pcg_alpha<<<g, kB, 0, s>>>(a);
-    k_pcg_update<<<g, kB, 0, s>>>(a);
-}
-void launch_spmv(const SellView& A, const ContactView* c, unsigned contact_grid, const float* p, float* y, uint32_t n,
-                 hipStream_t s)
-{
-    k_spmv_plain<<<grid_for(n, kB), kB, 0, s>>>(n, A, p, y);
-    if (c) k_contact_spmv<<<contact_grid, kB, 0, s>>>(*c, p, y, nullptr);
+    if (vc.ptr) k_contact_t_plain<<<contact_grid, kB, 0, s>>>(c, p, vc.tc);
+    k_spmv_plain<<<grid_for(n, kB), kB, 0, s>>>(n, A, vc, p, y);
 }
 void launch_apply_dx(uint32_t nv, const float* x_iter, const float* dx, float* x, const uint32_t* alpha_bits,
                      float alpha_const, hipStream_t s)

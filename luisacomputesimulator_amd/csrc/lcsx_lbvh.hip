@@ -271,8 +271,9 @@ __global__ __launch_bounds__(kBlock) void k_karras(uint32_t n, const uint64_t* _
     object_idx[nid] = kInvalid;
 }
 
-// update_*_leave_aabb (lbvh.cpp:636-683) + refit (lbvh.cpp:702-754): the second child to arrive
-// at an inner node (atomic ticket) computes its box; fences publish boxes across XCD L2s.
+// update_*_leave_aabb (lbvh.cpp:636-683) + refit (lbvh.cpp:702-754) in one pass, for trees taller
+// than the level-ordered refit handles: the second child to arrive at an inner node (atomic
+// ticket) computes its box; fences publish boxes across XCD L2s.
 __global__ __launch_bounds__(kBlock) void k_refit(uint32_t n, int arity, const uint32_t* __restrict__ prims,
                                                    const uint32_t* __restrict__ sorted_to_orig,
                                                    const float* __restrict__ x0, const float* __restrict__ x1,
@@ -322,14 +323,28 @@ __global__ __launch_bounds__(kBlock) void k_refit(uint32_t n, int arity, const u
         U.lo     = make_float4(smin(A.lo.x, B.lo.x), smin(A.lo.y, B.lo.y), smin(A.lo.z, B.lo.z), 0.0f);
         U.hi     = make_float4(smax(A.hi.x, B.hi.x), smax(A.hi.y, B.hi.y), smax(A.hi.z, B.hi.z), 0.0f);
         nodes[p] = U;
-        // child codes: leaves keep (element, element) in children[]
-        uint32_t ca = c.x >= ni ? (0x80000000u | children[c.x].x) : c.x;
-        uint32_t cb = c.y >= ni ? (0x80000000u | children[c.y].x) : c.y;
-        float4*  P  = packed + 4 * (size_t)p;
+        // child codes (leaves keep (element, element) in children[]) and the largest element id
+        // under each child (an inner child's record was published before its ticket)
+        uint32_t ca, cb, ma, mb;
+        if (c.x >= ni) ca = 0x80000000u | (ma = children[c.x].x);
+        else
+        {
+            ca = c.x;
+            const float4* C = packed + 4 * (size_t)c.x;
+            ma              = max(__float_as_uint(C[1].w), __float_as_uint(C[3].w));
+        }
+        if (c.y >= ni) cb = 0x80000000u | (mb = children[c.y].x);
+        else
+        {
+            cb = c.y;
+            const float4* C = packed + 4 * (size_t)c.y;
+            mb              = max(__float_as_uint(C[1].w), __float_as_uint(C[3].w));
+        }
+        float4* P = packed + 4 * (size_t)p;
         P[0] = make_float4(A.lo.x, A.lo.y, A.lo.z, __uint_as_float(ca));
-        P[1] = make_float4(A.hi.x, A.hi.y, A.hi.z, 0.0f);
+        P[1] = make_float4(A.hi.x, A.hi.y, A.hi.z, __uint_as_float(ma));
         P[2] = make_float4(B.lo.x, B.lo.y, B.lo.z, __uint_as_float(cb));
-        P[3] = make_float4(B.hi.x, B.hi.y, B.hi.z, 0.0f);
+        P[3] = make_float4(B.hi.x, B.hi.y, B.hi.z, __uint_as_float(mb));
         cur  = p;
     }
 }
@@ -340,98 +355,288 @@ __global__ void k_fill_u32(uint32_t* a, uint32_t n, uint32_t v)
     if (i < n) a[i] = v;
 }
 
+// ---------------------------------------------------------------- level-ordered refit
+// Cross-workgroup hand-off inside one kernel needs device-scope fences, which on a multi-XCD
+// part write back and invalidate L2. The refit therefore runs one launch per node height
+// (leaves 0, inner 1 + max of its children): nodes of one height are independent, and kernel
+// boundaries order the levels. Heights and the height-sorted node order are computed once per
+// construction (with the ticket protocol); the top levels run in one workgroup.
+constexpr uint32_t kMaxLevels = 256;
+
+__global__ __launch_bounds__(kBlock) void k_height(uint32_t n, const uint32_t* __restrict__ parent,
+                                                    const uint2* __restrict__ children, uint32_t* height,
+                                                    uint32_t* counters)
+{
+    uint32_t l = blockIdx.x * kBlock + threadIdx.x;
+    if (l >= n) return;
+    uint32_t ni  = n - 1;
+    uint32_t cur = ni + l;
+    for (uint32_t depth = 0; depth < n; depth++)
+    {
+        uint32_t p = parent[cur];
+        if (p == kInvalid || p >= ni) break;
+        __threadfence();
+        if (atomicAdd(&counters[p], 1u) == 0u) break;
+        __threadfence();
+        uint2    c  = children[p];
+        uint32_t ha = c.x >= ni ? 0u : height[c.x];
+        uint32_t hb = c.y >= ni ? 0u : height[c.y];
+        height[p]   = 1u + max(ha, hb);
+        cur         = p;
+    }
+}
+
+// hist[h] for h < kMaxLevels; taller nodes are counted in hist[kMaxLevels]
+__global__ __launch_bounds__(kBlock) void k_level_hist(uint32_t ni, const uint32_t* __restrict__ height, uint32_t* hist)
+{
+    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < ni) atomicAdd(&hist[min(height[i], kMaxLevels)], 1u);
+}
+
+__global__ __launch_bounds__(kBlock) void k_level_scatter(uint32_t ni, const uint32_t* __restrict__ height,
+                                                           const uint32_t* __restrict__ level_ptr, uint32_t* cursor,
+                                                           uint32_t* __restrict__ order)
+{
+    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= ni) return;
+    uint32_t h = height[i];
+    order[level_ptr[h] + atomicAdd(&cursor[h], 1u)] = i;
+}
+
+// update_*_leave_aabb, lbvh.cpp:636-683: leaf box = AABB(x0 and x1 of the element) +- pad[v0]
+LCSX_D NodeBox leaf_box(uint32_t l, int arity, const uint32_t* __restrict__ prims, const uint32_t* __restrict__ sorted_to_orig,
+                        const float* __restrict__ x0, const float* __restrict__ x1, const float* __restrict__ pad)
+{
+    uint32_t o = sorted_to_orig[l];
+    f3       lo, hi;
+    float    th;
+    if (arity == 3)
+    {
+        uint32_t a = prims[3 * o], b = prims[3 * o + 1], c = prims[3 * o + 2];
+        f3       s0 = ld3(x0, a), s1 = ld3(x0, b), s2 = ld3(x0, c);
+        f3       e0 = ld3(x1, a), e1 = ld3(x1, b), e2 = ld3(x1, c);
+        lo = vmin(vmin(s0, vmin(s1, s2)), vmin(e0, vmin(e1, e2)));
+        hi = vmax(vmax(s0, vmax(s1, s2)), vmax(e0, vmax(e1, e2)));
+        th = pad[a];
+    }
+    else
+    {
+        uint32_t a = prims[2 * o], b = prims[2 * o + 1];
+        f3       s0 = ld3(x0, a), s1 = ld3(x0, b), e0 = ld3(x1, a), e1 = ld3(x1, b);
+        lo = vmin(vmin(s0, s1), vmin(e0, e1));
+        hi = vmax(vmax(s0, s1), vmax(e0, e1));
+        th = pad[a];
+    }
+    lo = lo - mk(th, th, th);
+    hi = hi + mk(th, th, th);
+    return NodeBox{make_float4(lo.x, lo.y, lo.z, 0.0f), make_float4(hi.x, hi.y, hi.z, 0.0f)};
+}
+
+// fn_refit_tree_aabb, lbvh.cpp:702-754, for inner node p whose children are final: its box, and
+// its packed record (child boxes, child codes, largest element id under each child).
+LCSX_D void inner_box(uint32_t p, uint32_t ni, const uint2* __restrict__ children, NodeBox* nodes, float4* packed)
+{
+    uint2   c = children[p];
+    NodeBox A = nodes[c.x], B = nodes[c.y];
+    NodeBox U;
+    U.lo     = make_float4(smin(A.lo.x, B.lo.x), smin(A.lo.y, B.lo.y), smin(A.lo.z, B.lo.z), 0.0f);
+    U.hi     = make_float4(smax(A.hi.x, B.hi.x), smax(A.hi.y, B.hi.y), smax(A.hi.z, B.hi.z), 0.0f);
+    nodes[p] = U;
+    uint32_t ca, cb, ma, mb;
+    if (c.x >= ni) ca = 0x80000000u | (ma = children[c.x].x);
+    else
+    {
+        ca              = c.x;
+        const float4* C = packed + 4 * (size_t)c.x;
+        ma              = max(__float_as_uint(C[1].w), __float_as_uint(C[3].w));
+    }
+    if (c.y >= ni) cb = 0x80000000u | (mb = children[c.y].x);
+    else
+    {
+        cb              = c.y;
+        const float4* C = packed + 4 * (size_t)c.y;
+        mb              = max(__float_as_uint(C[1].w), __float_as_uint(C[3].w));
+    }
+    float4* P = packed + 4 * (size_t)p;
+    P[0]      = make_float4(A.lo.x, A.lo.y, A.lo.z, __uint_as_float(ca));
+    P[1]      = make_float4(A.hi.x, A.hi.y, A.hi.z, __uint_as_float(ma));
+    P[2]      = make_float4(B.lo.x, B.lo.y, B.lo.z, __uint_as_float(cb));
+    P[3]      = make_float4(B.hi.x, B.hi.y, B.hi.z, __uint_as_float(mb));
+}
+
+__global__ __launch_bounds__(kBlock) void k_refit_leaves(uint32_t n, int arity, const uint32_t* __restrict__ prims,
+                                                          const uint32_t* __restrict__ sorted_to_orig,
+                                                          const float* __restrict__ x0, const float* __restrict__ x1,
+                                                          const float* __restrict__ pad, NodeBox* __restrict__ nodes)
+{
+    uint32_t l = blockIdx.x * kBlock + threadIdx.x;
+    if (l >= n) return;
+    nodes[n - 1 + l] = leaf_box(l, arity, prims, sorted_to_orig, x0, x1, pad);
+}
+
+__global__ __launch_bounds__(kBlock) void k_refit_level(const uint32_t* __restrict__ order, uint32_t b, uint32_t e,
+                                                         uint32_t ni, const uint2* __restrict__ children, NodeBox* nodes,
+                                                         float4* packed)
+{
+    uint32_t k = b + blockIdx.x * kBlock + threadIdx.x;
+    if (k < e) inner_box(order[k], ni, children, nodes, packed);
+}
+
+// Levels h0 .. h1 (inclusive) in one workgroup, a barrier between levels.
+__global__ __launch_bounds__(1024) void k_refit_tail(const uint32_t* __restrict__ order, const uint32_t* __restrict__ level_ptr,
+                                                      uint32_t h0, uint32_t h1, uint32_t ni,
+                                                      const uint2* __restrict__ children, NodeBox* nodes, float4* packed)
+{
+    for (uint32_t h = h0; h <= h1; h++)
+    {
+        for (uint32_t k = level_ptr[h] + threadIdx.x; k < level_ptr[h + 1]; k += 1024u)
+            inner_box(order[k], ni, children, nodes, packed);
+        __threadfence_block();
+        __syncthreads();
+    }
+}
+
 // ---------------------------------------------------------------- traversal
 constexpr int kQueryBlock = 128;
 constexpr int kStack      = 32;
 
-// query_template2, lbvh.cpp:293-360: depth-first with a 32-entry stack staged in LDS; a full
-// stack ends the query (the reference's silent truncation), reported through *overflow.
+// lbvh.cpp:293-360 / aabb.h:96-100: strict overlap of a node box (lo4, hi4) with the query box
 LCSX_D bool overlap4(float4 lo4, float4 hi4, f3 lo, f3 hi)
 {
     return (lo4.x < hi.x && lo4.y < hi.y && lo4.z < hi.z) && (hi4.x > lo.x && hi4.y > lo.y && hi4.z > lo.z);
 }
 
+constexpr uint32_t kWaveBuf = 256;  // pairs staged per wavefront before one global reservation
+
+// Copies a wavefront's staged pairs to the global list: one atomic reserves the range, lanes
+// copy in strides of 64. Called by all 64 lanes.
+LCSX_D void flush_pairs(const uint2* buf, uint32_t nbuf, uint32_t* count, uint32_t capacity, uint32_t* pairs)
+{
+    __builtin_amdgcn_wave_barrier();
+    uint32_t base = 0;
+    if (lane_id() == 0) base = atomicAdd(count, nbuf);
+    base = __shfl(base, 0);
+    for (uint32_t k = lane_id(); k < nbuf; k += 64u)
+        if (base + k < capacity)
+        {
+            uint2 v                         = buf[k];
+            pairs[2 * (size_t)(base + k)]     = v.x;
+            pairs[2 * (size_t)(base + k) + 1] = v.y;
+        }
+    __builtin_amdgcn_wave_barrier();
+}
+
 // query_template2, lbvh.cpp:293-360: depth-first with a 32-entry stack staged in LDS; a full
 // stack ends the query (the reference's silent truncation), reported through *overflow. Each
 // step loads one packed inner node (both child boxes and codes); children are tested and inner
-// hits pushed in the reference's order, so pairs and overflow behave as the reference's.
+// hits pushed in the reference's order. The wavefront loops until all its lanes are done, so
+// hits are staged in LDS and written with one atomic per kWaveBuf/2 or more pairs.
 __global__ __launch_bounds__(kQueryBlock) void k_query(QueryArgs a, const float4* __restrict__ packed, uint32_t n_inner)
 {
     __shared__ uint32_t stack[kStack][kQueryBlock];
-    uint32_t            q = blockIdx.x * kQueryBlock + threadIdx.x;
-    if (q >= a.n_queries) return;
-    f3   lo, hi;
-    bool filt = false;
-    if (a.mode == QueryVerts)
+    __shared__ uint2    buf[kQueryBlock / 64][kWaveBuf];
+    const uint32_t      tid = threadIdx.x, wv = tid >> 6;
+    const uint32_t      q   = blockIdx.x * kQueryBlock + tid;
+    bool                active = q < a.n_queries;
+    f3                  lo = mk(0.0f, 0.0f, 0.0f), hi = mk(0.0f, 0.0f, 0.0f);
+    bool                filt = false;
+    if (active)
     {
-        f3    p0 = ld3(a.x0, q), p1 = ld3(a.x1, q);
-        float t  = a.offset[q] + a.dhat[q];
-        lo       = vmin(p0, p1) - mk(t, t, t);
-        hi       = vmax(p0, p1) + mk(t, t, t);
+        if (a.mode == QueryVerts)
+        {
+            f3    p0 = ld3(a.x0, q), p1 = ld3(a.x1, q);
+            float t  = a.offset[q] + a.dhat[q];
+            lo       = vmin(p0, p1) - mk(t, t, t);
+            hi       = vmax(p0, p1) + mk(t, t, t);
+        }
+        else if (a.mode == QueryEdges)
+        {
+            uint32_t e0 = a.edges[2 * q], e1 = a.edges[2 * q + 1];
+            f3       s0 = ld3(a.x0, e0), s1 = ld3(a.x0, e1), t0 = ld3(a.x1, e0), t1 = ld3(a.x1, e1);
+            float    t  = a.offset[e0] + a.dhat[e0];
+            lo          = vmin(vmin(s0, s1), vmin(t0, t1)) - mk(t, t, t);
+            hi          = vmax(vmax(s0, s1), vmax(t0, t1)) + mk(t, t, t);
+            filt        = true;
+        }
+        else
+        {
+            lo   = ld3(a.boxes, 2 * q);
+            hi   = ld3(a.boxes, 2 * q + 1);
+            filt = a.edge_filter != 0;
+        }
     }
-    else if (a.mode == QueryEdges)
-    {
-        uint32_t e0 = a.edges[2 * q], e1 = a.edges[2 * q + 1];
-        f3       s0 = ld3(a.x0, e0), s1 = ld3(a.x0, e1), t0 = ld3(a.x1, e0), t1 = ld3(a.x1, e1);
-        float    t  = a.offset[e0] + a.dhat[e0];
-        lo          = vmin(vmin(s0, s1), vmin(t0, t1)) - mk(t, t, t);
-        hi          = vmax(vmax(s0, s1), vmax(t0, t1)) + mk(t, t, t);
-        filt        = true;
-    }
-    else
-    {
-        lo   = ld3(a.boxes, 2 * q);
-        hi   = ld3(a.boxes, 2 * q + 1);
-        filt = a.edge_filter != 0;
-    }
-    const uint32_t tid = threadIdx.x;
-    int            sp  = 0;
-    stack[sp++][tid]   = 0u;
+    int sp = 0;
+    if (active) stack[sp++][tid] = 0u;
     // each inner node is visited at most once per query in a well-formed tree; the cap (as the
     // reference's 100000-iteration guard) only bounds a corrupted one
     uint32_t visits = 0;
-    while (sp > 0)
+    uint32_t nbuf   = 0;  // staged pairs of this wavefront (uniform across its lanes)
+    while (__ballot(active && sp > 0) != 0ull)
     {
-        if (++visits > n_inner + 1u)
+        bool     e0 = false, e1 = false;
+        uint32_t o0 = 0, o1 = 0;
+        if (active && sp > 0)
         {
-            if (a.overflow) atomicOr(a.overflow, 2u);
-            break;
-        }
-        uint32_t      nd = stack[--sp][tid];
-        const float4* P  = packed + 4 * (size_t)nd;
-        float4        r0 = P[0], r1 = P[1], r2 = P[2], r3 = P[3];
-        bool          brk = false;
-#pragma unroll
-        for (int ii = 0; ii < 2; ii++)
-        {
-            float4   blo  = ii == 0 ? r0 : r2;
-            float4   bhi  = ii == 0 ? r1 : r3;
-            uint32_t code = __float_as_uint(blo.w);
-            bool     hit  = overlap4(blo, bhi, lo, hi);
-            bool     leaf = (code & 0x80000000u) != 0u;
-            uint32_t obj  = code & 0x7fffffffu;
-            // a full stack at the first child ends the query before the second is tested
-            hit           = hit && !brk;
-            bool     emit = hit && leaf && (!filt || q < obj);
-            uint32_t slot = wave_append(a.count, emit);
-            if (emit && slot < a.capacity)
+            if (++visits > n_inner + 1u)
             {
-                a.pairs[2 * (size_t)slot]     = q;
-                a.pairs[2 * (size_t)slot + 1] = obj;
+                if (a.overflow) atomicOr(a.overflow, 2u);
+                active = false;
             }
-            if (hit && !leaf)
+            else
             {
-                if (sp < kStack)
-                    stack[sp++][tid] = code;
-                else
+                uint32_t      nd = stack[--sp][tid];
+                const float4* P  = packed + 4 * (size_t)nd;
+                float4        r0 = P[0], r1 = P[1], r2 = P[2], r3 = P[3];
+                uint32_t      c0 = __float_as_uint(r0.w), c1 = __float_as_uint(r2.w);
+                // with the edge filter (keep q < element) a subtree whose largest element id is
+                // <= q holds no pair: it is not descended (leaf: the largest id is the element)
+                bool h0 = overlap4(r0, r1, lo, hi) && (!filt || q < __float_as_uint(r1.w));
+                bool h1 = overlap4(r2, r3, lo, hi) && (!filt || q < __float_as_uint(r3.w));
+                bool l0 = (c0 & 0x80000000u) != 0u, l1 = (c1 & 0x80000000u) != 0u;
+                bool brk = false;
+                if (h0 && !l0)
                 {
-                    brk = true;
-                    if (a.overflow) atomicOr(a.overflow, 1u);
+                    if (sp < kStack) stack[sp++][tid] = c0;
+                    else
+                    {
+                        brk = true;  // a full stack at the first child ends the query before the second
+                        if (a.overflow) atomicOr(a.overflow, 1u);
+                    }
                 }
+                h1 = h1 && !brk;
+                if (h1 && !l1)
+                {
+                    if (sp < kStack) stack[sp++][tid] = c1;
+                    else
+                    {
+                        brk = true;
+                        if (a.overflow) atomicOr(a.overflow, 1u);
+                    }
+                }
+                e0 = h0 && l0;
+                e1 = h1 && l1;
+                o0 = c0 & 0x7fffffffu;
+                o1 = c1 & 0x7fffffffu;
+                if (brk) active = false;
             }
         }
-        if (brk) break;
+        uint64_t b0 = __ballot(e0), b1 = __ballot(e1);
+        uint32_t cnt = (uint32_t)(__popcll(b0) + __popcll(b1));
+        if (cnt)
+        {
+            uint64_t lt  = lanemask_lt();
+            uint32_t off = nbuf + (uint32_t)(__popcll(b0 & lt) + __popcll(b1 & lt));
+            if (e0) buf[wv][off++] = make_uint2(q, o0);
+            if (e1) buf[wv][off] = make_uint2(q, o1);
+            nbuf += cnt;
+            if (nbuf > kWaveBuf - 128u)
+            {
+                flush_pairs(buf[wv], nbuf, a.count, a.capacity, a.pairs);
+                nbuf = 0;
+            }
+        }
     }
+    if (nbuf) flush_pairs(buf[wv], nbuf, a.count, a.capacity, a.pairs);
 }
 }  // namespace
 
@@ -452,6 +657,11 @@ void Lbvh::init(int arity_, const uint32_t* device_prims, uint32_t n_leaves)
     nodes.alloc(n_nodes());
     packed.alloc(4 * (size_t)(n_inner() ? n_inner() : 1));
     counters.alloc(n_inner() ? n_inner() : 1);
+    height.alloc(n_inner() ? n_inner() : 1);
+    order.alloc(n_inner() ? n_inner() : 1);
+    level_hist.alloc(kMaxLevels + 1);
+    level_ptr.alloc(kMaxLevels + 2);
+    level_cursor.alloc(kMaxLevels + 1);
     gbox.alloc(6);
     sorted_to_orig = vals[0].p;
 }
@@ -478,11 +688,53 @@ void Lbvh::construct(const float* x, uint32_t n_verts_for_reduce, hipStream_t s)
                                                     object_idx.p, parent.p);
     if (n > 1) k_karras<<<grid_for(n - 1, kBlock), kBlock, 0, s>>>(n, morton_sorted.p, parent.p, children.p, object_idx.p);
     LCSX_HIP(hipGetLastError());
+    use_levels = false;
+    if (n < 2) return;
+    // node heights and the height-sorted order of inner nodes (one host read of the histogram)
+    const uint32_t ni = n - 1;
+    k_fill_u32<<<grid_for(ni, kBlock), kBlock, 0, s>>>(counters.p, ni, 0u);
+    k_height<<<grid_for(n, kBlock), kBlock, 0, s>>>(n, parent.p, children.p, height.p, counters.p);
+    k_fill_u32<<<grid_for(kMaxLevels + 1, kBlock), kBlock, 0, s>>>(level_hist.p, kMaxLevels + 1, 0u);
+    k_level_hist<<<grid_for(ni, kBlock), kBlock, 0, s>>>(ni, height.p, level_hist.p);
+    std::vector<uint32_t> hist(kMaxLevels + 1);
+    LCSX_HIP(hipMemcpyAsync(hist.data(), level_hist.p, hist.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    LCSX_HIP(hipStreamSynchronize(s));
+    if (hist[kMaxLevels] != 0) return;  // taller than kMaxLevels: keep the ticket refit
+    level_ptr_h.assign(kMaxLevels + 2, 0u);
+    levels = 0;
+    for (uint32_t h = 0; h <= kMaxLevels; h++)
+    {
+        level_ptr_h[h + 1] = level_ptr_h[h] + hist[h];
+        if (hist[h]) levels = h;
+    }
+    // the top levels, together at most kTailNodes nodes, are refit by one workgroup
+    constexpr uint32_t kTailNodes = 8192;
+    tail_level = levels;
+    while (tail_level > 1 && level_ptr_h[levels + 1] - level_ptr_h[tail_level - 1] <= kTailNodes) tail_level--;
+    LCSX_HIP(hipMemcpyAsync(level_ptr.p, level_ptr_h.data(), level_ptr_h.size() * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    k_fill_u32<<<grid_for(kMaxLevels + 1, kBlock), kBlock, 0, s>>>(level_cursor.p, kMaxLevels + 1, 0u);
+    k_level_scatter<<<grid_for(ni, kBlock), kBlock, 0, s>>>(ni, height.p, level_ptr.p, level_cursor.p, order.p);
+    LCSX_HIP(hipGetLastError());
+    use_levels = true;
 }
 
 void Lbvh::refit(const float* x0, const float* x1, const float* pad, hipStream_t s)
 {
     if (n == 0) return;
+    if (use_levels)
+    {
+        const uint32_t ni = n - 1;
+        k_refit_leaves<<<grid_for(n, kBlock), kBlock, 0, s>>>(n, arity, prims, sorted_to_orig, x0, x1, pad, nodes.p);
+        for (uint32_t h = 1; h < tail_level; h++)
+        {
+            uint32_t b = level_ptr_h[h], e = level_ptr_h[h + 1];
+            if (e > b)
+                k_refit_level<<<grid_for(e - b, kBlock), kBlock, 0, s>>>(order.p, b, e, ni, children.p, nodes.p, packed.p);
+        }
+        k_refit_tail<<<1, 1024, 0, s>>>(order.p, level_ptr.p, tail_level, levels, ni, children.p, nodes.p, packed.p);
+        LCSX_HIP(hipGetLastError());
+        return;
+    }
     k_fill_u32<<<grid_for((uint32_t)counters.n, kBlock), kBlock, 0, s>>>(counters.p, (uint32_t)counters.n, 0u);
     k_refit<<<grid_for(n, kBlock), kBlock, 0, s>>>(n, arity, prims, sorted_to_orig, x0, x1, pad, parent.p, children.p,
                                                    nodes.p, packed.p, counters.p);

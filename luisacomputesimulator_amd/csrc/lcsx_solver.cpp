@@ -118,6 +118,11 @@ void Solver::alloc_collision(uint32_t vf, uint32_t ee, uint32_t nc)
     c_nk.alloc(cap_c);
     c_w.alloc(cap_c);
     c_aux.alloc(cap_c);
+    vc_list.alloc(4 * (size_t)cap_c);
+    vc_row.alloc(4 * (size_t)cap_c);
+    vc_coef.alloc(4 * (size_t)cap_c);
+    vc_tc.alloc(cap_c);
+    vc_ready = false;
 }
 
 void Solver::init()
@@ -135,6 +140,7 @@ void Solver::init()
     const uint32_t nv = S.nv;
     for (auto* b : {&x, &v, &x_step, &x_iter, &x_tilde, &dx, &cgB, &r, &z, &p[0], &p[1], &q, &spmv_tmp}) b->alloc(3 * (size_t)nv);
     diag.alloc(9 * (size_t)nv);
+    diag_mat.alloc(9 * (size_t)nv);
     minv.alloc(9 * (size_t)nv);
     sell_val.alloc(9 * (size_t)std::max(S.nblocks, 1u));
     partials.alloc(grid_for(nv, 256) + 1);
@@ -142,6 +148,8 @@ void Solver::init()
     tree_face.init(3, faces.p, S.nf);
     tree_edge.init(2, edges.p, S.ne);
     alloc_collision(24 * nv, 24 * S.ne, 12 * nv);
+    vc_cnt.alloc(nv + 1);
+    vc_ptr.alloc(nv + 1);
     status.alloc(1);
     pcg_state.alloc(1);
     LCSX_HIP(hipMemsetAsync(pcg_state.p, 0, sizeof(PcgState), stream));
@@ -204,7 +212,14 @@ ContactView Solver::contact_view() const
 {
     return ContactView{c_idx.p, c_nk.p, c_w.p, c_aux.p, &status.p->n_contacts, cap_c};
 }
-SellView Solver::sell_view() const { return SellView{sl_off.p, sell_col.p, sell_val.p, diag.p, S.nblocks}; }
+VertContacts Solver::vert_contacts() const
+{
+    // no lists before the first contact assembly, or without self collision
+    return (prm.use_self_collision && vc_ready) ? VertContacts{vc_ptr.p, vc_list.p, vc_row.p, vc_coef.p, vc_tc.p}
+                                                : VertContacts{nullptr, nullptr, nullptr, nullptr, nullptr};
+}
+// The SpMV's diagonal is the material one: contact diagonal blocks enter through the contact slots.
+SellView Solver::sell_view() const { return SellView{sl_off.p, sell_col.p, sell_val.p, diag_mat.p, S.nblocks}; }
 
 // ------------------------------------------------------------------ stages
 void Solver::frame_begin()
@@ -273,6 +288,7 @@ void Solver::material()
     a.hinge_stage = hinge_stage.p;
     a.cgB      = cgB.p;
     a.diag     = diag.p;
+    a.diag_mat = diag_mat.p;
     launch_vert_assemble(a, stream);
     launch_offdiag(S.nblocks, blk_ptr.p, blk_contrib.p, face_stage.p, hinge_stage.p, sell_val.p, stream);
     toc("material");
@@ -322,6 +338,7 @@ void Solver::broad_phase(bool ccd)
 
 void Solver::dcd()
 {
+    vc_ready = false;  // new contact set: the per-vertex lists are rebuilt by contact_assemble
     tic();
     ContactBuf out{c_idx.p, c_nk.p, c_w.p, c_aux.p, &status.p->n_contacts, cap_c};
     DcdArgs    a{};
@@ -360,7 +377,10 @@ void Solver::contact_assemble()
 {
     if (!prm.use_self_collision) return;
     tic();
-    launch_contact_assemble(contact_view(), cgB.p, diag.p, S.nv, pair_grid(cap_c), stream);
+    launch_contact_adjacency(contact_view(), S.nv, pair_grid(cap_c), vc_cnt.p, vc_ptr.p, vc_list.p, vc_row.p, vc_coef.p,
+                             stream);
+    vc_ready = true;
+    launch_contact_assemble(contact_view(), vert_contacts(), cgB.p, diag.p, S.nv, stream);
     toc("contact_assemble");
 }
 
@@ -385,8 +405,9 @@ void Solver::pcg()
     a.check_every = (uint32_t)std::max(1, prm.pcg_check_every);
     launch_pcg_init(a, stream);
     SellView    A  = sell_view();
-    ContactView cv = contact_view();
-    PcgState    hs{};
+    ContactView  cv = contact_view();
+    VertContacts vc = vert_contacts();
+    PcgState     hs{};
     int         issued = 0;
     int         par    = 0;
     while (issued < prm.pcg_iter_count)
@@ -402,17 +423,13 @@ void Solver::pcg()
             if (timing_spmv)
             {
                 e0 = ev_get();
-                LCSX_HIP(hipEventRecord(e0, stream));
-            }
-            launch_pcg_spmv(b, A, stream);
-            if (timing_spmv)
-            {
                 e1 = ev_get();
-                LCSX_HIP(hipEventRecord(e1, stream));
                 spmv_events.push_back({e0, e1});
             }
-            if (prm.use_self_collision) launch_pcg_contact_spmv(b, cv, pair_grid(cap_c), stream);
-            launch_pcg_finish(b, stream);
+            if (vc.ptr) launch_pcg_contact_t(b, cv, vc.tc, pair_grid(cap_c), stream);
+            launch_pcg_spmv(b, A, vc, stream, e0, e1);
+            LCSX_HIP(hipGetLastError());
+            launch_pcg_update(b, stream);
             par = 1 - par;
         }
         issued += chunk;
@@ -423,6 +440,21 @@ void Solver::pcg()
     }
     toc("pcg");
     // only launches that did work (the first `iters`; later ones exit on the converged flag)
+    if (!spmv_events.empty())
+    {
+        // contact slots the timed launches processed ("spmv_slots": slots per launch)
+        uint32_t nc = 0;
+        if (vc.ptr)
+        {
+            LCSX_HIP(hipMemcpyAsync(&nc, &status.p->n_contacts, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+            LCSX_HIP(hipStreamSynchronize(stream));
+            nc = std::min(nc, cap_c);
+        }
+        uint32_t counted = std::min<uint32_t>(hs.iters, (uint32_t)spmv_events.size());
+        auto&    t       = times["spmv_slots"];
+        t.first += 4.0 * nc * counted;
+        t.second += (int)counted;
+    }
     for (size_t k = 0; k < spmv_events.size(); k++)
     {
         float ms = 0.0f;
@@ -579,8 +611,7 @@ void Solver::step()
 
 void Solver::spmv_dx()
 {
-    ContactView cv = contact_view();
-    launch_spmv(sell_view(), prm.use_self_collision ? &cv : nullptr, pair_grid(cap_c), dx.p, spmv_tmp.p, S.nv, stream);
+    launch_spmv(sell_view(), contact_view(), vert_contacts(), pair_grid(cap_c), dx.p, spmv_tmp.p, S.nv, stream);
     LCSX_HIP(hipMemcpyAsync(dx.p, spmv_tmp.p, dx.n * sizeof(float), hipMemcpyDeviceToDevice, stream));
 }
 

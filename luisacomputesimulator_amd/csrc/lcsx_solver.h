@@ -26,7 +26,7 @@ struct Solver
 
     // state (3 floats per vertex)
     DBuf<float> x, v, x_step, x_iter, x_tilde, dx, cgB, r, z, p[2], q, spmv_tmp;
-    DBuf<float> diag, minv, sell_val;  // 9 planes
+    DBuf<float> diag, diag_mat, minv, sell_val;  // 9 planes (diag: with contact blocks; diag_mat: material only)
     DBuf<float> partials;
     // constants
     DBuf<float>    rest_x, offset, dhat, vert_mass, vert_area, face_area, edge_area, targets;
@@ -40,6 +40,10 @@ struct Solver
     uint32_t       cap_vf = 0, cap_ee = 0, cap_c = 0;
     DBuf<uint4>    c_idx;
     DBuf<float4>   c_nk, c_w, c_aux;
+    DBuf<uint32_t> vc_cnt, vc_ptr, vc_list, vc_row;  // per-vertex contact slots (rebuilt with each contact assembly)
+    DBuf<float4>   vc_coef;                          // per slot
+    DBuf<float>    vc_tc;                    // per contact
+    bool           vc_ready = false;
     DBuf<Status>   status;
     DBuf<PcgState> pcg_state;
     Status         host_status{};
@@ -79,6 +83,7 @@ struct Solver
 
     ContactView contact_view() const;
     SellView    sell_view() const;
+    VertContacts vert_contacts() const;
     unsigned    pair_grid(uint32_t cap) const;
 
     // timing helpers
