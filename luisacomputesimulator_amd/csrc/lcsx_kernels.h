@@ -17,10 +17,12 @@ struct Status
     uint32_t n_vf_ccd, n_ee_ccd, pad0, pad1;
 };
 
+// PCG scalars shared between launches. r.z of iteration k is stored in rz[k & 1] by the SpMV of
+// iteration k; converged / iters / nan are set when an iteration's convergence test stops.
 struct PcgState
 {
-    float    rz, rz0, alpha, beta;
-    uint32_t converged, iters, ticket, nan;
+    float    rz[2], rz0;
+    uint32_t converged, iters, nan, pad[2];
 };
 
 struct SellView
@@ -47,9 +49,10 @@ struct VertContacts
 {
     const uint32_t* ptr;
     const uint32_t* list;
-    const uint32_t* row;   // per slot: its vertex
-    const float4*   coef;  // per slot: (k2 w_j n, contact id bits)
-    float*          tc;    // per contact: sum_j w_j n.p_j of the vector being multiplied
+    const uint32_t* row;     // per slot: its vertex
+    const float4*   coef;    // per slot: (k2 w_j n, contact id bits)
+    float*          tc;      // per contact: sum_j w_j n.p_j of the vector being multiplied
+    uint32_t        nslots;  // 4 x contacts (set by the host before an SpMV)
 };
 struct ContactBuf
 {
@@ -112,6 +115,12 @@ struct CcdArgs
     const uint32_t* edges;
     const float*    offset;
     uint32_t*       toi_bits;
+    // two passes: the first runs every pair with an iteration budget and lists the pairs that
+    // need more; the second (rest = 1) runs the listed pairs with the full budget
+    int             budget;
+    int             rest;
+    uint32_t*       rest_list;   // capacity pair_capacity
+    uint32_t*       rest_count;
 };
 
 struct LsArgs
@@ -139,12 +148,17 @@ struct PcgArgs
     float*       p_old;
     float*       p_new;
     float*       q;
-    float*       partials;
+    float*       qc;       // contact part of q (3 per row, kept zero between iterations), null without contacts
+    float*       part_rz;  // per-block r.z partials of init / update (n_rz of them)
+    float*       part_pq;  // per-block p.q partials of the SpMV (n_pq of them)
+    uint32_t     n_rz, n_pq;
     PcgState*    s;
     float        rel2, abs_tol;
     uint32_t     check_every;
+    uint32_t     k;  // iteration index of the launch
 };
 
+void launch_fill_u32(uint32_t* a, uint32_t n, uint32_t v, hipStream_t s);
 void launch_pin_velocity(uint32_t nfix, const uint32_t* ids, const float* targets, const float* x, float* v, float dt,
                          hipStream_t s);
 void launch_predict(uint32_t nv, const float* x_step, const float* v, const uint8_t* fixed, float h, const float g[3],

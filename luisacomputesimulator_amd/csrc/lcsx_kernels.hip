@@ -26,30 +26,32 @@ LCSX_D float block_sum(float v, float* sh)
     return r;
 }
 
-// Deterministic grid reduction: every block writes its partial; the last block to arrive sums
-// the partials in index order (strided per thread, then block_sum) and returns true in that block.
-LCSX_D bool last_block_sum(float local, float* partials, uint32_t* ticket, float* total, float* sh)
+// Sum of n partials in a fixed order (strided per thread, then block_sum), returned in every
+// thread of the block. Every block of a launch computes the same value: the PCG scalars are
+// reduced this way from the previous launch's partials instead of by a last-block hand-off,
+// which would need device-scope fences (L2 write-back across XCDs) and a contended ticket.
+LCSX_D float block_reduce_all(const float* __restrict__ part, uint32_t n, float* sh)
 {
-    float bs = block_sum(local, sh);
-    __shared__ bool last;
-    if (threadIdx.x == 0)
-    {
-        partials[blockIdx.x] = bs;
-        __threadfence();
-        last = atomicAdd(ticket, 1u) == gridDim.x - 1;
-    }
+    __shared__ float bc;
+    float            v = 0.0f;
+    for (uint32_t i = threadIdx.x; i < n; i += kB) v = v + part[i];
+    v = block_sum(v, sh);
+    if (threadIdx.x == 0) bc = v;
     __syncthreads();
-    if (!last) return false;
-    __threadfence();
-    float s = 0.0f;
-    for (uint32_t i = threadIdx.x; i < gridDim.x; i += kB) s = s + partials[i];
-    s = block_sum(s, sh);
-    if (threadIdx.x == 0)
-    {
-        *total  = s;
-        *ticket = 0u;
-    }
-    return true;
+    v = bc;
+    __syncthreads();
+    return v;
+}
+
+// The flag read once per block (a launch may set it while other blocks start).
+LCSX_D bool block_flag(const uint32_t* flag)
+{
+    __shared__ uint32_t f;
+    if (threadIdx.x == 0) f = *flag;
+    __syncthreads();
+    bool r = f != 0u;
+    __syncthreads();
+    return r;
 }
 
 LCSX_D m3 ld_planes(const float* __restrict__ P, size_t stride, size_t i)
@@ -468,30 +470,68 @@ LCSX_D void contact_t(const ContactView& c, float* __restrict__ tc, PF pfun)
     }
 }
 
-// Contact slots of the block's rows [r0, r1): slots are contiguous in row order, so the whole
-// block walks them in strides of kB (coalesced, balanced over threads) and accumulates a t_c per
-// row in LDS. Called by every thread of the block.
-LCSX_D void contact_block(const VertContacts& vc, uint32_t r0, uint32_t r1, float* acc)
+// One 64-slot chunk per wavefront, starting at `base`: slot contributions a t_c are summed over
+// each run of equal rows by a segmented shuffle scan (slots are in row order); the last lane of
+// a run returns true with the run's row and sum. Called by all 64 lanes.
+LCSX_D bool slot_chunk(const VertContacts& vc, uint32_t base, uint32_t& row, f3& sum)
 {
-    for (uint32_t k = threadIdx.x; k < 3 * kB; k += kB) acc[k] = 0.0f;
-    __syncthreads();
-    const uint32_t s0 = vc.ptr[r0], s1 = vc.ptr[r1];
-    for (uint32_t e = s0 + threadIdx.x; e < s1; e += kB)
+    const uint32_t lane = lane_id();
+    uint32_t       e    = base + lane;
+    bool           ok   = e < vc.nslots;
+    row                 = 0xffffffffu;
+    float vx = 0.0f, vy = 0.0f, vz = 0.0f;
+    if (ok)
     {
-        float4   C   = vc.coef[e];
-        uint32_t row = vc.row[e] - r0;
-        float    t   = vc.tc[__float_as_uint(C.w)];
-        atomicAdd(&acc[3 * row], C.x * t);
-        atomicAdd(&acc[3 * row + 1], C.y * t);
-        atomicAdd(&acc[3 * row + 2], C.z * t);
+        float4 C = vc.coef[e];
+        float  t = vc.tc[__float_as_uint(C.w)];
+        row      = vc.row[e];
+        vx       = C.x * t;
+        vy       = C.y * t;
+        vz       = C.z * t;
     }
-    __syncthreads();
+    for (int off = 1; off < 64; off <<= 1)
+    {
+        float    ux = __shfl_up(vx, off), uy = __shfl_up(vy, off), uz = __shfl_up(vz, off);
+        uint32_t ur = __shfl_up(row, off);
+        if ((int)lane >= off && ur == row)
+        {
+            vx = vx + ux;
+            vy = vy + uy;
+            vz = vz + uz;
+        }
+    }
+    uint32_t rn = __shfl_down(row, 1);
+    sum         = mk(vx, vy, vz);
+    return ok && (lane == 63u || rn != row);
+}
+
+// Scalars of PCG iteration k (precond_cg.cpp:681-726 and the loop of 118-330): r.z from the
+// previous launch's partials, the convergence test (every check_every iterations: r.z below the
+// absolute tolerance, or below rel^2 r0.z0, or NaN) and beta = r.z / r.z_old (0 below 1e-10).
+struct IterScalars
+{
+    float rz, beta;
+    bool  stop;
+};
+LCSX_D IterScalars iter_scalars(const PcgArgs& a, float* sh)
+{
+    IterScalars o;
+    o.rz          = block_reduce_all(a.part_rz, a.n_rz, sh);
+    float rz0     = a.k == 0 ? o.rz : a.s->rz0;
+    bool  check   = (a.k % a.check_every) == 0u;
+    o.stop        = !(o.rz == o.rz) || (check && (o.rz < a.abs_tol || o.rz <= a.rel2 * rz0));
+    float rz_old  = a.k == 0 ? 0.0f : a.s->rz[(a.k - 1u) & 1u];
+    o.beta        = rz_old < 1e-10f ? 0.0f : o.rz / rz_old;
+    return o;
 }
 
 __global__ __launch_bounds__(kB) void k_contact_t_pcg(PcgArgs a, ContactView c, float* __restrict__ tc)
 {
-    if (a.s->converged) return;
-    const float beta = a.s->beta;
+    __shared__ float sh[4];
+    if (block_flag(&a.s->converged)) return;
+    IterScalars sc = iter_scalars(a, sh);
+    if (sc.stop) return;
+    const float beta = sc.beta;
     contact_t(c, tc, [&](uint32_t v) { return ld3(a.z, v) + beta * ld3(a.p_old, v); });
 }
 
@@ -535,123 +575,146 @@ __global__ __launch_bounds__(kB) void k_pcg_init(PcgArgs a)
         st3(a.z, i, z);
         st3(a.x, i, mk(0.0f, 0.0f, 0.0f));
         st3(a.p_old, i, mk(0.0f, 0.0f, 0.0f));
+        if (a.qc) st3(a.qc, i, mk(0.0f, 0.0f, 0.0f));
         loc = dot(r, z);
     }
-    float tot;
-    if (last_block_sum(loc, a.partials, &a.s->ticket, &tot, sh) && threadIdx.x == 0)
+    float bs = block_sum(loc, sh);
+    if (threadIdx.x == 0)
     {
-        a.s->rz     = tot;
-        a.s->rz0    = tot;
-        a.s->beta   = 0.0f;  // rz_old = 0 < 1e-10
-        a.s->iters  = 0;
-        a.s->converged = (!(tot == tot) || tot < a.abs_tol || tot <= a.rel2 * tot) ? 1u : 0u;
-        a.s->nan = (tot == tot) ? 0u : 1u;
+        a.part_rz[blockIdx.x] = bs;
+        if (blockIdx.x == 0)
+        {
+            a.s->converged = 0u;
+            a.s->iters     = 0u;
+            a.s->nan       = 0u;
+        }
     }
 }
 
 // q = A p_new with p_new = z + beta p_old formed on the fly for every gathered column: diagonal
 // block, SELL-64 material blocks, then the row's contact slots; p.q is reduced in the same launch
 // and the last block sets alpha = r.z / p.q (precond_cg.cpp:24-31 safe divide).
-__global__ __launch_bounds__(kB) void k_spmv_pcg(PcgArgs a, SellView A, VertContacts vc)
+// q = A p_new with p_new = z + beta p_old formed on the fly. Blocks [0, row_blocks) do the rows:
+// material diagonal and SELL-64 blocks, q written. The remaining blocks do the contact slots in
+// 64-slot chunks (balanced whatever the distribution of contacts over rows), adding each row's
+// sum into the contact accumulator qc (folded into q by the update). p.q is reduced over both
+// parts in the same launch and the last block sets alpha = r.z / p.q (precond_cg.cpp:24-31).
+__global__ __launch_bounds__(kB) void k_spmv_pcg(PcgArgs a, SellView A, VertContacts vc, uint32_t row_blocks)
 {
     __shared__ float sh[4];
-    __shared__ float acc[3 * kB];
-    if (a.s->converged) return;
-    uint32_t    i    = blockIdx.x * kB + threadIdx.x;
-    const float beta = a.s->beta;
-    f3          pi   = mk(0.0f, 0.0f, 0.0f), y = mk(0.0f, 0.0f, 0.0f);
-    if (i < a.n)
+    if (block_flag(&a.s->converged)) return;
+    IterScalars sc = iter_scalars(a, sh);
+    if (blockIdx.x == 0 && threadIdx.x == 0)
     {
-        pi = ld3(a.z, i) + beta * ld3(a.p_old, i);
-        st3(a.p_new, i, pi);
-        y           = mul(ld_planes(A.diag, a.n, i), pi);
-        uint32_t s  = i >> 6, l = i & 63u;
-        uint32_t k0 = A.sl_off[s], k1 = A.sl_off[s + 1];
-        for (uint32_t k = k0; k < k1; k++)
+        a.s->rz[a.k & 1u] = sc.rz;
+        if (a.k == 0) a.s->rz0 = sc.rz;
+        if (sc.stop)
         {
-            size_t   b  = (size_t)k * 64 + l;
-            uint32_t cl = A.col[b];
-            f3       pc = ld3(a.z, cl) + beta * ld3(a.p_old, cl);
-            m3       M;
-#pragma unroll
-            for (int cc = 0; cc < 9; cc++) M.m[cc] = A.val[cc * (size_t)A.nblocks + b];
-            y = y + mul(M, pc);
+            a.s->iters     = a.k;
+            a.s->nan       = (sc.rz == sc.rz) ? 0u : 1u;
+            a.s->converged = 1u;
         }
     }
-    if (vc.ptr)
+    if (sc.stop) return;
+    const float beta = sc.beta;
+    float       loc  = 0.0f;
+    if (blockIdx.x < row_blocks)
     {
-        uint32_t r0 = blockIdx.x * kB;
-        contact_block(vc, r0, min(r0 + kB, a.n), acc);
-        if (i < a.n) y = y + mk(acc[3 * threadIdx.x], acc[3 * threadIdx.x + 1], acc[3 * threadIdx.x + 2]);
+        uint32_t i = blockIdx.x * kB + threadIdx.x;
+        if (i < a.n)
+        {
+            f3 pi = ld3(a.z, i) + beta * ld3(a.p_old, i);
+            st3(a.p_new, i, pi);
+            f3       y  = mul(ld_planes(A.diag, a.n, i), pi);
+            uint32_t s  = i >> 6, l = i & 63u;
+            uint32_t k0 = A.sl_off[s], k1 = A.sl_off[s + 1];
+            for (uint32_t k = k0; k < k1; k++)
+            {
+                size_t   b  = (size_t)k * 64 + l;
+                uint32_t cl = A.col[b];
+                f3       pc = ld3(a.z, cl) + beta * ld3(a.p_old, cl);
+                m3       M;
+#pragma unroll
+                for (int cc = 0; cc < 9; cc++) M.m[cc] = A.val[cc * (size_t)A.nblocks + b];
+                y = y + mul(M, pc);
+            }
+            st3(a.q, i, y);
+            loc = dot(pi, y);
+        }
     }
-    float loc = 0.0f;
-    if (i < a.n)
+    else
     {
-        st3(a.q, i, y);
-        loc = dot(pi, y);
+        uint32_t row;
+        f3       sum;
+        if (slot_chunk(vc, (blockIdx.x - row_blocks) * kB + (threadIdx.x & ~63u), row, sum))
+        {
+            atomicAdd(&a.qc[3 * (size_t)row], sum.x);
+            atomicAdd(&a.qc[3 * (size_t)row + 1], sum.y);
+            atomicAdd(&a.qc[3 * (size_t)row + 2], sum.z);
+            loc = dot(ld3(a.z, row) + beta * ld3(a.p_old, row), sum);
+        }
     }
-    float tot;
-    if (last_block_sum(loc, a.partials, &a.s->ticket, &tot, sh) && threadIdx.x == 0)
-        a.s->alpha = tot < 1e-10f ? 0.0f : a.s->rz / tot;
+    float bs = block_sum(loc, sh);
+    if (threadIdx.x == 0) a.part_pq[blockIdx.x] = bs;
 }
 
-// y = A p (diagonal, material blocks and contacts), one thread per row.
-__global__ __launch_bounds__(kB) void k_spmv_plain(uint32_t n, SellView A, VertContacts vc, const float* __restrict__ p,
-                                                    float* __restrict__ y)
+// y = A p: rows (diagonal and material blocks) in one launch, then the contact slots added to y.
+__global__ __launch_bounds__(kB) void k_spmv_plain(uint32_t n, SellView A, const float* __restrict__ p, float* __restrict__ y)
 {
-    __shared__ float acc[3 * kB];
-    uint32_t         i   = blockIdx.x * kB + threadIdx.x;
-    f3               out = mk(0.0f, 0.0f, 0.0f);
-    if (i < n)
+    uint32_t i = blockIdx.x * kB + threadIdx.x;
+    if (i >= n) return;
+    f3       out = mul(ld_planes(A.diag, n, i), ld3(p, i));
+    uint32_t s   = i >> 6, l = i & 63u;
+    for (uint32_t k = A.sl_off[s]; k < A.sl_off[s + 1]; k++)
     {
-        out        = mul(ld_planes(A.diag, n, i), ld3(p, i));
-        uint32_t s = i >> 6, l = i & 63u;
-        for (uint32_t k = A.sl_off[s]; k < A.sl_off[s + 1]; k++)
-        {
-            size_t b = (size_t)k * 64 + l;
-            m3     M;
+        size_t b = (size_t)k * 64 + l;
+        m3     M;
 #pragma unroll
-            for (int cc = 0; cc < 9; cc++) M.m[cc] = A.val[cc * (size_t)A.nblocks + b];
-            out = out + mul(M, ld3(p, A.col[b]));
-        }
+        for (int cc = 0; cc < 9; cc++) M.m[cc] = A.val[cc * (size_t)A.nblocks + b];
+        out = out + mul(M, ld3(p, A.col[b]));
     }
-    if (vc.ptr)
+    st3(y, i, out);
+}
+
+__global__ __launch_bounds__(kB) void k_slots_plain(VertContacts vc, float* __restrict__ y)
+{
+    uint32_t row;
+    f3       sum;
+    if (slot_chunk(vc, blockIdx.x * kB + (threadIdx.x & ~63u), row, sum))
     {
-        uint32_t r0 = blockIdx.x * kB;
-        contact_block(vc, r0, min(r0 + kB, n), acc);
-        if (i < n) out = out + mk(acc[3 * threadIdx.x], acc[3 * threadIdx.x + 1], acc[3 * threadIdx.x + 2]);
+        atomicAdd(&y[3 * (size_t)row], sum.x);
+        atomicAdd(&y[3 * (size_t)row + 1], sum.y);
+        atomicAdd(&y[3 * (size_t)row + 2], sum.z);
     }
-    if (i < n) st3(y, i, out);
 }
 
 __global__ __launch_bounds__(kB) void k_pcg_update(PcgArgs a)
 {
     __shared__ float sh[4];
-    if (a.s->converged) return;
-    uint32_t i   = blockIdx.x * kB + threadIdx.x;
-    float    loc = 0.0f;
+    if (block_flag(&a.s->converged)) return;
+    const float pq    = block_reduce_all(a.part_pq, a.n_pq, sh);
+    const float rz    = a.s->rz[a.k & 1u];
+    const float alpha = pq < 1e-10f ? 0.0f : rz / pq;
+    uint32_t    i     = blockIdx.x * kB + threadIdx.x;
+    float       loc   = 0.0f;
     if (i < a.n)
     {
-        float alpha = a.s->alpha;
-        f3    pn    = ld3(a.p_new, i);
+        f3 pn = ld3(a.p_new, i);
         st3(a.x, i, ld3(a.x, i) + alpha * pn);
-        f3 r = ld3(a.r, i) - alpha * ld3(a.q, i);
+        f3 q = ld3(a.q, i);
+        if (a.qc)
+        {
+            q = q + ld3(a.qc, i);
+            st3(a.qc, i, mk(0.0f, 0.0f, 0.0f));
+        }
+        f3 r = ld3(a.r, i) - alpha * q;
         st3(a.r, i, r);
         f3 z = mul(ld_planes(a.minv, a.n, i), r);
         st3(a.z, i, z);
         loc = dot(r, z);
     }
-    float tot;
-    if (last_block_sum(loc, a.partials, &a.s->ticket, &tot, sh) && threadIdx.x == 0)
-    {
-        float rz_old = a.s->rz;
-        a.s->beta    = rz_old < 1e-10f ? 0.0f : tot / rz_old;
-        a.s->rz      = tot;
-        a.s->iters   = a.s->iters + 1;
-        bool check   = (a.s->iters % a.check_every) == 0;
-        if (!(tot == tot)) a.s->nan = 1u;
-        if (!(tot == tot) || (check && (tot < a.abs_tol || tot <= a.rel2 * a.s->rz0))) a.s->converged = 1u;
-    }
+    float bs = block_sum(loc, sh);
+    if (threadIdx.x == 0) a.part_rz[blockIdx.x] = bs;
 }
 
 // ---------------------------------------------------------------- line search
@@ -707,10 +770,11 @@ __global__ __launch_bounds__(kB) void k_ground_ccd_maxdx(LsArgs a)
 __global__ __launch_bounds__(kB) void k_ccd(CcdArgs a)
 {
     __shared__ float sh[4];
-    uint32_t         n   = min(*a.pair_count, a.pair_capacity);
+    uint32_t         n    = a.rest ? min(*a.rest_count, a.pair_capacity) : min(*a.pair_count, a.pair_capacity);
     float            tmin = kLineSearchMaxT;
-    for (uint32_t p = blockIdx.x * kB + threadIdx.x; p < n; p += gridDim.x * kB)
+    for (uint32_t idx = blockIdx.x * kB + threadIdx.x; idx < n; idx += gridDim.x * kB)
     {
+        uint32_t p = a.rest ? a.rest_list[idx] : idx;
         uint32_t q = a.pairs[2 * (size_t)p], o = a.pairs[2 * (size_t)p + 1];
         uint32_t v[4];
         if (a.kind == 0)
@@ -735,8 +799,12 @@ __global__ __launch_bounds__(kB) void k_ccd(CcdArgs a)
             x0[k] = ld3(a.x0, v[k]);
             x1[k] = ld3(a.x1, v[k]);
         }
-        float off = a.offset[v[0]] + a.offset[a.kind == 0 ? v[1] : v[2]];
-        tmin      = fminf(tmin, accd(x0, x1, off, a.kind));
+        float off  = a.offset[v[0]] + a.offset[a.kind == 0 ? v[1] : v[2]];
+        bool  done = true;
+        float t    = accd_budget(x0, x1, off, a.kind, a.rest ? 0x7fffffff : a.budget, &done);
+        if (done) tmin = fminf(tmin, t);
+        uint32_t slot = wave_append(a.rest_count, !done);
+        if (!done && slot < a.pair_capacity) a.rest_list[slot] = p;
     }
     tmin  = wave_min(tmin);
     int w = threadIdx.x >> 6;
@@ -784,6 +852,10 @@ __global__ void k_reset_status(Status* st)
 }  // namespace
 
 // ---------------------------------------------------------------- launchers
+void launch_fill_u32(uint32_t* a, uint32_t n, uint32_t v, hipStream_t s)
+{
+    if (n) k_fill_u32<<<grid_for(n, kB), kB, 0, s>>>(a, n, v);
+}
 void launch_pin_velocity(uint32_t nfix, const uint32_t* ids, const float* targets, const float* x, float* v, float dt, hipStream_t s)
 {
     if (nfix) k_pin_velocity<<<grid_for(nfix, kB), kB, 0, s>>>(nfix, ids, targets, x, v, dt);
@@ -841,17 +913,23 @@ void launch_pcg_contact_t(const PcgArgs& a, const ContactView& c, float* tc, uns
 void launch_pcg_spmv(const PcgArgs& a, const SellView& A, const VertContacts& vc, hipStream_t s, hipEvent_t start,
                      hipEvent_t stop)
 {
+    uint32_t rb = grid_for(a.n, kB);
+    uint32_t g  = rb + (vc.ptr ? grid_for(vc.nslots, kB) : 0u);
     if (start)
-        hipExtLaunchKernelGGL(k_spmv_pcg, dim3(grid_for(a.n, kB)), dim3(kB), 0, s, start, stop, 0, a, A, vc);
+        hipExtLaunchKernelGGL(k_spmv_pcg, dim3(g), dim3(kB), 0, s, start, stop, 0, a, A, vc, rb);
     else
-        k_spmv_pcg<<<grid_for(a.n, kB), kB, 0, s>>>(a, A, vc);
+        k_spmv_pcg<<<g, kB, 0, s>>>(a, A, vc, rb);
 }
 void launch_pcg_update(const PcgArgs& a, hipStream_t s) { k_pcg_update<<<grid_for(a.n, kB), kB, 0, s>>>(a); }
 void launch_spmv(const SellView& A, const ContactView& c, const VertContacts& vc, unsigned contact_grid, const float* p,
                  float* y, uint32_t n, hipStream_t s)
 {
-    if (vc.ptr) k_contact_t_plain<<<contact_grid, kB, 0, s>>>(c, p, vc.tc);
-    k_spmv_plain<<<grid_for(n, kB), kB, 0, s>>>(n, A, vc, p, y);
+    k_spmv_plain<<<grid_for(n, kB), kB, 0, s>>>(n, A, p, y);
+    if (vc.ptr && vc.nslots)
+    {
+        k_contact_t_plain<<<contact_grid, kB, 0, s>>>(c, p, vc.tc);
+        k_slots_plain<<<grid_for(vc.nslots, kB), kB, 0, s>>>(vc, y);
+    }
 }
 void launch_apply_dx(uint32_t nv, const float* x_iter, const float* dx, float* x, const uint32_t* alpha_bits,
                      float alpha_const, hipStream_t s)

@@ -386,21 +386,40 @@ __global__ __launch_bounds__(kBlock) void k_height(uint32_t n, const uint32_t* _
     }
 }
 
-// hist[h] for h < kMaxLevels; taller nodes are counted in hist[kMaxLevels]
+// hist[h] for h < kMaxLevels; taller nodes are counted in hist[kMaxLevels]. Per-block LDS
+// histograms, one global atomic per non-empty bin and block.
 __global__ __launch_bounds__(kBlock) void k_level_hist(uint32_t ni, const uint32_t* __restrict__ height, uint32_t* hist)
 {
+    __shared__ uint32_t cnt[kMaxLevels + 1];
+    for (uint32_t b = threadIdx.x; b <= kMaxLevels; b += kBlock) cnt[b] = 0u;
+    __syncthreads();
     uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i < ni) atomicAdd(&hist[min(height[i], kMaxLevels)], 1u);
+    if (i < ni) atomicAdd(&cnt[min(height[i], kMaxLevels)], 1u);
+    __syncthreads();
+    for (uint32_t b = threadIdx.x; b <= kMaxLevels; b += kBlock)
+        if (cnt[b]) atomicAdd(&hist[b], cnt[b]);
 }
 
+// order[level_ptr[h] ..] = inner nodes of height h (any order within a level)
 __global__ __launch_bounds__(kBlock) void k_level_scatter(uint32_t ni, const uint32_t* __restrict__ height,
                                                            const uint32_t* __restrict__ level_ptr, uint32_t* cursor,
                                                            uint32_t* __restrict__ order)
 {
+    __shared__ uint32_t cnt[kMaxLevels], base[kMaxLevels];
+    for (uint32_t b = threadIdx.x; b < kMaxLevels; b += kBlock) cnt[b] = 0u;
+    __syncthreads();
     uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= ni) return;
-    uint32_t h = height[i];
-    order[level_ptr[h] + atomicAdd(&cursor[h], 1u)] = i;
+    uint32_t h = 0, r = 0;
+    if (i < ni)
+    {
+        h = height[i];
+        r = atomicAdd(&cnt[h], 1u);
+    }
+    __syncthreads();
+    for (uint32_t b = threadIdx.x; b < kMaxLevels; b += kBlock)
+        if (cnt[b]) base[b] = atomicAdd(&cursor[b], cnt[b]);
+    __syncthreads();
+    if (i < ni) order[level_ptr[h] + base[h] + r] = i;
 }
 
 // update_*_leave_aabb, lbvh.cpp:636-683: leaf box = AABB(x0 and x1 of the element) +- pad[v0]

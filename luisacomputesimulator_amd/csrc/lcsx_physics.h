@@ -156,8 +156,17 @@ LCSX_HD float ccd_dist2(const f3 y[4], int kind)
     return kind == 0 ? pt_dist2(y[0], y[1], y[2], y[3]) : ee_dist2(y[0], y[1], y[2], y[3]);
 }
 
-// Raw ACCD time of impact in (0, 1.25]; 1.25 means no impact before the cap.
-LCSX_HD float accd(const f3 a0[4], const f3 a1[4], float offset, int kind)
+// Raw ACCD time of impact in (0, 1.25]; 1.25 means no impact before the cap. With a budget
+// below the reference's 10001 iterations, a pair that needs more returns with *done = false
+// (its result is then obtained by running it again with the full budget: the iteration is
+// deterministic, so the split changes nothing).
+//
+// The iteration's only state is toi and its exits depend only on toi, so once toi repeats a
+// value it is periodic and ends only at the iteration cap, with toi = 0.001. A toi saved at
+// power-of-two iteration counts (Brent's cycle detection) finds such a repeat and returns that
+// result early; the result is the same as running to the cap.
+LCSX_HD float accd_budget(const f3 a0[4], const f3 a1[4], float offset, int kind, int budget, bool* done)
+
 {
     f3 x0[4], dx[4];
     f3 m0 = mk(0.0f, 0.0f, 0.0f), m1 = mk(0.0f, 0.0f, 0.0f);
@@ -184,6 +193,7 @@ LCSX_HD float accd(const f3 a0[4], const f3 a1[4], float offset, int kind)
             um    = smax(um, dot(du, du));
         }
     float u_max = sqrtf(um);
+    *done       = true;
     if (u_max == 0.0f) return kLineSearchMaxT;
     float toi = 0.0f, max_t = kLineSearchMaxT;
     float eps       = kCcdReduction * (sqrtf(ccd_dist2(x0, kind)) - offset);
@@ -191,11 +201,24 @@ LCSX_HD float accd(const f3 a0[4], const f3 a1[4], float offset, int kind)
     float eps_sqr   = eps * eps;
     float inv_u_max = 1.0f / u_max;
     int   it        = 0;
+    float saved     = -1.0f;  // toi at the last checkpoint (toi is never negative)
+    int   lap = 0, power = 1;
     while (true)
     {
-        if (it > 10000)
+        if (it > 10000 || toi == saved)
         {
             toi = 0.001f;
+            break;
+        }
+        if (++lap == power)
+        {
+            saved = toi;
+            power <<= 1;
+            lap = 0;
+        }
+        if (it >= budget)
+        {
+            *done = false;
             break;
         }
         it += 1;
@@ -222,6 +245,12 @@ LCSX_HD float accd(const f3 a0[4], const f3 a1[4], float offset, int kind)
         }
     }
     return toi;
+}
+
+LCSX_HD float accd(const f3 a0[4], const f3 a1[4], float offset, int kind)
+{
+    bool done;
+    return accd_budget(a0, a1, offset, kind, 0x7fffffff, &done);
 }
 
 // ---------------------------------------------------------------- log barrier

@@ -12,6 +12,9 @@ namespace lcsx
 namespace
 {
 constexpr unsigned kPairGridMax = 4096;
+// ACCD iterations per pair in the first CCD pass; pairs that need more are run again in a
+// second pass over a compact list, so long iterations do not hold whole wavefronts
+constexpr int      kCcdFirstPassIters = 32;
 constexpr int      kPcgChunk    = 10;
 
 struct EventPool
@@ -118,7 +121,11 @@ void Solver::alloc_collision(uint32_t vf, uint32_t ee, uint32_t nc)
     c_nk.alloc(cap_c);
     c_w.alloc(cap_c);
     c_aux.alloc(cap_c);
+    ccd_rest_vf.alloc(cap_vf);
+    ccd_rest_ee.alloc(cap_ee);
     vc_list.alloc(4 * (size_t)cap_c);
+    // per-block partial sums of the SpMV: row blocks + slot blocks (4 slots per contact)
+    part_pq.alloc(grid_for(S.nv, 256) + (4 * (size_t)cap_c + 255) / 256 + 1);
     vc_row.alloc(4 * (size_t)cap_c);
     vc_coef.alloc(4 * (size_t)cap_c);
     vc_tc.alloc(cap_c);
@@ -143,12 +150,14 @@ void Solver::init()
     diag_mat.alloc(9 * (size_t)nv);
     minv.alloc(9 * (size_t)nv);
     sell_val.alloc(9 * (size_t)std::max(S.nblocks, 1u));
-    partials.alloc(grid_for(nv, 256) + 1);
+    vc_qc.alloc(3 * (size_t)nv);
+    part_rz.alloc(grid_for(nv, 256) + 1);
     upload_state_constants();
     tree_face.init(3, faces.p, S.nf);
     tree_edge.init(2, edges.p, S.ne);
     alloc_collision(24 * nv, 24 * S.ne, 12 * nv);
     vc_cnt.alloc(nv + 1);
+    ccd_rest_count.alloc(2);
     vc_ptr.alloc(nv + 1);
     status.alloc(1);
     pcg_state.alloc(1);
@@ -212,11 +221,17 @@ ContactView Solver::contact_view() const
 {
     return ContactView{c_idx.p, c_nk.p, c_w.p, c_aux.p, &status.p->n_contacts, cap_c};
 }
-VertContacts Solver::vert_contacts() const
+VertContacts Solver::vert_contacts()
 {
-    // no lists before the first contact assembly, or without self collision
-    return (prm.use_self_collision && vc_ready) ? VertContacts{vc_ptr.p, vc_list.p, vc_row.p, vc_coef.p, vc_tc.p}
-                                                : VertContacts{nullptr, nullptr, nullptr, nullptr, nullptr};
+    // no lists before the first contact assembly, or without self collision; the slot count
+    // sizes the SpMV's grid, so it is read back once (one small synchronisation)
+    if (!(prm.use_self_collision && vc_ready)) return VertContacts{nullptr, nullptr, nullptr, nullptr, nullptr, 0u};
+    uint32_t nc = 0;
+    LCSX_HIP(hipMemcpyAsync(&nc, &status.p->n_contacts, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+    LCSX_HIP(hipStreamSynchronize(stream));
+    nc = std::min(nc, cap_c);
+    if (nc == 0) return VertContacts{nullptr, nullptr, nullptr, nullptr, nullptr, 0u};
+    return VertContacts{vc_ptr.p, vc_list.p, vc_row.p, vc_coef.p, vc_tc.p, 4u * nc};
 }
 // The SpMV's diagonal is the material one: contact diagonal blocks enter through the contact slots.
 SellView Solver::sell_view() const { return SellView{sl_off.p, sell_col.p, sell_val.p, diag_mat.p, S.nblocks}; }
@@ -380,7 +395,8 @@ void Solver::contact_assemble()
     launch_contact_adjacency(contact_view(), S.nv, pair_grid(cap_c), vc_cnt.p, vc_ptr.p, vc_list.p, vc_row.p, vc_coef.p,
                              stream);
     vc_ready = true;
-    launch_contact_assemble(contact_view(), vert_contacts(), cgB.p, diag.p, S.nv, stream);
+    VertContacts vc{vc_ptr.p, vc_list.p, vc_row.p, vc_coef.p, vc_tc.p, 0u};  // assembly walks the per-vertex lists
+    launch_contact_assemble(contact_view(), vc, cgB.p, diag.p, S.nv, stream);
     toc("contact_assemble");
 }
 
@@ -398,15 +414,19 @@ void Solver::pcg()
     a.p_old    = p[0].p;
     a.p_new    = p[1].p;
     a.q        = q.p;
-    a.partials = partials.p;
+    a.part_rz  = part_rz.p;
+    a.part_pq  = part_pq.p;
     a.s        = pcg_state.p;
     a.rel2     = prm.pcg_rel_tol * prm.pcg_rel_tol;
     a.abs_tol  = prm.pcg_abs_tol;
     a.check_every = (uint32_t)std::max(1, prm.pcg_check_every);
-    launch_pcg_init(a, stream);
-    SellView    A  = sell_view();
+    SellView     A  = sell_view();
     ContactView  cv = contact_view();
     VertContacts vc = vert_contacts();
+    a.qc            = vc.ptr ? vc_qc.p : nullptr;
+    a.n_rz          = grid_for(S.nv, 256);                                   // init / update blocks
+    a.n_pq          = a.n_rz + (vc.ptr ? grid_for(vc.nslots, 256) : 0u);    // SpMV row + slot blocks
+    launch_pcg_init(a, stream);
     PcgState     hs{};
     int         issued = 0;
     int         par    = 0;
@@ -419,6 +439,7 @@ void Solver::pcg()
             PcgArgs b = a;
             b.p_old   = p[par].p;
             b.p_new   = p[1 - par].p;
+            b.k       = (uint32_t)(issued + k);
             hipEvent_t e0 = nullptr, e1 = nullptr;
             if (timing_spmv)
             {
@@ -439,26 +460,21 @@ void Solver::pcg()
         if (hs.converged) break;
     }
     toc("pcg");
+    // iterations run: the stopping one when the test stopped the loop, else the cap
+    const uint32_t iters = hs.converged ? hs.iters : (uint32_t)issued;
     // only launches that did work (the first `iters`; later ones exit on the converged flag)
     if (!spmv_events.empty())
     {
         // contact slots the timed launches processed ("spmv_slots": slots per launch)
-        uint32_t nc = 0;
-        if (vc.ptr)
-        {
-            LCSX_HIP(hipMemcpyAsync(&nc, &status.p->n_contacts, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
-            LCSX_HIP(hipStreamSynchronize(stream));
-            nc = std::min(nc, cap_c);
-        }
-        uint32_t counted = std::min<uint32_t>(hs.iters, (uint32_t)spmv_events.size());
+        uint32_t counted = std::min<uint32_t>(iters, (uint32_t)spmv_events.size());
         auto&    t       = times["spmv_slots"];
-        t.first += 4.0 * nc * counted;
+        t.first += (double)vc.nslots * counted;
         t.second += (int)counted;
     }
     for (size_t k = 0; k < spmv_events.size(); k++)
     {
         float ms = 0.0f;
-        if (k < hs.iters && hipEventElapsedTime(&ms, spmv_events[k].first, spmv_events[k].second) == hipSuccess)
+        if (k < iters && hipEventElapsedTime(&ms, spmv_events[k].first, spmv_events[k].second) == hipSuccess)
         {
             auto& t = times["spmv"];
             t.first += ms;
@@ -468,9 +484,11 @@ void Solver::pcg()
         ev_put(spmv_events[k].second);
     }
     spmv_events.clear();
-    stats.pcg_iters = (int)hs.iters;
-    stats.pcg_iters_total += (int)hs.iters;
-    stats.pcg_rel   = hs.rz0 > 0.0f ? std::sqrt(hs.rz / hs.rz0) : 0.0f;
+    stats.pcg_iters = (int)iters;
+    stats.pcg_iters_total += (int)iters;
+    // r.z of the last evaluated iteration (the stopping one, or the last of the cap)
+    const float rz_last = hs.rz[(hs.converged ? hs.iters : (uint32_t)std::max(issued - 1, 0)) & 1u];
+    stats.pcg_rel   = hs.rz0 > 0.0f ? std::sqrt(rz_last / hs.rz0) : 0.0f;
     if (hs.nan)
     {
         set_error("NaN/Inf in the PCG iteration");
@@ -501,16 +519,26 @@ void Solver::ccd()
         c.edges    = edges.p;
         c.offset   = offset.p;
         c.toi_bits = &status.p->toi_bits;
-        c.kind     = 0;
-        c.pairs    = pairs_vf.p;
-        c.pair_count    = &status.p->n_vf_ccd;
-        c.pair_capacity = cap_vf;
-        launch_ccd(c, pair_grid(cap_vf), stream);
-        c.kind          = 1;
-        c.pairs         = pairs_ee.p;
-        c.pair_count    = &status.p->n_ee_ccd;
-        c.pair_capacity = cap_ee;
-        launch_ccd(c, pair_grid(cap_ee), stream);
+        c.budget   = kCcdFirstPassIters;
+        launch_fill_u32(ccd_rest_count.p, 2, 0u, stream);
+        for (int rest = 0; rest < 2; rest++)
+        {
+            c.rest          = rest;
+            c.kind          = 0;
+            c.pairs         = pairs_vf.p;
+            c.pair_count    = &status.p->n_vf_ccd;
+            c.pair_capacity = cap_vf;
+            c.rest_list     = ccd_rest_vf.p;
+            c.rest_count    = ccd_rest_count.p;
+            launch_ccd(c, pair_grid(cap_vf), stream);
+            c.kind          = 1;
+            c.pairs         = pairs_ee.p;
+            c.pair_count    = &status.p->n_ee_ccd;
+            c.pair_capacity = cap_ee;
+            c.rest_list     = ccd_rest_ee.p;
+            c.rest_count    = ccd_rest_count.p + 1;
+            launch_ccd(c, pair_grid(cap_ee), stream);
+        }
     }
     toc("ccd");
 }

@@ -27,7 +27,7 @@ struct Solver
     // state (3 floats per vertex)
     DBuf<float> x, v, x_step, x_iter, x_tilde, dx, cgB, r, z, p[2], q, spmv_tmp;
     DBuf<float> diag, diag_mat, minv, sell_val;  // 9 planes (diag: with contact blocks; diag_mat: material only)
-    DBuf<float> partials;
+    DBuf<float> part_rz, part_pq;  // PCG per-block partial sums
     // constants
     DBuf<float>    rest_x, offset, dhat, vert_mass, vert_area, face_area, edge_area, targets;
     DBuf<uint8_t>  is_fixed;
@@ -43,6 +43,8 @@ struct Solver
     DBuf<uint32_t> vc_cnt, vc_ptr, vc_list, vc_row;  // per-vertex contact slots (rebuilt with each contact assembly)
     DBuf<float4>   vc_coef;                          // per slot
     DBuf<float>    vc_tc;                    // per contact
+    DBuf<float>    vc_qc;                    // contact part of the SpMV (3 per vertex)
+    DBuf<uint32_t> ccd_rest_vf, ccd_rest_ee, ccd_rest_count;  // CCD pairs left for the full-budget pass
     bool           vc_ready = false;
     DBuf<Status>   status;
     DBuf<PcgState> pcg_state;
@@ -83,7 +85,7 @@ struct Solver
 
     ContactView contact_view() const;
     SellView    sell_view() const;
-    VertContacts vert_contacts() const;
+    VertContacts vert_contacts();
     unsigned    pair_grid(uint32_t cap) const;
 
     // timing helpers

@@ -2,8 +2,12 @@
 // the same source the HIP kernels run) against the oracle restatement (oracle/orc_*.h), bit for bit
 // in FP32 on random inputs. Prints "OK <cases>" or the first mismatch and exits non-zero.
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <fstream>
 #include <random>
+#include <sstream>
+#include <string>
 #include "../../luisacomputesimulator_amd/csrc/lcsx_physics.h"
 #include "../../oracle/orc_collision.h"
 #include "../../oracle/orc_energy.h"
@@ -22,8 +26,63 @@ static void same(float a, float b, const char* what, int k)
     }
 }
 
-int main()
+// ACCD inputs that run into the 10001-iteration cap (tests/golden/accd_stall_cases.txt): the
+// product's early exit on a repeated toi must give the oracle's full-length result.
+static int stall_cases(const char* path)
 {
+    std::ifstream f(path);
+    if (!f)
+    {
+        std::printf("MISSING %s\n", path);
+        g_fail++;
+        return 0;
+    }
+    std::string line;
+    int         n = 0;
+    while (std::getline(f, line))
+    {
+        std::istringstream ss(line);
+        int                kind;
+        std::string        tok;
+        ss >> kind;
+        float v[25];
+        for (int i = 0; i < 25; i++)
+        {
+            ss >> tok;
+            v[i] = std::strtof(tok.c_str(), nullptr);
+        }
+        lcsx::f3       a[4], b[4];
+        orc::V3<float> oa[4], ob[4];
+        for (int k = 0; k < 4; k++)
+        {
+            a[k]  = lcsx::mk(v[1 + 3 * k], v[2 + 3 * k], v[3 + 3 * k]);
+            b[k]  = lcsx::mk(v[13 + 3 * k], v[14 + 3 * k], v[15 + 3 * k]);
+            oa[k] = orc::V3<float>(a[k].x, a[k].y, a[k].z);
+            ob[k] = orc::V3<float>(b[k].x, b[k].y, b[k].z);
+        }
+        int   iters = 0;
+        float ref   = orc::accd(oa, ob, v[0], kind, &iters);
+        if (iters <= 10000)
+        {
+            std::printf("NOT A STALL case %d (%d iterations)\n", n, iters);
+            g_fail++;
+        }
+        same(lcsx::accd(a, b, v[0], kind), ref, "accd_stall", n);
+        bool  done = true;
+        float part = lcsx::accd_budget(a, b, v[0], kind, 32, &done);
+        if (done && part != ref)
+        {
+            std::printf("BUDGET case %d finished early with %.9g\n", n, part);
+            g_fail++;
+        }
+        n++;
+    }
+    return n;
+}
+
+int main(int argc, char** argv)
+{
+    int stalls = stall_cases(argc > 1 ? argv[1] : "tests/golden/accd_stall_cases.txt");
     std::mt19937_64                       rng(1234);
     std::uniform_real_distribution<float> U(-1.0f, 1.0f);
     auto v  = [&]() { return lcsx::mk(U(rng), U(rng), U(rng)); };
@@ -105,6 +164,6 @@ int main()
         std::printf("FAIL %d mismatches\n", g_fail);
         return 1;
     }
-    std::printf("OK %d\n", N);
+    std::printf("OK %d random cases, %d ACCD stall cases\n", N, stalls);
     return 0;
 }

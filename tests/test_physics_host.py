@@ -19,5 +19,6 @@ def test_device_math_matches_oracle_on_host(tmp_path):
            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", src, "-o", exe]
     r = subprocess.run(cmd, capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
-    out = subprocess.run([exe], capture_output=True, text=True, timeout=300)
+    stalls = os.path.join(ROOT, "tests", "golden", "accd_stall_cases.txt")
+    out = subprocess.run([exe, stalls], capture_output=True, text=True, timeout=300)
     assert out.returncode == 0 and out.stdout.startswith("OK"), out.stdout + out.stderr
