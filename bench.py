@@ -39,16 +39,18 @@ def spmv_bytes(n_rows: int, n_blocks: int, slots: float = 0.0) -> float:
 
 
 def traffic_from_profiles():
-    """HBM bytes per SpMV launch from the committed rocprofv3 counter summary, if present
-    (the line `hbm_bytes_per_launch: N` of profiles/k_spmv_pcg_tube88k_counters.md)."""
+    """HBM bytes per SpMV launch and the algorithmic bytes of that same launch, from the committed
+    rocprofv3 counter summary (lines `hbm_bytes_per_launch:` / `algorithmic_bytes_per_launch:` of
+    profiles/k_spmv_pcg_tube88k_counters.md), or (None, None)."""
     path = os.path.join(ROOT, "profiles", "k_spmv_pcg_tube88k_counters.md")
-    if not os.path.exists(path):
-        return None
-    with open(path) as f:
-        for line in f:
-            if line.strip().startswith("hbm_bytes_per_launch:"):
-                return float(line.split(":", 1)[1].strip())
-    return None
+    out = {}
+    if os.path.exists(path):
+        with open(path) as f:
+            for line in f:
+                key, _, val = line.strip().partition(":")
+                if key in ("hbm_bytes_per_launch", "algorithmic_bytes_per_launch") and val.strip():
+                    out[key] = float(val.strip())
+    return out.get("hbm_bytes_per_launch"), out.get("algorithmic_bytes_per_launch")
 
 
 def profile_pcg(g, copy_mib: int = 256) -> None:
@@ -137,6 +139,7 @@ def main():
     nbytes = spmv_bytes(n_rows, n_blocks, slots)
     achieved = nbytes / (spmv_ms * 1e-3) / 1e9 if spmv_n else None
     st = g.stats()
+    traffic = traffic_from_profiles()
     line = {
         "metric": METRIC,
         "value": newton / elapsed,
@@ -159,7 +162,8 @@ def main():
                    "contacts_last_iter": st["num_contacts"],
                    "pairs_last_iter": st["num_pairs_vf"] + st["num_pairs_ee"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": traffic_from_profiles(),
+                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": traffic[0],
+                     "traffic_algorithmic_bytes": traffic[1],
                      "kernel": "k_spmv_pcg", "bytes_per_launch": nbytes, "avg_launch_ms": spmv_ms,
                      "contact_slots_per_launch": slots,
                      "launches": spmv_n},
