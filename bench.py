@@ -91,7 +91,11 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--rings", type=int, default=297)
+    ap.add_argument("--workload", choices=("tube", "twist"), default="tube",
+                    help="tube: the 297 x 297 cloth tube on the spinning cylinder (the headline workload); "
+                         "twist: the 513 x 513 sheet twisted by its two pinned edges")
+    ap.add_argument("--rings", type=int, default=297, help="tube: rings (= vertices per ring)")
+    ap.add_argument("--n", type=int, default=513, help="twist: vertices per side")
     ap.add_argument("--profile-pcg", action="store_true",
                     help="after the warm-up frames run one calibration copy and one PCG solve, then exit "
                          "(workload for rocprofv3 counter runs)")
@@ -104,7 +108,14 @@ def main():
 
     from luisacomputesimulator_amd import NewtonSolver, hiprt, scenes
 
-    sc = scenes.cylinder_tube(rings=args.rings, per_ring=args.rings)
+    if args.workload == "tube":
+        sc = scenes.cylinder_tube(rings=args.rings, per_ring=args.rings)
+        label = (f"cloth tube {args.rings}x{args.rings} ({sc.meshes[0].verts.shape[0]} verts, "
+                 f"{sc.meshes[0].tris.shape[0]} tris) around a spinning kinematic cylinder, dt 1/60 s, kappa 1e9, FP32")
+    else:
+        sc = scenes.twisting_sheet(args.n)
+        label = (f"twisting sheet {args.n}x{args.n} ({sc.num_verts} verts, {sc.num_faces} tris), edges turned "
+                 "at -/+ pi rad/s, d_hat 5e-4 m, dt 1/60 s, FP32")
     g = NewtonSolver.from_scene(sc)
     g.set_timing(2)
     n_rows = g.num_verts
@@ -153,9 +164,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"cloth tube {args.rings}x{args.rings} ({sc.meshes[0].verts.shape[0]} verts, "
-                               f"{sc.meshes[0].tris.shape[0]} tris) around a spinning kinematic cylinder, "
-                               "dt 1/60 s, kappa 1e9, FP32",
+        "config": {"workload": label,
                    "verts_total": n_rows, "material_blocks": n_blocks, "frames": args.steps,
                    "newton_iters": newton, "pcg_iters": pcg,
                    "frames_per_s": args.steps / elapsed,

@@ -148,15 +148,16 @@ def grid_triangles(rows: int, cols: int, wrap_cols: bool = False) -> np.ndarray:
     return tris
 
 
-def staggered_ring_triangles(rings: int, per_ring: int) -> np.ndarray:
-    """Triangles of a closed tube whose odd rings are turned by half a vertex spacing (vertex
-    id = r * per_ring + c): every triangle is isosceles with its apex on the neighbouring ring,
-    so no vertex lies closer to a non-incident triangle than the smaller of the ring spacing and
-    the triangle's slant altitude."""
-    tris = np.empty(((rings - 1) * per_ring * 2, 3), dtype=np.int32)
+def staggered_ring_triangles(rings: int, per_ring: int, wrap: bool = True) -> np.ndarray:
+    """Triangles of a grid whose odd rows (rings) are shifted by half a vertex spacing (vertex
+    id = r * per_ring + c), closed around the row direction when `wrap` (a tube): every triangle
+    is isosceles with its apex on the neighbouring row, so no vertex lies closer to a
+    non-incident triangle than the smaller of the row spacing and the triangle's slant altitude."""
+    ncols = per_ring if wrap else per_ring - 1
+    tris = np.empty(((rings - 1) * ncols * 2, 3), dtype=np.int32)
     k = 0
     for r in range(rings - 1):
-        for c in range(per_ring):
+        for c in range(ncols):
             c1 = (c + 1) % per_ring
             a, b = r * per_ring + c, r * per_ring + c1
             u, w = (r + 1) * per_ring + c, (r + 1) * per_ring + c1
@@ -236,16 +237,22 @@ def cylinder_tube(rings: int = 297, per_ring: int = 297, radius: float = 0.11, l
 
 
 def twisting_sheet(n: int = 513, omega_deg: float = 180.0) -> Scene:
-    """n x n sheet of 1 x 1 m whose two opposite edges rotate in opposite directions about the
-    sheet's centre line (x axis), d_hat 5e-4 m, no gravity, no ground."""
-    xs = np.linspace(-0.5, 0.5, n)
-    z, x = np.meshgrid(xs, xs, indexing="ij")
+    """n x n sheet of 1 x 1 m whose two opposite edges (x = -0.5 and x = +0.5) rotate in opposite
+    directions about the sheet's centre line (the x axis), d_hat 5e-4 m, no gravity, no ground.
+
+    Rows of constant z run along x; odd rows are shifted by half a spacing (staggered
+    triangulation): split on diagonals, a 1.95 mm cell's altitude (1.38 mm) would lie inside the
+    1.5 mm contact band (two 0.5 mm offsets + d_hat) and every cell would start in contact."""
+    h = 1.0 / (n - 1)
+    r, c = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    x = -0.5 + h * (c + 0.5 * (r % 2))
+    z = -0.5 + h * r
     verts = np.stack([x, np.zeros_like(x), z], -1).reshape(-1, 3).astype(np.float32)
     mat = Material(thickness=1e-3, youngs_modulus=1e4, poisson_ratio=0.3, area_bending_stiffness=1e-4,
                    density=200.0, d_hat=5e-4)
     left = Anim(use_rotate=True, rot_axis=(1.0, 0.0, 0.0), rot_deg_per_s=omega_deg)
     right = Anim(use_rotate=True, rot_axis=(1.0, 0.0, 0.0), rot_deg_per_s=-omega_deg)
-    mesh = Mesh("sheet", verts, grid_triangles(n, n), mat,
+    mesh = Mesh("sheet", verts, staggered_ring_triangles(n, n, wrap=False), mat,
                 [FixedPoints("Left", 0.001, left), FixedPoints("Right", 0.001, right)])
     return Scene(f"twisting_sheet_{n}", [mesh], Params(use_floor=False, gravity=(0.0, 0.0, 0.0)), frames=120)
 
