@@ -9,12 +9,18 @@ frame runs >= 3 Newton iterations, each with <= 100 PCG iterations); `value` is 
 iterations per second over the timed frames, with every input resident on the GPU.
 
 The JSON line carries:
-  roofline      the PCG block SpMV k_spmv_pcg (material blocks and contact slots): algorithmic
-                bytes per launch = 40 B per off-diagonal block (36 B values + 4 B column) +
-                84 B per row (36 B diagonal block, z and p_old of the row 24 B, p_new and q
-                writes 24 B) + 4 B per 64-row slice + 24 B per contact slot,
+  value         Newton steps/s in FP32 (the reference's precision).
+  mixed         the same frames in mixed precision (FP16 off-diagonal blocks with a power-of-two
+                scale per block inside the PCG, FP32 vectors with residual replacement every 25
+                iterations, FP64 CCD and barrier terms): Newton steps/s and its SpMV figures.
+  roofline      the PCG block SpMV k_spmv_pcg of the FP32 run (material blocks and contact
+                slots): algorithmic bytes per launch = 40 B per off-diagonal block (36 B values +
+                4 B column) + 84 B per row (36 B diagonal block, z and p_old of the row 24 B,
+                p_new and q writes 24 B) + 4 B per 64-row slice + 24 B per contact slot,
                 divided by the launch's average duration from hipEvents attached to the
-                launches (hipExtLaunchKernelGGL) on the solver stream; peak = 8 TB/s HBM3E.
+                launches (hipExtLaunchKernelGGL) on the solver stream; peak = 8 TB/s HBM3E. The
+                FP16 launch moves 23 B per off-diagonal block (18 B values, 1 B exponent, 4 B
+                column) and the same per row.
   cpu_baseline  the CPU oracle (C++ restatement of the reference path, 1 thread) timed on
                 frame 0 of the same workload on this host.
 """
@@ -31,11 +37,12 @@ METRIC = "Newton steps/s @88K-vert cloth, mixed vs FP32; SpMV HBM GB/s; scene-st
 HBM_PEAK_GBS = 8000.0
 
 
-def spmv_bytes(n_rows: int, n_blocks: int, slots: float = 0.0) -> float:
-    """Algorithmic bytes of one k_spmv_pcg launch: 40 B per off-diagonal block + 84 B per row +
-    4 B per 64-row slice offset, and 24 B per contact slot (16 B coefficient, 4 B row, the
-    contact's 4 B t_c)."""
-    return 40 * n_blocks + 84 * n_rows + 4 * ((n_rows + 63) // 64) + 24 * slots
+def spmv_bytes(n_rows: int, n_blocks: int, slots: float = 0.0, block_bytes: int = 40) -> float:
+    """Algorithmic bytes of one k_spmv_pcg launch: block_bytes per off-diagonal block (40 in FP32:
+    36 B values + 4 B column; 23 with FP16 blocks: 18 B values + 1 B exponent + 4 B column) +
+    84 B per row + 4 B per 64-row slice offset, and 24 B per contact slot (16 B coefficient, 4 B
+    row, the contact's 4 B t_c)."""
+    return block_bytes * n_blocks + 84 * n_rows + 4 * ((n_rows + 63) // 64) + 24 * slots
 
 
 def traffic_from_profiles():
@@ -96,6 +103,10 @@ def main():
                          "twist: the 513 x 513 sheet twisted by its two pinned edges")
     ap.add_argument("--rings", type=int, default=297, help="tube: rings (= vertices per ring)")
     ap.add_argument("--n", type=int, default=513, help="twist: vertices per side")
+    ap.add_argument("--precision", choices=("fp32", "mixed", "both"), default="both",
+                    help="fp32: the reference's precision (the headline value); mixed: FP16 blocks in the PCG "
+                         "with FP32 residual replacement, FP64 CCD/barrier; both: FP32 headline plus a mixed "
+                         "run of the same frames")
     ap.add_argument("--profile-pcg", action="store_true",
                     help="after the warm-up frames run one calibration copy and one PCG solve, then exit "
                          "(workload for rocprofv3 counter runs)")
@@ -108,76 +119,106 @@ def main():
 
     from luisacomputesimulator_amd import NewtonSolver, hiprt, scenes
 
-    if args.workload == "tube":
-        sc = scenes.cylinder_tube(rings=args.rings, per_ring=args.rings)
-        label = (f"cloth tube {args.rings}x{args.rings} ({sc.meshes[0].verts.shape[0]} verts, "
-                 f"{sc.meshes[0].tris.shape[0]} tris) around a spinning kinematic cylinder, dt 1/60 s, kappa 1e9, FP32")
-    else:
-        sc = scenes.twisting_sheet(args.n)
-        label = (f"twisting sheet {args.n}x{args.n} ({sc.num_verts} verts, {sc.num_faces} tris), edges turned "
-                 "at -/+ pi rad/s, d_hat 5e-4 m, dt 1/60 s, FP32")
-    g = NewtonSolver.from_scene(sc)
-    g.set_timing(2)
-    n_rows = g.num_verts
-    n_blocks = len(g.read("csr")) - (n_rows + 1)
-    def progress(tag, f, st, t):
-        print(f"[bench] {tag} frame {f}: {st['newton_iters']} Newton, {st['pcg_iters_total']} PCG iterations, "
-              f"{st['num_contacts']} contacts, {t:.2f} s", file=sys.stderr, flush=True)
+    def make_scene():
+        return scenes.cylinder_tube(rings=args.rings, per_ring=args.rings) if args.workload == "tube" \
+            else scenes.twisting_sheet(args.n)
 
-    for f in range(args.warmup):
-        tw = time.perf_counter()
-        g.physics_step_gpu(1)
-        progress("warmup", f, g.stats(), time.perf_counter() - tw)
-    if args.profile_pcg:
-        profile_pcg(g)
-        return
-    g.kernel_times(reset=True)
-    hiprt.synchronize()
-    newton = pcg = 0
-    t0 = time.perf_counter()
-    for f in range(args.steps):
-        tf = time.perf_counter()
-        g.physics_step_gpu(1)
-        st = g.stats()
-        newton += st["newton_iters"]
-        pcg += st["pcg_iters_total"]
-        progress("timed", f, st, time.perf_counter() - tf)
-    hiprt.synchronize()
-    elapsed = time.perf_counter() - t0
-    kt = g.kernel_times()
-    spmv_ms, spmv_n = kt.get("spmv", (float("nan"), 0))
-    slots = kt.get("spmv_slots", (0.0, 0))[0]
-    nbytes = spmv_bytes(n_rows, n_blocks, slots)
-    achieved = nbytes / (spmv_ms * 1e-3) / 1e9 if spmv_n else None
-    st = g.stats()
+    sc = make_scene()
+    if args.workload == "tube":
+        label = (f"cloth tube {args.rings}x{args.rings} ({sc.meshes[0].verts.shape[0]} verts, "
+                 f"{sc.meshes[0].tris.shape[0]} tris) around a spinning kinematic cylinder, dt 1/60 s, kappa 1e9")
+    else:
+        label = (f"twisting sheet {args.n}x{args.n} ({sc.num_verts} verts, {sc.num_faces} tris), edges turned "
+                 "at -/+ pi rad/s, d_hat 5e-4 m, dt 1/60 s")
+
+    def run(precision):
+        """Warm-up and timed frames of a fresh solver in one precision."""
+        s = make_scene()
+        s.params.precision = precision
+        g = NewtonSolver.from_scene(s)
+        g.set_timing(2)
+        tag = precision
+
+        def progress(kind, f, st, t):
+            print(f"[bench] {tag} {kind} frame {f}: {st['newton_iters']} Newton, {st['pcg_iters_total']} PCG "
+                  f"iterations, {st['num_contacts']} contacts, {t:.2f} s", file=sys.stderr, flush=True)
+
+        for f in range(args.warmup):
+            tw = time.perf_counter()
+            g.physics_step_gpu(1)
+            progress("warmup", f, g.stats(), time.perf_counter() - tw)
+        if args.profile_pcg:
+            profile_pcg(g)
+            return None
+        g.kernel_times(reset=True)
+        hiprt.synchronize()
+        newton = pcg = 0
+        t0 = time.perf_counter()
+        for f in range(args.steps):
+            tf = time.perf_counter()
+            g.physics_step_gpu(1)
+            st = g.stats()
+            newton += st["newton_iters"]
+            pcg += st["pcg_iters_total"]
+            progress("timed", f, st, time.perf_counter() - tf)
+        hiprt.synchronize()
+        elapsed = time.perf_counter() - t0
+        kt = g.kernel_times()
+        n_rows = g.num_verts
+        n_blocks = len(g.read("csr")) - (n_rows + 1)
+        spmv_ms, spmv_n = kt.get("spmv", (float("nan"), 0))
+        slots = kt.get("spmv_slots", (0.0, 0))[0]
+        nbytes = spmv_bytes(n_rows, n_blocks, slots, 23 if precision == "mixed" else 40)
+        return {"elapsed": elapsed, "newton": newton, "pcg": pcg, "kt": kt, "st": g.stats(), "n_rows": n_rows,
+                "n_blocks": n_blocks, "spmv_ms": spmv_ms, "spmv_n": spmv_n, "slots": slots, "nbytes": nbytes,
+                "achieved": nbytes / (spmv_ms * 1e-3) / 1e9 if spmv_n else None}
+
+    precisions = {"fp32": ["fp32"], "mixed": ["mixed"], "both": ["fp32", "mixed"]}[args.precision]
+    res = {}
+    for prec in precisions:
+        res[prec] = run(prec)
+        if res[prec] is None:
+            return
+    head = res[precisions[0]]
+    n_rows, n_blocks, st, kt = head["n_rows"], head["n_blocks"], head["st"], head["kt"]
     traffic = traffic_from_profiles()
     line = {
         "metric": METRIC,
-        "value": newton / elapsed,
+        "value": head["newton"] / head["elapsed"],
         "unit": "Newton steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": 1e3 * elapsed / args.steps,
+        "ms_per_step": 1e3 * head["elapsed"] / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "f32" if precisions[0] == "fp32" else "f16/f32/f64 mixed",
         "data": "synthetic",
-        "config": {"workload": label,
+        "config": {"workload": label + (", FP32" if precisions[0] == "fp32" else ", mixed precision"),
                    "verts_total": n_rows, "material_blocks": n_blocks, "frames": args.steps,
-                   "newton_iters": newton, "pcg_iters": pcg,
-                   "frames_per_s": args.steps / elapsed,
+                   "newton_iters": head["newton"], "pcg_iters": head["pcg"],
+                   "frames_per_s": args.steps / head["elapsed"],
                    "contacts_last_iter": st["num_contacts"],
                    "pairs_last_iter": st["num_pairs_vf"] + st["num_pairs_ee"]},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": traffic[0],
-                     "traffic_algorithmic_bytes": traffic[1],
-                     "kernel": "k_spmv_pcg", "bytes_per_launch": nbytes, "avg_launch_ms": spmv_ms,
-                     "contact_slots_per_launch": slots,
-                     "launches": spmv_n},
+        "roofline": {"bound": "hbm", "achieved": head["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": (head["achieved"] / HBM_PEAK_GBS) if head["achieved"] else None,
+                     "traffic": traffic[0] if precisions[0] == "fp32" else None,
+                     "traffic_algorithmic_bytes": traffic[1] if precisions[0] == "fp32" else None,
+                     "kernel": "k_spmv_pcg", "bytes_per_launch": head["nbytes"], "avg_launch_ms": head["spmv_ms"],
+                     "contact_slots_per_launch": head["slots"],
+                     "launches": head["spmv_n"]},
         "kernel_ms": {k: round(v[0], 4) for k, v in kt.items() if k != "spmv_slots"},
     }
+    if "mixed" in res and precisions[0] != "mixed":
+        m = res["mixed"]
+        line["mixed"] = {"value": m["newton"] / m["elapsed"], "unit": "Newton steps/s",
+                         "speedup_vs_fp32": (m["newton"] / m["elapsed"]) / line["value"],
+                         "ms_per_step": 1e3 * m["elapsed"] / args.steps,
+                         "newton_iters": m["newton"], "pcg_iters": m["pcg"],
+                         "spmv_avg_launch_ms": m["spmv_ms"], "spmv_bytes_per_launch": m["nbytes"],
+                         "spmv_achieved_gbs": m["achieved"],
+                         "pcg_ms": round(m["kt"].get("pcg", (float("nan"), 0))[0], 4)}
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(sc)
     print(json.dumps(line), flush=True)

@@ -88,6 +88,9 @@ typedef struct lcsx_params
     float damping_cloth;        /* velocity damping, v *= exp(-damping * dt)                 */
     float bending_scaling;      /* stiffness_bending_ui                                      */
     float newton_tol;           /* converged when max|dx| < newton_tol * dt                  */
+    int   precision;            /* 0: FP32 (the reference's path); 1: mixed precision: FP16 block
+                                   storage with per-block power-of-two scale in the PCG SpMV inside
+                                   FP32 iterative refinement, FP64 CCD, distances and barrier   */
 } lcsx_params;
 
 /* Scalars of the last Newton iteration / frame. */
@@ -106,6 +109,8 @@ typedef struct lcsx_stats
     int   num_contacts;       /* active barrier contacts                                    */
     int   converged;
     int   broadphase_overflow;/* a 32-entry traversal stack overflowed (reference: silent)  */
+    double pcg_rz_true;       /* mixed precision: r.z of the last true residual (compensated
+                                 sum), -1 when the last PCG replaced none                    */
 } lcsx_stats;
 
 /* ---------------------------------------------------------------- library */

@@ -47,14 +47,14 @@ class lcsx_params(C.Structure):
                 ("pcg_abs_tol", C.c_float), ("gravity", C.c_float * 3), ("use_floor", C.c_int),
                 ("use_self_collision", C.c_int), ("use_ccd_linesearch", C.c_int), ("floor_y", C.c_float),
                 ("stiffness_collision", C.c_float), ("stiffness_dirichlet", C.c_float), ("damping_cloth", C.c_float),
-                ("bending_scaling", C.c_float), ("newton_tol", C.c_float)]
+                ("bending_scaling", C.c_float), ("newton_tol", C.c_float), ("precision", C.c_int)]
 
 
 class lcsx_stats(C.Structure):
     _fields_ = [("frame", C.c_int), ("newton_iters", C.c_int), ("pcg_iters", C.c_int), ("pcg_iters_total", C.c_int),
                 ("toi", C.c_float), ("max_dx", C.c_float), ("pin_delta", C.c_float), ("pcg_rel", C.c_float),
                 ("num_pairs_vf", C.c_int), ("num_pairs_ee", C.c_int), ("num_contacts", C.c_int),
-                ("converged", C.c_int), ("broadphase_overflow", C.c_int)]
+                ("converged", C.c_int), ("broadphase_overflow", C.c_int), ("pcg_rz_true", C.c_double)]
 
 
 # Every exported symbol of include/lcsx.h with its ctypes signature.
@@ -135,6 +135,7 @@ def params_to_c(p: Params) -> lcsx_params:
     c.damping_cloth = p.damping_cloth
     c.bending_scaling = p.bending_scaling
     c.newton_tol = p.newton_tol
+    c.precision = {"fp32": 0, "mixed": 1}[p.precision]
     return c
 
 

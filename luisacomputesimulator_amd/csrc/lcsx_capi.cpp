@@ -295,6 +295,7 @@ void lcsx_params_default(lcsx_params* p)
     p->damping_cloth        = 2.0f;
     p->bending_scaling      = 1.0f;
     p->newton_tol           = 1e-2f;
+    p->precision            = 0;
 }
 void lcsx_material_default(lcsx_material* m)
 {
@@ -310,6 +311,7 @@ void lcsx_material_default(lcsx_material* m)
 lcsx_status lcsx_solver_create(const lcsx_params* params, int device, lcsx_solver** out)
 {
     if (!out) return arg_error("out is NULL");
+    if (params && params->precision != 0 && params->precision != 1) return arg_error("precision must be 0 (FP32) or 1 (mixed)");
     auto* h = new lcsx_solver();
     if (params) h->s.prm = *params;
     else lcsx_params_default(&h->s.prm);

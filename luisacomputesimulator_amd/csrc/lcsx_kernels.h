@@ -1,5 +1,6 @@
 // Kernel argument structs and launchers of lcsx_kernels.hip.
 #pragma once
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -32,6 +33,8 @@ struct SellView
     const float*    val;   // 9 planes of nblocks
     const float*    diag;  // 9 planes of n
     uint32_t        nblocks;
+    const __half*   val_h;  // mixed precision: 9 planes of nblocks, block scaled by 2^-exp
+    const int8_t*   val_e;  // per block exponent
 };
 
 struct ContactView
@@ -100,6 +103,7 @@ struct DcdArgs
     const float*    offset;
     const float*    dhat;
     float           kappa;
+    int             fp64;  // mixed-precision mode: distances and barrier in double
     ContactBuf      out;
 };
 
@@ -119,6 +123,7 @@ struct CcdArgs
     // need more; the second (rest = 1) runs the listed pairs with the full budget
     int             budget;
     int             rest;
+    int             fp64;  // mixed-precision mode: ACCD in double, step bound rounded down to FP32
     uint32_t*       rest_list;   // capacity pair_capacity
     uint32_t*       rest_count;
 };
@@ -186,6 +191,12 @@ void launch_pcg_contact_t(const PcgArgs& a, const ContactView& c, float* tc, uns
 void launch_pcg_spmv(const PcgArgs& a, const SellView& A, const VertContacts& vc, hipStream_t s,
                      hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
 void launch_pcg_update(const PcgArgs& a, hipStream_t s);
+// Mixed precision: FP16 copy of the off-diagonal blocks with a power-of-two scale per block.
+void launch_pack_fp16(uint32_t nblocks, const float* val, __half* val_h, int8_t* val_e, hipStream_t s);
+// Residual replacement: r = b - y, z = Minv r, per-block r.z with TwoSum compensation into
+// part_rz[block] (sum + error) and part_res[2 block] (sum, error).
+void launch_residual(uint32_t n, const float* b, const float* y, const float* minv, float* r, float* z, float* part_rz,
+                     float* part_res, hipStream_t s);
 // y = A p (diagonal + material blocks + contacts), for the spmv stage
 void launch_spmv(const SellView& A, const ContactView& c, const VertContacts& vc, unsigned contact_grid, const float* p,
                  float* y, uint32_t n, hipStream_t s);

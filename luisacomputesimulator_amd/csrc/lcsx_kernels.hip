@@ -2,6 +2,7 @@
 // narrow phase, contact assembly, SELL-64 block SpMV, block-Jacobi PCG and line search.
 // Reference counterparts are cited per kernel (paths relative to the reference root).
 #include <hip/hip_ext.h>
+#include "lcsx_fp64.h"
 #include "lcsx_kernels.h"
 #include "lcsx_physics.h"
 
@@ -276,7 +277,30 @@ __global__ __launch_bounds__(kB) void k_dcd_vf(DcdArgs a)
         f3       nrm  = mk(0.0f, 0.0f, 0.0f);
         float    k1 = 0.0f, k2 = 0.0f, ak = 0.0f;
         f3       b  = mk(0.0f, 0.0f, 0.0f);
-        if (!(vid == f0 || vid == f1 || vid == f2))
+        if (!(vid == f0 || vid == f1 || vid == f2) && a.fp64)
+        {
+            // mixed-precision mode: barycentrics, distance and barrier in double
+            using namespace fp64;
+            d3     P = wide(ld3(a.x, vid)), t0 = wide(ld3(a.x, f0)), t1 = wide(ld3(a.x, f1)), t2 = wide(ld3(a.x, f2));
+            d3     bd = pt_coeff_unclassified(P, t0, t1, t2);
+            d3     X  = (bd.x * (P - t0) + bd.y * (P - t1)) + bd.z * (P - t2);
+            double d2 = fp64::dot(X, X);
+            double dh = ((double)a.dhat[vid] + (double)a.dhat[f0]) * 0.5;
+            double th = (double)a.offset[vid] + (double)a.offset[f0];
+            b         = mk((float)bd.x, (float)bd.y, (float)bd.z);
+            if (d2 < (th + dh) * (th + dh))
+            {
+                double d  = sqrt(d2);
+                d3     nd = X / d;
+                nrm       = mk((float)nd.x, (float)nd.y, (float)nd.z);
+                double aa = 0.5 * ((double)a.area_a[vid] + (double)a.area_b[fid]);
+                k1        = (float)(((-aa) * (double)a.kappa) * fp64::barrier_d1(d - th, dh));
+                k2        = (float)((aa * (double)a.kappa) * fp64::barrier_d2(d - th, dh));
+                ak        = (float)(aa * (double)a.kappa);
+                emit      = true;
+            }
+        }
+        else if (!(vid == f0 || vid == f1 || vid == f2))
         {
             f3 P = ld3(a.x, vid), t0 = ld3(a.x, f0), t1 = ld3(a.x, f1), t2 = ld3(a.x, f2);
             b         = pt_coeff_unclassified(P, t0, t1, t2);
@@ -312,7 +336,29 @@ __global__ __launch_bounds__(kB) void k_dcd_ee(DcdArgs a)
         f3       nrm  = mk(0.0f, 0.0f, 0.0f);
         float    k1 = 0.0f, k2 = 0.0f, ak = 0.0f;
         c4       c  = {{0.0f, 0.0f, 0.0f, 0.0f}};
-        if (!(a0 == b0 || a0 == b1 || a1 == b0 || a1 == b1))
+        if (!(a0 == b0 || a0 == b1 || a1 == b0 || a1 == b1) && a.fp64)
+        {
+            using namespace fp64;
+            d3     pa0 = wide(ld3(a.x, a0)), pa1 = wide(ld3(a.x, a1)), pb0 = wide(ld3(a.x, b0)), pb1 = wide(ld3(a.x, b1));
+            dc4    cd  = ee_coeff_unclassified(pa0, pa1, pb0, pb1);
+            d3     X   = (cd.v[0] * pa0 + cd.v[1] * pa1) - (cd.v[2] * pb0 + cd.v[3] * pb1);
+            double d2  = fp64::dot(X, X);
+            double dh  = ((double)a.dhat[a0] + (double)a.dhat[b0]) * 0.5;
+            double th  = (double)a.offset[a0] + (double)a.offset[b0];
+            for (int k = 0; k < 4; k++) c.v[k] = (float)cd.v[k];
+            if (d2 < (th + dh) * (th + dh))
+            {
+                double d  = sqrt(d2);
+                d3     nd = X / d;
+                nrm       = mk((float)nd.x, (float)nd.y, (float)nd.z);
+                double aa = 0.5 * ((double)a.area_a[l] + (double)a.area_b[r]);
+                k1        = (float)(((-aa) * (double)a.kappa) * fp64::barrier_d1(d - th, dh));
+                k2        = (float)((aa * (double)a.kappa) * fp64::barrier_d2(d - th, dh));
+                ak        = (float)(aa * (double)a.kappa);
+                emit      = true;
+            }
+        }
+        else if (!(a0 == b0 || a0 == b1 || a1 == b0 || a1 == b1))
         {
             f3 pa0 = ld3(a.x, a0), pa1 = ld3(a.x, a1), pb0 = ld3(a.x, b0), pb1 = ld3(a.x, b1);
             c          = ee_coeff_unclassified(pa0, pa1, pb0, pb1);
@@ -599,6 +645,7 @@ __global__ __launch_bounds__(kB) void k_pcg_init(PcgArgs a)
 // 64-slot chunks (balanced whatever the distribution of contacts over rows), adding each row's
 // sum into the contact accumulator qc (folded into q by the update). p.q is reduced over both
 // parts in the same launch and the last block sets alpha = r.z / p.q (precond_cg.cpp:24-31).
+template <bool LP>
 __global__ __launch_bounds__(kB) void k_spmv_pcg(PcgArgs a, SellView A, VertContacts vc, uint32_t row_blocks)
 {
     __shared__ float sh[4];
@@ -634,8 +681,18 @@ __global__ __launch_bounds__(kB) void k_spmv_pcg(PcgArgs a, SellView A, VertCont
                 uint32_t cl = A.col[b];
                 f3       pc = ld3(a.z, cl) + beta * ld3(a.p_old, cl);
                 m3       M;
+                if (LP)
+                {
+                    // FP16 entries times the block's power of two (exact)
+                    float sc = __int_as_float((127 + (int)A.val_e[b]) << 23);
 #pragma unroll
-                for (int cc = 0; cc < 9; cc++) M.m[cc] = A.val[cc * (size_t)A.nblocks + b];
+                    for (int cc = 0; cc < 9; cc++) M.m[cc] = __half2float(A.val_h[cc * (size_t)A.nblocks + b]) * sc;
+                }
+                else
+                {
+#pragma unroll
+                    for (int cc = 0; cc < 9; cc++) M.m[cc] = A.val[cc * (size_t)A.nblocks + b];
+                }
                 y = y + mul(M, pc);
             }
             st3(a.q, i, y);
@@ -656,6 +713,84 @@ __global__ __launch_bounds__(kB) void k_spmv_pcg(PcgArgs a, SellView A, VertCont
     }
     float bs = block_sum(loc, sh);
     if (threadIdx.x == 0) a.part_pq[blockIdx.x] = bs;
+}
+
+// Mixed precision: FP16 copy of the off-diagonal blocks, each block scaled by 2^-e with 2^e the
+// smallest power of two above its largest magnitude (entries in (-1, 1), 11 significant bits).
+__global__ __launch_bounds__(kB) void k_pack_fp16(uint32_t nb, const float* __restrict__ val, __half* __restrict__ vh,
+                                                   int8_t* __restrict__ ve)
+{
+    uint32_t b = blockIdx.x * kB + threadIdx.x;
+    if (b >= nb) return;
+    float v[9], m = 0.0f;
+#pragma unroll
+    for (int c = 0; c < 9; c++)
+    {
+        v[c] = val[c * (size_t)nb + b];
+        m    = fmaxf(m, fabsf(v[c]));
+    }
+    int e = 0;
+    if (m > 0.0f) frexpf(m, &e);  // m = f 2^e, f in [0.5, 1)
+    e        = max(-100, min(100, e));
+    float sc = __int_as_float((127 - e) << 23);
+#pragma unroll
+    for (int c = 0; c < 9; c++) vh[c * (size_t)nb + b] = __float2half_rn(v[c] * sc);
+    ve[b] = (int8_t)e;
+}
+
+// Knuth's TwoSum: s + e = a + b exactly.
+LCSX_D void two_sum(float a, float b, float& s, float& e)
+{
+    s        = a + b;
+    float bb = s - a;
+    e        = (a - (s - bb)) + (b - bb);
+}
+
+// Residual replacement of the mixed-precision PCG: r = b - y with y = A x from the FP32 blocks (the
+// true residual), z = Minv r, and the block's r.z summed pairwise with TwoSum, the rounding errors
+// carried alongside: part_rz[block] = sum + error (read by the next iteration), part_res[2 block]
+// = (sum, error) for the host's double total.
+__global__ __launch_bounds__(kB) void k_residual(uint32_t n, const float* __restrict__ b, const float* __restrict__ y,
+                                                  const float* __restrict__ minv, float* __restrict__ r,
+                                                  float* __restrict__ z, float* __restrict__ part_rz,
+                                                  float* __restrict__ part_res)
+{
+    __shared__ float ss[kB], se[kB];
+    uint32_t         i = blockIdx.x * kB + threadIdx.x;
+    float            s = 0.0f, e = 0.0f;
+    if (i < n)
+    {
+        f3 ri = ld3(b, i) - ld3(y, i);
+        st3(r, i, ri);
+        f3 zi = mul(ld_planes(minv, n, i), ri);
+        st3(z, i, zi);
+        float p0 = ri.x * zi.x, p1 = ri.y * zi.y, p2 = ri.z * zi.z;
+        float t, u;
+        two_sum(p0, p1, s, e);
+        two_sum(s, p2, t, u);
+        s = t;
+        e = e + u;
+    }
+    ss[threadIdx.x] = s;
+    se[threadIdx.x] = e;
+    __syncthreads();
+    for (uint32_t h = kB / 2; h > 0; h >>= 1)
+    {
+        if (threadIdx.x < h)
+        {
+            float t, u;
+            two_sum(ss[threadIdx.x], ss[threadIdx.x + h], t, u);
+            ss[threadIdx.x] = t;
+            se[threadIdx.x] = (se[threadIdx.x] + se[threadIdx.x + h]) + u;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+    {
+        part_rz[blockIdx.x]          = ss[0] + se[0];
+        part_res[2 * blockIdx.x]     = ss[0];
+        part_res[2 * blockIdx.x + 1] = se[0];
+    }
 }
 
 // y = A p: rows (diagonal and material blocks) in one launch, then the contact slots added to y.
@@ -771,7 +906,8 @@ __global__ __launch_bounds__(kB) void k_ccd(CcdArgs a)
 {
     __shared__ float sh[4];
     uint32_t         n    = a.rest ? min(*a.rest_count, a.pair_capacity) : min(*a.pair_count, a.pair_capacity);
-    float            tmin = kLineSearchMaxT;
+    float            tmin = kLineSearchMaxT;  // FP32 mode: smallest raw toi
+    float            bmin = 1.0f;             // FP64 mode: smallest step bound toi / 1.25
     for (uint32_t idx = blockIdx.x * kB + threadIdx.x; idx < n; idx += gridDim.x * kB)
     {
         uint32_t p = a.rest ? a.rest_list[idx] : idx;
@@ -801,19 +937,38 @@ __global__ __launch_bounds__(kB) void k_ccd(CcdArgs a)
         }
         float off  = a.offset[v[0]] + a.offset[a.kind == 0 ? v[1] : v[2]];
         bool  done = true;
-        float t    = accd_budget(x0, x1, off, a.kind, a.rest ? 0x7fffffff : a.budget, &done);
-        if (done) tmin = fminf(tmin, t);
+        if (a.fp64)
+        {
+            // double ACCD on the FP32 positions; its step bound toi / 1.25 rounded toward zero, so
+            // the FP32 bound is never later than the double one
+            fp64::d3 d0[4], d1[4];
+            for (int k = 0; k < 4; k++)
+            {
+                d0[k] = fp64::wide(x0[k]);
+                d1[k] = fp64::wide(x1[k]);
+            }
+            double t = fp64::accd_budget(d0, d1, (double)a.offset[v[0]] + (double)a.offset[a.kind == 0 ? v[1] : v[2]],
+                                         a.kind, a.rest ? 0x7fffffff : a.budget, &done);
+            if (done) bmin = fminf(bmin, __double2float_rd(t / fp64::kMaxT));
+        }
+        else
+        {
+            float t = accd_budget(x0, x1, off, a.kind, a.rest ? 0x7fffffff : a.budget, &done);
+            if (done) tmin = fminf(tmin, t);
+        }
         uint32_t slot = wave_append(a.rest_count, !done);
         if (!done && slot < a.pair_capacity) a.rest_list[slot] = p;
     }
-    tmin  = wave_min(tmin);
-    int w = threadIdx.x >> 6;
-    if (lane_id() == 0) sh[w] = tmin;
+    // FP32: toi / 1.25 of the smallest raw toi (division by a positive constant keeps the order)
+    float bound = a.fp64 ? bmin : (tmin != kLineSearchMaxT ? tmin / kLineSearchMaxT : 1.0f);
+    bound       = wave_min(bound);
+    int w       = threadIdx.x >> 6;
+    if (lane_id() == 0) sh[w] = bound;
     __syncthreads();
     if (threadIdx.x == 0)
     {
         float t = fminf(fminf(sh[0], sh[1]), fminf(sh[2], sh[3]));
-        if (t != kLineSearchMaxT) atomicMin(a.toi_bits, f2ord(t / kLineSearchMaxT));
+        if (t < 1.0f) atomicMin(a.toi_bits, f2ord(t));
     }
 }
 
@@ -915,10 +1070,26 @@ void launch_pcg_spmv(const PcgArgs& a, const SellView& A, const VertContacts& vc
 {
     uint32_t rb = grid_for(a.n, kB);
     uint32_t g  = rb + (vc.ptr ? grid_for(vc.nslots, kB) : 0u);
-    if (start)
-        hipExtLaunchKernelGGL(k_spmv_pcg, dim3(g), dim3(kB), 0, s, start, stop, 0, a, A, vc, rb);
+    if (A.val_h)
+    {
+        if (start)
+            hipExtLaunchKernelGGL(k_spmv_pcg<true>, dim3(g), dim3(kB), 0, s, start, stop, 0, a, A, vc, rb);
+        else
+            k_spmv_pcg<true><<<g, kB, 0, s>>>(a, A, vc, rb);
+    }
+    else if (start)
+        hipExtLaunchKernelGGL(k_spmv_pcg<false>, dim3(g), dim3(kB), 0, s, start, stop, 0, a, A, vc, rb);
     else
-        k_spmv_pcg<<<g, kB, 0, s>>>(a, A, vc, rb);
+        k_spmv_pcg<false><<<g, kB, 0, s>>>(a, A, vc, rb);
+}
+void launch_pack_fp16(uint32_t nblocks, const float* val, __half* val_h, int8_t* val_e, hipStream_t s)
+{
+    if (nblocks) k_pack_fp16<<<grid_for(nblocks, kB), kB, 0, s>>>(nblocks, val, val_h, val_e);
+}
+void launch_residual(uint32_t n, const float* b, const float* y, const float* minv, float* r, float* z, float* part_rz,
+                     float* part_res, hipStream_t s)
+{
+    k_residual<<<grid_for(n, kB), kB, 0, s>>>(n, b, y, minv, r, z, part_rz, part_res);
 }
 void launch_pcg_update(const PcgArgs& a, hipStream_t s) { k_pcg_update<<<grid_for(a.n, kB), kB, 0, s>>>(a); }
 void launch_spmv(const SellView& A, const ContactView& c, const VertContacts& vc, unsigned contact_grid, const float* p,

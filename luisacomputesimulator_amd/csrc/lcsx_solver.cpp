@@ -16,6 +16,7 @@ constexpr unsigned kPairGridMax = 4096;
 // second pass over a compact list, so long iterations do not hold whole wavefronts
 constexpr int      kCcdFirstPassIters = 32;
 constexpr int      kPcgChunk    = 10;
+constexpr int      kMixedSweep  = 25;  // PCG iterations between residual replacements (mixed precision)
 
 struct EventPool
 {
@@ -152,6 +153,12 @@ void Solver::init()
     sell_val.alloc(9 * (size_t)std::max(S.nblocks, 1u));
     vc_qc.alloc(3 * (size_t)nv);
     part_rz.alloc(grid_for(nv, 256) + 1);
+    if (prm.precision == 1)
+    {
+        sell_h.alloc(9 * (size_t)std::max(S.nblocks, 1u));
+        sell_e.alloc(std::max(S.nblocks, 1u));
+        part_res.alloc(2 * (size_t)grid_for(nv, 256));
+    }
     upload_state_constants();
     tree_face.init(3, faces.p, S.nf);
     tree_edge.init(2, edges.p, S.ne);
@@ -234,7 +241,11 @@ VertContacts Solver::vert_contacts()
     return VertContacts{vc_ptr.p, vc_list.p, vc_row.p, vc_coef.p, vc_tc.p, 4u * nc};
 }
 // The SpMV's diagonal is the material one: contact diagonal blocks enter through the contact slots.
-SellView Solver::sell_view() const { return SellView{sl_off.p, sell_col.p, sell_val.p, diag_mat.p, S.nblocks}; }
+SellView Solver::sell_view() const { return SellView{sl_off.p, sell_col.p, sell_val.p, diag_mat.p, S.nblocks, nullptr, nullptr}; }
+SellView Solver::sell_view_lp() const
+{
+    return SellView{sl_off.p, sell_col.p, sell_val.p, diag_mat.p, S.nblocks, sell_h.p, sell_e.p};
+}
 
 // ------------------------------------------------------------------ stages
 void Solver::frame_begin()
@@ -306,6 +317,7 @@ void Solver::material()
     a.diag_mat = diag_mat.p;
     launch_vert_assemble(a, stream);
     launch_offdiag(S.nblocks, blk_ptr.p, blk_contrib.p, face_stage.p, hinge_stage.p, sell_val.p, stream);
+    if (prm.precision == 1) launch_pack_fp16(S.nblocks, sell_val.p, sell_h.p, sell_e.p, stream);
     toc("material");
 }
 
@@ -363,6 +375,7 @@ void Solver::dcd()
     a.offset = offset.p;
     a.dhat   = dhat.p;
     a.kappa  = prm.stiffness_collision;
+    a.fp64   = prm.precision == 1;
     a.out    = out;
     a.pairs = pairs_vf.p;
     a.pair_count = &status.p->n_vf;
@@ -400,40 +413,19 @@ void Solver::contact_assemble()
     toc("contact_assemble");
 }
 
-void Solver::pcg()
+int Solver::pcg_run(PcgArgs& a, const SellView& A, const ContactView& cv, const VertContacts& vc, int max_iters,
+                    int replace_every, PcgState& hs)
 {
-    tic();
-    PcgArgs a{};
-    a.n        = S.nv;
-    a.b        = cgB.p;
-    a.diag     = diag.p;
-    a.minv     = minv.p;
-    a.x        = dx.p;
-    a.r        = r.p;
-    a.z        = z.p;
-    a.p_old    = p[0].p;
-    a.p_new    = p[1].p;
-    a.q        = q.p;
-    a.part_rz  = part_rz.p;
-    a.part_pq  = part_pq.p;
-    a.s        = pcg_state.p;
-    a.rel2     = prm.pcg_rel_tol * prm.pcg_rel_tol;
-    a.abs_tol  = prm.pcg_abs_tol;
-    a.check_every = (uint32_t)std::max(1, prm.pcg_check_every);
-    SellView     A  = sell_view();
-    ContactView  cv = contact_view();
-    VertContacts vc = vert_contacts();
-    a.qc            = vc.ptr ? vc_qc.p : nullptr;
-    a.n_rz          = grid_for(S.nv, 256);                                   // init / update blocks
-    a.n_pq          = a.n_rz + (vc.ptr ? grid_for(vc.nslots, 256) : 0u);    // SpMV row + slot blocks
     launch_pcg_init(a, stream);
-    PcgState     hs{};
-    int         issued = 0;
-    int         par    = 0;
-    while (issued < prm.pcg_iter_count)
+    hs         = PcgState{};
+    int issued = 0;
+    int par    = 0;
+    while (issued < max_iters)
     {
-        // one host check per convergence-test interval (at least kPcgChunk launches)
-        int chunk = std::min(std::max(kPcgChunk, a.check_every > 1 ? (int)a.check_every : 0), prm.pcg_iter_count - issued);
+        // one host check per convergence-test interval (at least kPcgChunk launches), chunks
+        // ending at the residual replacements
+        int chunk = std::min(std::max(kPcgChunk, a.check_every > 1 ? (int)a.check_every : 0), max_iters - issued);
+        if (replace_every > 0) chunk = std::min(chunk, replace_every - issued % replace_every);
         for (int k = 0; k < chunk; k++)
         {
             PcgArgs b = a;
@@ -454,21 +446,53 @@ void Solver::pcg()
             par = 1 - par;
         }
         issued += chunk;
+        const bool replace = replace_every > 0 && issued % replace_every == 0 && issued < max_iters;
+        if (replace)
+        {
+            // r = b - A x with the FP32 blocks; the next iteration's r.z (and its stopping test)
+            // is this residual's. After convergence x no longer changes and this is harmless.
+            launch_spmv(sell_view(), cv, vc, pair_grid(cap_c), a.x, spmv_tmp.p, S.nv, stream);
+            launch_residual(S.nv, a.b, spmv_tmp.p, a.minv, a.r, a.z, a.part_rz, part_res.p, stream);
+        }
         LCSX_HIP(hipMemcpyAsync(&hs, pcg_state.p, sizeof(PcgState), hipMemcpyDeviceToHost, stream));
+        std::vector<float> res;
+        if (replace)
+        {
+            res.resize(part_res.n);
+            LCSX_HIP(hipMemcpyAsync(res.data(), part_res.p, res.size() * sizeof(float), hipMemcpyDeviceToHost, stream));
+        }
         LCSX_HIP(hipStreamSynchronize(stream));
         flush_events();
-        if (hs.converged) break;
+        if (!hs.converged && !res.empty())
+        {
+            double t = 0.0;
+            for (size_t k = 0; k < res.size(); k += 2) t += (double)res[k] + (double)res[k + 1];
+            rz_true = t;
+        }
+        if (hs.converged || hs.nan) break;
     }
-    toc("pcg");
+    if (hs.nan)
+    {
+        set_error("NaN/Inf in the PCG iteration");
+        throw HipFailure();
+    }
     // iterations run: the stopping one when the test stopped the loop, else the cap
     const uint32_t iters = hs.converged ? hs.iters : (uint32_t)issued;
-    // only launches that did work (the first `iters`; later ones exit on the converged flag)
+    account_spmv(iters, vc.nslots);
+    // r.z of the last evaluated iteration kept in hs.rz[0]
+    hs.rz[0] = hs.rz[(hs.converged ? hs.iters : (uint32_t)std::max(issued - 1, 0)) & 1u];
+    return (int)iters;
+}
+
+// Timed SpMV launches: only the first `iters` did work (later ones exit on the converged flag).
+void Solver::account_spmv(uint32_t iters, uint32_t nslots)
+{
     if (!spmv_events.empty())
     {
         // contact slots the timed launches processed ("spmv_slots": slots per launch)
         uint32_t counted = std::min<uint32_t>(iters, (uint32_t)spmv_events.size());
         auto&    t       = times["spmv_slots"];
-        t.first += (double)vc.nslots * counted;
+        t.first += (double)nslots * counted;
         t.second += (int)counted;
     }
     for (size_t k = 0; k < spmv_events.size(); k++)
@@ -484,16 +508,51 @@ void Solver::pcg()
         ev_put(spmv_events[k].second);
     }
     spmv_events.clear();
-    stats.pcg_iters = (int)iters;
-    stats.pcg_iters_total += (int)iters;
-    // r.z of the last evaluated iteration (the stopping one, or the last of the cap)
-    const float rz_last = hs.rz[(hs.converged ? hs.iters : (uint32_t)std::max(issued - 1, 0)) & 1u];
-    stats.pcg_rel   = hs.rz0 > 0.0f ? std::sqrt(rz_last / hs.rz0) : 0.0f;
-    if (hs.nan)
-    {
-        set_error("NaN/Inf in the PCG iteration");
-        throw HipFailure();
-    }
+}
+
+// ConjugateGradientSolver (pcg_solver.cpp:15-197), with the stopping test of the reference on
+// r.z. FP32: every block in FP32. Mixed precision: the SpMV inside the iteration reads FP16
+// blocks (half the bytes); x, r, z, p stay FP32, and after every kMixedSweep iterations the
+// recursive residual is replaced by the true residual b - A x computed with the FP32 blocks and
+// a compensated r.z (residual replacement: the iteration keeps its search direction, so the
+// Krylov progress made so far is not thrown away as a restarted refinement would).
+void Solver::pcg()
+{
+    tic();
+    PcgArgs a{};
+    a.n        = S.nv;
+    a.b        = cgB.p;
+    a.diag     = diag.p;
+    a.minv     = minv.p;
+    a.x        = dx.p;
+    a.r        = r.p;
+    a.z        = z.p;
+    a.p_old    = p[0].p;  // zeroed by the init (the first iteration reads it)
+    a.p_new    = p[1].p;
+    a.q        = q.p;
+    a.part_rz  = part_rz.p;
+    a.part_pq  = part_pq.p;
+    a.s        = pcg_state.p;
+    a.rel2     = prm.pcg_rel_tol * prm.pcg_rel_tol;
+    a.abs_tol  = prm.pcg_abs_tol;
+    a.check_every = (uint32_t)std::max(1, prm.pcg_check_every);
+    ContactView  cv = contact_view();
+    VertContacts vc = vert_contacts();
+    a.qc            = vc.ptr ? vc_qc.p : nullptr;
+    a.n_rz          = grid_for(S.nv, 256);                                   // init / update blocks
+    a.n_pq          = a.n_rz + (vc.ptr ? grid_for(vc.nslots, 256) : 0u);    // SpMV row + slot blocks
+    PcgState   hs{};
+    const bool mixed = prm.precision == 1;
+    // replacements at multiples of the test interval, so the test after one sees the true residual
+    const int every = (int)a.check_every;
+    const int repl  = mixed ? every * ((kMixedSweep + every - 1) / every) : 0;
+    rz_true         = -1.0;
+    const int iters = pcg_run(a, mixed ? sell_view_lp() : sell_view(), cv, vc, prm.pcg_iter_count, repl, hs);
+    toc("pcg");
+    stats.pcg_iters = iters;
+    stats.pcg_iters_total += iters;
+    stats.pcg_rel   = hs.rz0 > 0.0f ? std::sqrt(hs.rz[0] / hs.rz0) : 0.0f;
+    stats.pcg_rz_true = rz_true;
 }
 
 void Solver::ccd()
@@ -519,6 +578,7 @@ void Solver::ccd()
         c.edges    = edges.p;
         c.offset   = offset.p;
         c.toi_bits = &status.p->toi_bits;
+        c.fp64     = prm.precision == 1;
         c.budget   = kCcdFirstPassIters;
         launch_fill_u32(ccd_rest_count.p, 2, 0u, stream);
         for (int rest = 0; rest < 2; rest++)

@@ -28,6 +28,12 @@ struct Solver
     DBuf<float> x, v, x_step, x_iter, x_tilde, dx, cgB, r, z, p[2], q, spmv_tmp;
     DBuf<float> diag, diag_mat, minv, sell_val;  // 9 planes (diag: with contact blocks; diag_mat: material only)
     DBuf<float> part_rz, part_pq;  // PCG per-block partial sums
+    // mixed precision: FP16 off-diagonal blocks with a per-block exponent, and the compensated
+    // (sum, error) partials of the replaced residual's r.z
+    DBuf<__half> sell_h;
+    DBuf<int8_t> sell_e;
+    DBuf<float>  part_res;
+    double       rz_true = 0.0;  // r.z of the last replaced residual (compensated, summed in double)
     // constants
     DBuf<float>    rest_x, offset, dhat, vert_mass, vert_area, face_area, edge_area, targets;
     DBuf<uint8_t>  is_fixed;
@@ -85,6 +91,13 @@ struct Solver
 
     ContactView contact_view() const;
     SellView    sell_view() const;
+    SellView    sell_view_lp() const;  // FP16 blocks (mixed precision)
+    // PCG iterations on a.b from x = 0, at most max_iters; with replace_every > 0 the recursive
+    // residual is replaced by the true one (FP32 blocks) after every replace_every iterations.
+    // Returns the iterations run.
+    int         pcg_run(PcgArgs& a, const SellView& A, const ContactView& cv, const VertContacts& vc, int max_iters,
+                        int replace_every, PcgState& hs);
+    void        account_spmv(uint32_t iters, uint32_t nslots);
     VertContacts vert_contacts();
     unsigned    pair_grid(uint32_t cap) const;
 

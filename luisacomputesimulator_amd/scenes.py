@@ -100,6 +100,9 @@ class Params:
     damping_cloth: float = 2.0
     bending_scaling: float = 1.0
     newton_tol: float = 1e-2
+    # "fp32" (the reference's path) or "mixed" (FP16 blocks + FP32 refinement, FP64 CCD and barrier);
+    # not part of pack(): the oracle runs FP32 or its FP64 twin
+    precision: str = "fp32"
 
     def pack(self) -> List[float]:
         return [self.implicit_dt, self.nonlinear_iter_count, self.newton_max_iters, self.pcg_iter_count,

@@ -8,12 +8,24 @@
 #include <random>
 #include <sstream>
 #include <string>
+#include "../../luisacomputesimulator_amd/csrc/lcsx_fp64.h"
 #include "../../luisacomputesimulator_amd/csrc/lcsx_physics.h"
 #include "../../oracle/orc_collision.h"
 #include "../../oracle/orc_energy.h"
 #include "../../oracle/orc_lbvh.h"
 
 static int g_fail = 0;
+static void same64(double a, double b, const char* what, int k)
+{
+    uint64_t ua, ub;
+    std::memcpy(&ua, &a, 8);
+    std::memcpy(&ub, &b, 8);
+    if (ua != ub && !(a != a && b != b))
+    {
+        if (g_fail < 10) std::printf("MISMATCH %s case %d: %.17g vs %.17g\n", what, k, a, b);
+        g_fail++;
+    }
+}
 static void same(float a, float b, const char* what, int k)
 {
     uint32_t ua, ub;
@@ -68,6 +80,18 @@ static int stall_cases(const char* path)
             g_fail++;
         }
         same(lcsx::accd(a, b, v[0], kind), ref, "accd_stall", n);
+        {
+            lcsx::fp64::d3  da[4], db[4];
+            orc::V3<double> qa[4], qb[4];
+            for (int k = 0; k < 4; k++)
+            {
+                da[k] = lcsx::fp64::wide(a[k]);
+                db[k] = lcsx::fp64::wide(b[k]);
+                qa[k] = orc::V3<double>(da[k].x, da[k].y, da[k].z);
+                qb[k] = orc::V3<double>(db[k].x, db[k].y, db[k].z);
+            }
+            same64(lcsx::fp64::accd(da, db, (double)v[0], kind), orc::accd(qa, qb, (double)v[0], kind), "accd64_stall", n);
+        }
         bool  done = true;
         float part = lcsx::accd_budget(a, b, v[0], kind, 32, &done);
         if (done && part != ref)
@@ -102,6 +126,36 @@ int main(int argc, char** argv)
         for (int i = 0; i < 4; i++) same(e.v[i], oe.v[i], "ee_coeff", k);
         same(lcsx::pt_dist2(p[0], p[1], p[2], p[3]), orc::pt_dist2(ov(p[0]), ov(p[1]), ov(p[2]), ov(p[3])), "pt_dist2", k);
         same(lcsx::ee_dist2(p[0], p[1], p[2], p[3]), orc::ee_dist2(ov(p[0]), ov(p[1]), ov(p[2]), ov(p[3])), "ee_dist2", k);
+        // FP64 path (mixed-precision mode) against the oracle's double instantiation
+        {
+            lcsx::fp64::d3  d[4];
+            orc::V3<double> q[4];
+            for (int i = 0; i < 4; i++)
+            {
+                d[i] = lcsx::fp64::wide(p[i]);
+                q[i] = orc::V3<double>(d[i].x, d[i].y, d[i].z);
+            }
+            same64(lcsx::fp64::pt_dist2(d[0], d[1], d[2], d[3]), orc::pt_dist2(q[0], q[1], q[2], q[3]), "pt_dist2_64", k);
+            same64(lcsx::fp64::ee_dist2(d[0], d[1], d[2], d[3]), orc::ee_dist2(q[0], q[1], q[2], q[3]), "ee_dist2_64", k);
+            double dd = 1e-4 + 1e-3 * (0.5 + 0.5 * (double)U(rng)), dh = 1e-3;
+            same64(lcsx::fp64::barrier_d1(dd, dh), orc::barrier_d1(dd, dh), "barrier_d1_64", k);
+            same64(lcsx::fp64::barrier_d2(dd, dh), orc::barrier_d2(dd, dh), "barrier_d2_64", k);
+            if (k < 2000)
+            {
+                lcsx::fp64::d3  e[4];
+                orc::V3<double> r[4];
+                for (int i = 0; i < 4; i++)
+                {
+                    e[i] = lcsx::fp64::wide(p[i] + 0.5f * v());
+                    r[i] = orc::V3<double>(e[i].x, e[i].y, e[i].z);
+                }
+                for (int kind = 0; kind < 2; kind++)
+                {
+                    double off = (double)(0.01f * (1.0f + U(rng)));
+                    same64(lcsx::fp64::accd(d, e, off, kind), orc::accd(q, r, off, kind), kind ? "accd64_ee" : "accd64_vf", k);
+                }
+            }
+        }
         // ACCD on a displaced copy
         if (k < 4000)
         {

@@ -33,3 +33,22 @@ def test_error_paths_without_gpu():
         s.register_mesh("bad", [[0, 0, 0], [1, 0, 0], [0, 1, 0]], [[0, 1, 5]])  # index out of range
     with pytest.raises(LcsxError):
         s.init_solver()  # no meshes
+
+
+def test_precision_parameter_is_validated():
+    import ctypes as C
+    from luisacomputesimulator_amd import LIB_PATH, lcsx_params, _SIGS
+    from luisacomputesimulator_amd import NewtonSolver, scenes
+    lib = C.CDLL(LIB_PATH)
+    for n, (res, args) in _SIGS.items():
+        getattr(lib, n).restype, getattr(lib, n).argtypes = res, args
+    p = lcsx_params()
+    lib.lcsx_params_default(C.byref(p))
+    assert p.precision == 0
+    p.precision = 2
+    h = C.c_void_p()
+    assert lib.lcsx_solver_create(C.byref(p), 0, C.byref(h)) != 0
+    assert b"precision" in lib.lcsx_last_error()
+    NewtonSolver(scenes.Params(precision="mixed"))  # accepted
+    with pytest.raises(KeyError):
+        NewtonSolver(scenes.Params(precision="fp16"))

@@ -1,0 +1,164 @@
+"""GPU parity of the mixed-precision form against the FP64 oracle.
+
+Mixed precision (Params.precision = "mixed"): the PCG's SpMV reads FP16 off-diagonal blocks with a
+power-of-two scale per block, x/r/z/p stay FP32 and the recursive residual is replaced every 25
+iterations by the true residual (FP32 blocks, compensated r.z); DCD distances and barrier
+derivatives and the ACCD time of impact are computed in FP64 and rounded once. Each test states the
+error against the FP64 twin of the oracle and the bar it is held to.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from luisacomputesimulator_amd import NewtonSolver, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def mixed(sc):
+    sc.params.precision = "mixed"
+    print(f"[gpu-mixed] scene={sc.name} verts={sc.num_verts}", flush=True)
+    return NewtonSolver.from_scene(sc)
+
+
+def f32_round_down(t: float) -> np.float32:
+    f = np.float32(t)
+    return np.nextafter(f, np.float32(-np.inf)) if float(f) > t else f
+
+
+def contact_key(idx, typ):
+    idx = idx.reshape(-1, 4)
+    k = np.concatenate([typ.reshape(-1, 1), idx], 1)
+    order = np.lexsort(k.T[::-1])
+    return k[order], order
+
+
+def test_mixed_ccd_bound_is_fp64_rounded_down(require_gpu):
+    """ACCD in FP64 on the FP32 positions: the step bound equals the FP64 oracle's, rounded toward
+    zero to FP32 (so it is never later than the double one)."""
+    sc = scenes.small_two_sheets(9, gap=1.6e-3)
+    g, o64 = mixed(sc), oracle.Oracle(sc, 64)
+    for s in ("frame_begin", "predict", "iter_begin"):
+        g.stage(s)
+        o64.stage(s)
+    x0 = o64.get("x").reshape(-1, 3).astype(np.float32)
+    x1 = x0.copy()
+    x1[81:, 1] -= np.float32(4e-3)  # the upper sheet moves through the lower one
+    g.write("x_iter", x0)
+    g.write("x", x1)
+    o64.set("x_iter", x0)
+    o64.set("x", x1)
+    for s in ("broad_phase_ccd", "ccd"):
+        g.stage(s)
+        o64.stage(s)
+    t64 = o64.scalars()["toi"]
+    tg = np.float32(g.read("toi")[0])
+    print(f"[gpu-mixed] CCD bound gpu {float(tg)!r} fp64 {t64!r}", flush=True)
+    assert 0.0 < t64 < 1.0
+    assert tg == f32_round_down(t64)
+
+
+def test_mixed_dcd_contact_terms_vs_fp64(require_gpu):
+    """Active set identical to the FP64 oracle's; barrier stiffnesses, normals and weights within
+    2e-6 relative of it (one FP32 rounding of FP64 values, plus FP32 products for the normals)."""
+    sc = scenes.small_two_sheets(9, gap=1.6e-3)
+    g, o64 = mixed(sc), oracle.Oracle(sc, 64)
+    for s in ("frame_begin", "predict", "iter_begin"):
+        g.stage(s)
+        o64.stage(s)
+    x = o64.get("x").reshape(-1, 3).astype(np.float32)
+    x[81:] += np.float32(-4e-4) * np.array([0, 1, 0], dtype=np.float32)
+    g.write("x", x)
+    o64.set("x", x)
+    g.stage("update_contacts")
+    o64.stage("update_contacts")
+    kg, pg = contact_key(g.read("contact_idx"), g.read("contact_type"))
+    ko, po = contact_key(o64.get("contact_idx"), o64.get("contact_type"))
+    assert len(ko) > 50
+    assert np.array_equal(kg, ko)
+    for name, width in (("contact_k", 3), ("contact_n", 3), ("contact_w", 4)):
+        a = g.read(name).reshape(-1, width)[pg].astype(np.float64)
+        b = o64.get(name).reshape(-1, width)[po].astype(np.float64)
+        err = np.abs(a - b).max() / np.abs(b).max()
+        print(f"[gpu-mixed] {name} max rel err vs FP64 {err:.3e}", flush=True)
+        assert err < 2e-6, name
+
+
+def _system(sc, o64):
+    g = mixed(sc)
+    for s in ("frame_begin", "predict", "iter_begin", "material", "update_contacts", "contact_assemble"):
+        g.stage(s)
+        o64.stage(s)
+    return g
+
+
+def test_mixed_pcg_solution_vs_fp64(require_gpu):
+    """PCG on the square cloth's first Newton system (relative stopping test 1e-4): the mixed
+    solution's residual under the FP64 operator is within 2e-3 of |b| and the solution within
+    2e-3 of the FP64 oracle's (the bars of the FP32 path)."""
+    sc = scenes.square_cloth(33)
+    o64 = oracle.Oracle(sc, 64)
+    g = _system(sc, o64)
+    g.stage("pcg")
+    o64.stage("pcg")
+    st = g.stats()
+    dxg, dx64 = g.read("dx").astype(np.float64), o64.get("dx").astype(np.float64)
+    b = o64.get("cgB").astype(np.float64)
+    o64.set("dx", dxg)
+    o64.stage("spmv_dx")
+    res = np.linalg.norm(b - o64.get("dx")) / np.linalg.norm(b)
+    err = np.linalg.norm(dxg - dx64) / np.linalg.norm(dx64)
+    print(f"[gpu-mixed] square33 PCG: iters {st['pcg_iters']} rz_true {st['pcg_rz_true']:.3e} "
+          f"residual {res:.3e} error vs FP64 {err:.3e}", flush=True)
+    assert res <= 2e-3
+    assert err <= 2e-3
+
+
+def test_mixed_newton_iteration_in_contact_vs_fp64(require_gpu):
+    """One Newton iteration of the small tube in barrier contact (state after three FP64 oracle
+    frames). PCG stops at its 100-iteration cap here, so every precision differs from FP64 by its
+    truncated Krylov path: the FP32 oracle by ~3e-4. Bar: the mixed error is at most three times
+    the FP32 oracle's, plus 1e-6."""
+    sc = scenes.cylinder_tube(rings=41, per_ring=48, cyl_segments=24, cyl_rings=9)
+    o, o64 = oracle.Oracle(sc, 32), oracle.Oracle(sc, 64)
+    g = mixed(sc)
+    for _ in range(3):
+        o.step()
+        o64.step()
+        g.physics_step_gpu(1)
+    x0, v0 = o.get("x_outer"), o.get("v_outer")
+    g.set_state(x0, v0)
+    o64.set("x_outer", x0)
+    o64.set("v_outer", v0)
+    for s in ("frame_begin", "predict", "newton_iteration"):
+        g.stage(s)
+        o.stage(s)
+        o64.stage(s)
+    st = g.stats()
+    x64 = o64.get("x").reshape(-1, 3)
+    scale = np.abs(x64).max()
+    err_g = np.abs(g.read("x").reshape(-1, 3) - x64).max() / scale
+    err_o = np.abs(o.get("x").reshape(-1, 3) - x64).max() / scale
+    print(f"[gpu-mixed] contact iteration error vs FP64: mixed {err_g:.3e}, FP32 oracle {err_o:.3e}, {st}", flush=True)
+    assert st["num_contacts"] > 0
+    assert err_g <= 3.0 * err_o + 1e-6
+
+
+@pytest.mark.parametrize("name,frames", [("square33", 6), ("tube_small", 2)])
+def test_mixed_frames_vs_fp64(require_gpu, name, frames):
+    """Whole frames in mixed precision against the FP64 oracle, the FP32 oracle's own error beside
+    it. Bar per frame: at most three times the FP32 oracle's error, plus 1e-5."""
+    sc = {"square33": lambda: scenes.square_cloth(33),
+          "tube_small": lambda: scenes.cylinder_tube(rings=41, per_ring=48, cyl_segments=24, cyl_rings=9)}[name]()
+    o, o64 = oracle.Oracle(sc, 32), oracle.Oracle(sc, 64)
+    g = mixed(sc)
+    for f in range(frames):
+        g.physics_step_gpu(1)
+        o.step()
+        o64.step()
+        x64 = o64.get("x").reshape(-1, 3)
+        scale = np.abs(x64).max()
+        err_g = np.abs(g.get_simulation_results() - x64).max() / scale
+        err_o = np.abs(o.get("x").reshape(-1, 3) - x64).max() / scale
+        print(f"[gpu-mixed] {name} frame {f}: mixed {err_g:.3e}, FP32 oracle {err_o:.3e}, {g.stats()}", flush=True)
+        assert err_g <= 3.0 * err_o + 1e-5
