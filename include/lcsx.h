@@ -157,6 +157,22 @@ lcsx_status lcsx_solver_set_pinned_targets(lcsx_solver* s, int mesh_id, const fl
 lcsx_status lcsx_solver_get_stats(lcsx_solver* s, lcsx_stats* out);
 int         lcsx_solver_num_verts(lcsx_solver* s);
 
+/* ------------------------------------------------------- batched form
+ * No counterpart in the reference (one scene per NewtonSolver): one solver handle steps a batch
+ * of independent scenes together, every stage one launch over all of them. Meshes added (or
+ * scene files loaded) after lcsx_solver_begin_scene belong to a new scene; meshes added before
+ * the first call form scene 0. All scenes share the parameters (dt, stiffnesses, tolerances).
+ * Each scene keeps its own LBVH (a segment of the shared forest), its own PCG scalars and
+ * convergence test, its own CCD step bound and its own Newton loop; a scene whose loop has
+ * stopped keeps its positions while the others iterate. Positions and velocities of all scenes
+ * are concatenated in scene order; lcsx_solver_scene_range gives each scene's vertex range.
+ * lcsx_solver_get_stats returns batch totals (iteration counts summed over scenes, the smallest
+ * step bound, the largest |dx|); lcsx_solver_get_scene_stats one scene's. */
+lcsx_status lcsx_solver_begin_scene(lcsx_solver* s, int* scene_id);
+int         lcsx_solver_num_scenes(lcsx_solver* s);
+lcsx_status lcsx_solver_scene_range(lcsx_solver* s, int scene, int* first_vertex, int* num_verts);
+lcsx_status lcsx_solver_get_scene_stats(lcsx_solver* s, int scene, lcsx_stats* out);
+
 /* ----------------------------------------------- per-stage entry points
  * The reference's subsystem methods, applied to the solver's own device buffers. Used for
  * parity checks and per-kernel measurement. stage is one of:

@@ -79,6 +79,10 @@ _SIGS = {
     "lcsx_solver_set_pinned_targets": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_float), C.c_int]),
     "lcsx_solver_get_stats": (C.c_int, [C.c_void_p, C.POINTER(lcsx_stats)]),
     "lcsx_solver_num_verts": (C.c_int, [C.c_void_p]),
+    "lcsx_solver_begin_scene": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
+    "lcsx_solver_num_scenes": (C.c_int, [C.c_void_p]),
+    "lcsx_solver_scene_range": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "lcsx_solver_get_scene_stats": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(lcsx_stats)]),
     "lcsx_solver_run_stage": (C.c_int, [C.c_void_p, C.c_char_p]),
     "lcsx_solver_read": (C.c_int, [C.c_void_p, C.c_char_p, C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t)]),
     "lcsx_solver_write": (C.c_int, [C.c_void_p, C.c_char_p, C.c_void_p, C.c_size_t]),
@@ -196,6 +200,47 @@ class NewtonSolver:
         if scene.initial_position is not None or scene.initial_velocity is not None:
             s.set_state(scene.initial_position, scene.initial_velocity)
         return s
+
+    @classmethod
+    def from_scenes(cls, scenes: List[Scene], device: int = 0) -> "NewtonSolver":
+        """The batched form: every scene of the list in one solver (the first scene's params apply
+        to all), stepped together with one launch per stage."""
+        s = cls(scenes[0].params, device)
+        for k, sc in enumerate(scenes):
+            if k:
+                s.begin_scene()
+            for m in sc.meshes:
+                mid = s.register_mesh(m.name, m.verts, m.tris, m.material)
+                for fp in m.fixed:
+                    s.add_fixed_points(mid, fp.method, fp.range, fp.anim)
+        s.init_solver()
+        if any(sc.initial_position is not None or sc.initial_velocity is not None for sc in scenes):
+            x = np.concatenate([np.asarray(sc.initial_position if sc.initial_position is not None else
+                                           np.concatenate([m.verts for m in sc.meshes]), np.float32).reshape(-1, 3)
+                                for sc in scenes])
+            v = np.concatenate([np.asarray(sc.initial_velocity if sc.initial_velocity is not None else
+                                           np.zeros((sc.num_verts, 3)), np.float32).reshape(-1, 3) for sc in scenes])
+            s.set_state(x, v)
+        return s
+
+    def begin_scene(self) -> int:
+        sid = C.c_int()
+        _check(self.lib.lcsx_solver_begin_scene(self.h, C.byref(sid)))
+        return sid.value
+
+    @property
+    def num_scenes(self) -> int:
+        return int(self.lib.lcsx_solver_num_scenes(self.h))
+
+    def scene_range(self, scene: int) -> tuple:
+        a, n = C.c_int(), C.c_int()
+        _check(self.lib.lcsx_solver_scene_range(self.h, scene, C.byref(a), C.byref(n)))
+        return a.value, n.value
+
+    def scene_stats(self, scene: int) -> dict:
+        s = lcsx_stats()
+        _check(self.lib.lcsx_solver_get_scene_stats(self.h, scene, C.byref(s)))
+        return {f: getattr(s, f) for f, _ in lcsx_stats._fields_}
 
     def __del__(self):
         try:

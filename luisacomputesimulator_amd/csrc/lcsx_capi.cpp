@@ -34,7 +34,8 @@ using namespace lcsx;
 
 struct lcsx_solver
 {
-    Solver s;
+    Solver   s;
+    uint32_t cur_scene = 0;  // scene of the meshes added next
 };
 struct lcsx_lbvh
 {
@@ -118,9 +119,9 @@ bool read_tree(const Lbvh& t, const std::string& f, hipStream_t s, Blob& out)
     }
     else if (f == "global_aabb")
     {
-        auto               g = dl(t.gbox, 6, s);
-        std::vector<float> h(6);
-        for (int k = 0; k < 6; k++) h[k] = ord2f(g[k]);
+        auto               g = dl(t.gbox, 6 * (size_t)t.nseg, s);
+        std::vector<float> h(g.size());
+        for (size_t k = 0; k < g.size(); k++) h[k] = ord2f(g[k]);
         out.set(h);
     }
     else
@@ -191,7 +192,9 @@ bool read_named(Solver& S, const std::string& name, Blob& out)
     if (name == "toi")
     {
         S.read_status();
-        out.set(std::vector<float>{ord2f(S.host_status.toi_bits)});
+        std::vector<float> t(S.S.nscenes);
+        for (uint32_t k = 0; k < S.S.nscenes; k++) t[k] = ord2f(S.host_sst[k].toi_bits);
+        out.set(t);
         return true;
     }
     if (name.rfind("face_tree.", 0) == 0) return read_tree(S.tree_face, name.substr(10), s, out);
@@ -337,6 +340,7 @@ lcsx_status lcsx_solver_add_mesh(lcsx_solver* s, const float* verts, int n_verts
     m.tris.assign(tris, tris + 3 * (size_t)n_tris);
     if (material) m.mat = *material;
     else lcsx_material_default(&m.mat);
+    m.scene = s->cur_scene;
     s->s.meshes.push_back(std::move(m));
     if (mesh_id) *mesh_id = (int)s->s.meshes.size() - 1;
     return LCSX_OK;
@@ -364,11 +368,45 @@ lcsx_status lcsx_solver_load_scene_json(lcsx_solver* s, const char* json_path, c
     if (!s || !json_path) return arg_error("invalid arguments");
     if (s->s.inited) { set_error("load_scene_json after init"); return LCSX_ERR_STATE; }
     std::string err;
+    const size_t first = s->s.meshes.size();
     if (!load_scene_json(json_path, mesh_dir ? mesh_dir : "", s->s.prm, s->s.meshes, err))
     {
         set_error(err);
         return LCSX_ERR_IO;
     }
+    for (size_t k = first; k < s->s.meshes.size(); k++) s->s.meshes[k].scene = s->cur_scene;
+    return LCSX_OK;
+}
+
+lcsx_status lcsx_solver_begin_scene(lcsx_solver* s, int* scene_id)
+{
+    if (!s) return arg_error("solver is NULL");
+    if (s->s.inited) { set_error("begin_scene after init"); return LCSX_ERR_STATE; }
+    if (!s->s.meshes.empty() && s->s.meshes.back().scene == s->cur_scene) s->cur_scene++;
+    if (scene_id) *scene_id = (int)s->cur_scene;
+    return LCSX_OK;
+}
+
+int lcsx_solver_num_scenes(lcsx_solver* s) { return s && (s->s.inited || s->s.setup_ready) ? (int)s->s.S.nscenes : 0; }
+
+lcsx_status lcsx_solver_scene_range(lcsx_solver* s, int scene, int* first_vertex, int* num_verts)
+{
+    if (!s || !(s->s.inited || s->s.setup_ready) || scene < 0 || scene >= (int)s->s.S.nscenes)
+        return arg_error("invalid scene");
+    if (first_vertex) *first_vertex = (int)s->s.S.sc_v0[scene];
+    if (num_verts) *num_verts = (int)(s->s.S.sc_v0[scene + 1] - s->s.S.sc_v0[scene]);
+    return LCSX_OK;
+}
+
+lcsx_status lcsx_solver_get_scene_stats(lcsx_solver* s, int scene, lcsx_stats* out)
+{
+    if (!s || !out || !s->s.inited || scene < 0 || scene >= (int)s->s.scene_stats.size())
+        return arg_error("invalid scene");
+    *out = s->s.scene_stats[scene];
+    out->num_pairs_vf = s->s.stats.num_pairs_vf;
+    out->num_pairs_ee = s->s.stats.num_pairs_ee;
+    out->num_contacts = s->s.stats.num_contacts;
+    out->pcg_rz_true  = s->s.stats.pcg_rz_true;
     return LCSX_OK;
 }
 
@@ -438,10 +476,10 @@ lcsx_status lcsx_solver_run_stage(lcsx_solver* s, const char* stage)
         else if (st == "predict") S.predict();
         else if (st == "iter_begin") S.iter_begin();
         else if (st == "material") S.material();
-        else if (st == "broad_phase_dcd") { launch_reset_status(S.status.p, S.stream); S.broad_phase(false); }
-        else if (st == "broad_phase_ccd") { launch_reset_line_search(S.status.p, S.stream); S.broad_phase(true); }
+        else if (st == "broad_phase_dcd") { launch_reset_status(S.status.p, S.sst.p, S.S.nscenes, S.stream); S.broad_phase(false); }
+        else if (st == "broad_phase_ccd") { launch_reset_line_search(S.status.p, S.sst.p, S.S.nscenes, S.stream); S.broad_phase(true); }
         else if (st == "dcd") S.dcd();
-        else if (st == "update_contacts") { launch_reset_status(S.status.p, S.stream); S.update_contacts(); }
+        else if (st == "update_contacts") { launch_reset_status(S.status.p, S.sst.p, S.S.nscenes, S.stream); S.update_contacts(); }
         else if (st == "contact_assemble") S.contact_assemble();
         else if (st == "pcg") S.pcg();
         else if (st == "ccd") { S.ccd(); }
@@ -538,7 +576,8 @@ lcsx_status lcsx_lbvh_construct(lcsx_lbvh* t, const float* x, int n_verts, void*
 {
     if (!t || !x || n_verts <= 0) return arg_error("invalid arguments");
     t->last = (hipStream_t)stream;
-    LCSX_GUARD(t->t.construct(x, (uint32_t)n_verts, (hipStream_t)stream); return LCSX_OK;)
+    const uint32_t nvs = (uint32_t)n_verts;
+    LCSX_GUARD(t->t.construct(x, &nvs, (hipStream_t)stream); return LCSX_OK;)
 }
 lcsx_status lcsx_lbvh_refit(lcsx_lbvh* t, const float* x0, const float* x1, const float* pad, void* stream)
 {

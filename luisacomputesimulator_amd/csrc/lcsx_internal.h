@@ -65,11 +65,20 @@ struct NodeBox
 };
 
 // ------------------------------------------------------------------ LBVH
+// A forest of independent trees, one per segment (scene of a batch): the leaves of segment g are
+// the elements [seg0[g], seg0[g+1]). Leaves are sorted by (segment, Morton key, element), each
+// segment's tree is the Karras tree of its own keys, and queries start at their segment's root.
+// Node ids: inner nodes [0, ni) with segment g's at [ib_g, ib_g + n_g - 1), then leaves
+// [ni, ni + n) in sorted order; with one segment this is the reference's numbering.
 struct Lbvh
 {
     int               arity = 3;
     uint32_t          n     = 0;  // leaves
+    uint32_t          ni    = 0;  // inner nodes: n minus the number of non-empty segments
     const uint32_t*   prims = nullptr;  // device, arity per leaf (not owned)
+    uint32_t              nseg = 1;
+    std::vector<uint32_t> seg0_h, ib_h, roots_h, cover_h, tab_h;  // per segment (seg0_h: nseg + 1)
+    DBuf<uint32_t>    seg_of_leaf, inner_seg, seg_tab, roots, khi[2];
     DBuf<uint32_t>    keys[2], vals[2], hist;
     DBuf<float4>      center;
     DBuf<uint64_t>    morton_sorted;
@@ -86,11 +95,15 @@ struct Lbvh
     bool                  use_levels = false;          // false: one-pass ticket refit (very tall trees)
     uint32_t*         sorted_to_orig = nullptr;  // points into vals[?] after construct
 
-    void init(int arity_, const uint32_t* device_prims, uint32_t n_leaves);
-    void construct(const float* x, uint32_t n_verts_for_reduce, hipStream_t s);
+    // seg0: nseg + 1 leaf offsets (null: one segment of all leaves)
+    void init(int arity_, const uint32_t* device_prims, uint32_t n_leaves, const uint32_t* seg0 = nullptr,
+              uint32_t nseg_ = 1);
+    // verts_per_seg: vertices of each segment; the global box of segment g covers its first
+    // min(n_g, 256 ceil(verts_g / 256)) leaves (lbvh.cpp:958, 968)
+    void construct(const float* x, const uint32_t* verts_per_seg, hipStream_t s);
     void refit(const float* x0, const float* x1, const float* pad, hipStream_t s);
-    uint32_t n_inner() const { return n ? n - 1 : 0; }
-    uint32_t n_nodes() const { return n ? 2 * n - 1 : 0; }
+    uint32_t n_inner() const { return ni; }
+    uint32_t n_nodes() const { return n + ni; }
 };
 
 // Query modes for the solver's broad phase.
@@ -104,6 +117,8 @@ struct QueryArgs
 {
     int             mode;
     uint32_t        n_queries;
+    const uint32_t* vert_scene;  // per vertex: segment of the query (null: segment 0)
+    const uint8_t*  active;      // per segment: 0 skips its queries (null: all)
     const float*    x0;
     const float*    x1;
     const uint32_t* edges;

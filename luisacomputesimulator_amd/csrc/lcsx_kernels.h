@@ -11,11 +11,35 @@ namespace lcsx
 struct Status
 {
     uint32_t n_vf, n_ee, n_contacts, overflow;  // broad-phase counts, contacts, stack overflow flag
-    uint32_t toi_bits;                          // ordered key (f2ord) of the step bound
-    uint32_t maxdx_bits;                        // float bits of max |dx|
-    uint32_t pin_bits;                          // float bits of max pinned |x - x_tilde|
     uint32_t nan;
-    uint32_t n_vf_ccd, n_ee_ccd, pad0, pad1;
+    uint32_t n_vf_ccd, n_ee_ccd, pad0;
+};
+// Line-search results of one scene of the batch.
+struct SceneStatus
+{
+    uint32_t toi_bits;    // ordered key (f2ord) of the step bound
+    uint32_t maxdx_bits;  // float bits of max |dx|
+    uint32_t pin_bits;    // float bits of max pinned |x - x_tilde|
+    uint32_t pad;
+};
+
+// Scenes of a batch. Vertices of scene s are [sc_v0[s], sc_v0[s+1]); row blocks of 256 rows
+// never straddle two scenes (block b: scene rb_scene[b], first row rb_row0[b]; scene s owns
+// blocks [sc_rb0[s], sc_rb0[s+1])), and neither do the 256-slot chunks of the contact slots
+// (chunk c: scene cb_scene[c], first slot cb_slot0[c]; scene s owns [sc_cb0[s], sc_cb0[s+1])),
+// so every per-block partial sum belongs to one scene. One scene: the trivial tables.
+struct SegView
+{
+    uint32_t        nscenes, n_rb;
+    const uint32_t* sc_v0;
+    const uint32_t* sc_rb0;
+    const uint32_t* rb_scene;
+    const uint32_t* rb_row0;
+    const uint32_t* vert_scene;
+    const uint8_t*  active;  // per scene: 1 while its Newton loop runs
+    const uint32_t* sc_cb0;
+    const uint32_t* cb_scene;
+    const uint32_t* cb_slot0;
 };
 
 // PCG scalars shared between launches. r.z of iteration k is stored in rz[k & 1] by the SpMV of
@@ -126,6 +150,8 @@ struct CcdArgs
     int             fp64;  // mixed-precision mode: ACCD in double, step bound rounded down to FP32
     uint32_t*       rest_list;   // capacity pair_capacity
     uint32_t*       rest_count;
+    SceneStatus*    sst;         // step bound per scene (scene of the pair's first vertex)
+    const uint32_t* vert_scene;
 };
 
 struct LsArgs
@@ -139,6 +165,8 @@ struct LsArgs
     float          floor_y;
     int            use_floor;
     Status*        st;
+    SceneStatus*   sst;
+    SegView        seg;  // launched over the scene-aligned row blocks
 };
 
 struct PcgArgs
@@ -154,10 +182,10 @@ struct PcgArgs
     float*       p_new;
     float*       q;
     float*       qc;       // contact part of q (3 per row, kept zero between iterations), null without contacts
-    float*       part_rz;  // per-block r.z partials of init / update (n_rz of them)
-    float*       part_pq;  // per-block p.q partials of the SpMV (n_pq of them)
-    uint32_t     n_rz, n_pq;
-    PcgState*    s;
+    float*       part_rz;  // per row block: r.z partial of init / update
+    float*       part_pq;  // p.q partials of the SpMV: row blocks, then contact-slot chunks
+    SegView      seg;
+    PcgState*    s;        // per scene
     float        rel2, abs_tol;
     uint32_t     check_every;
     uint32_t     k;  // iteration index of the launch
@@ -177,9 +205,12 @@ void launch_vert_assemble(const VertArgs& a, hipStream_t s);
 void launch_offdiag(uint32_t nblocks, const uint32_t* ptr, const uint32_t* con, const float* fs, const float* hs, float* val,
                     hipStream_t s);
 void launch_dcd(const DcdArgs& a, int kind, unsigned grid, hipStream_t s);
-// Per-vertex contact lists (cnt: nv + 1 scratch counters, ptr: nv + 1, list: 4 per contact).
+// Per-vertex contact lists (cnt: nv + 1 scratch counters, ptr: nv + 1, list: 4 per contact), and
+// the scene-aligned 256-slot chunks of the slot list (sc_cb0: nscenes + 1, cb_scene / cb_slot0:
+// up to slots / 256 + nscenes entries).
 void launch_contact_adjacency(const ContactView& c, uint32_t nv, unsigned grid, uint32_t* cnt, uint32_t* ptr,
-                              uint32_t* list, uint32_t* row, float4* coef, hipStream_t s);
+                              uint32_t* list, uint32_t* row, float4* coef, const SegView& g, uint32_t* sc_cb0,
+                              uint32_t* cb_scene, uint32_t* cb_slot0, hipStream_t s);
 void launch_contact_assemble(const ContactView& c, const VertContacts& vc, float* cgB, float* diag, uint32_t nv,
                              hipStream_t s);
 void launch_pcg_init(const PcgArgs& a, hipStream_t s);
@@ -187,26 +218,31 @@ void launch_pcg_init(const PcgArgs& a, hipStream_t s);
 // (p = z + beta p formed on the fly, material blocks and contact slots, p.q and alpha), then the
 // update (x, r, z, r.z, beta, convergence flag). With start/stop events the SpMV is launched with
 // hipExtLaunchKernelGGL, which timestamps the kernel's own execution.
-void launch_pcg_contact_t(const PcgArgs& a, const ContactView& c, float* tc, unsigned grid, hipStream_t s);
-void launch_pcg_spmv(const PcgArgs& a, const SellView& A, const VertContacts& vc, hipStream_t s,
+// max_chunks: an upper bound of the contact-slot chunks (slots / 256 + scenes); blocks past the
+// actual count return at once.
+void launch_pcg_contact_t(const PcgArgs& a, const ContactView& c, const VertContacts& vc, unsigned max_chunks,
+                          hipStream_t s);
+void launch_pcg_spmv(const PcgArgs& a, const SellView& A, const VertContacts& vc, unsigned max_chunks, hipStream_t s,
                      hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
 void launch_pcg_update(const PcgArgs& a, hipStream_t s);
 // Mixed precision: FP16 copy of the off-diagonal blocks with a power-of-two scale per block.
 void launch_pack_fp16(uint32_t nblocks, const float* val, __half* val_h, int8_t* val_e, hipStream_t s);
 // Residual replacement: r = b - y, z = Minv r, per-block r.z with TwoSum compensation into
 // part_rz[block] (sum + error) and part_res[2 block] (sum, error).
-void launch_residual(uint32_t n, const float* b, const float* y, const float* minv, float* r, float* z, float* part_rz,
-                     float* part_res, hipStream_t s);
+void launch_residual(const SegView& g, const float* b, const float* y, const float* minv, float* r, float* z,
+                     float* part_rz, float* part_res, hipStream_t s);
 // y = A p (diagonal + material blocks + contacts), for the spmv stage
 void launch_spmv(const SellView& A, const ContactView& c, const VertContacts& vc, unsigned contact_grid, const float* p,
                  float* y, uint32_t n, hipStream_t s);
-void launch_apply_dx(uint32_t nv, const float* x_iter, const float* dx, float* x, const uint32_t* alpha_key,
-                     float alpha_const, hipStream_t s);
+// x = x_iter + alpha dx with alpha the step bound of the vertex's scene (sst) or alpha_const
+// (sst null); vertices of scenes whose Newton loop has stopped keep x = x_iter.
+void launch_apply_dx(uint32_t nv, const float* x_iter, const float* dx, float* x, const SceneStatus* sst,
+                     float alpha_const, const SegView& g, hipStream_t s);
 void launch_ground_ccd_maxdx(const LsArgs& a, hipStream_t s);
 void launch_ccd(const CcdArgs& a, unsigned grid, hipStream_t s);
-void launch_pin_delta(uint32_t nfix, const uint32_t* ids, const float* x, const float* x_tilde, uint32_t* bits,
-                      hipStream_t s);
-void launch_reset_status(Status* st, hipStream_t s);
-void launch_reset_line_search(Status* st, hipStream_t s);
+void launch_pin_delta(uint32_t nfix, const uint32_t* ids, const float* x, const float* x_tilde, SceneStatus* sst,
+                      const uint32_t* vert_scene, hipStream_t s);
+void launch_reset_status(Status* st, SceneStatus* sst, uint32_t nscenes, hipStream_t s);
+void launch_reset_line_search(Status* st, SceneStatus* sst, uint32_t nscenes, hipStream_t s);
 
 }  // namespace lcsx

@@ -44,6 +44,39 @@ LCSX_D float block_reduce_all(const float* __restrict__ part, uint32_t n, float*
     return v;
 }
 
+// Sum of the partials [b0, e0) then [b1, e1), as block_reduce_all over their concatenation.
+LCSX_D float block_reduce_ranges(const float* __restrict__ part, uint32_t b0, uint32_t e0, uint32_t b1, uint32_t e1,
+                                 float* sh)
+{
+    __shared__ float bc;
+    const uint32_t   n0 = e0 - b0, n = n0 + (e1 - b1);
+    float            v  = 0.0f;
+    for (uint32_t i = threadIdx.x; i < n; i += kB) v = v + part[i < n0 ? b0 + i : b1 + (i - n0)];
+    v = block_sum(v, sh);
+    if (threadIdx.x == 0) bc = v;
+    __syncthreads();
+    v = bc;
+    __syncthreads();
+    return v;
+}
+
+// Row block of a scene-aligned launch: its scene and this thread's row (valid below the end of
+// the block or of the scene).
+struct RowBlk
+{
+    uint32_t scene, row;
+    bool     valid;
+};
+LCSX_D RowBlk row_blk(const SegView& g)
+{
+    RowBlk   r;
+    uint32_t b = blockIdx.x;
+    r.scene    = g.rb_scene[b];
+    r.row      = g.rb_row0[b] + threadIdx.x;
+    r.valid    = r.row < g.sc_v0[r.scene + 1];
+    return r;
+}
+
 // The flag read once per block (a launch may set it while other blocks start).
 LCSX_D bool block_flag(const uint32_t* flag)
 {
@@ -499,6 +532,43 @@ __global__ __launch_bounds__(kB) void k_vc_coeffs(ContactView c, const uint32_t*
     }
 }
 
+// Scene-aligned 256-slot chunks of the slot list: scene s's slots [ptr[v0_s], ptr[v0_{s+1}])
+// are cut into ceil(slots / 256) chunks (one workgroup).
+__global__ __launch_bounds__(1024) void k_chunk_table(SegView g, const uint32_t* __restrict__ ptr,
+                                                       uint32_t* __restrict__ sc_cb0, uint32_t* __restrict__ cb_scene,
+                                                       uint32_t* __restrict__ cb_slot0)
+{
+    __shared__ uint32_t part[1024];
+    const uint32_t t   = threadIdx.x, S = g.nscenes;
+    const uint32_t per = (S + 1023u) / 1024u;
+    const uint32_t b   = min(S, t * per), e = min(S, b + per);
+    auto nch = [&](uint32_t sc) { return (ptr[g.sc_v0[sc + 1]] - ptr[g.sc_v0[sc]] + 255u) / 256u; };
+    uint32_t sum = 0;
+    for (uint32_t sc = b; sc < e; sc++) sum += nch(sc);
+    part[t] = sum;
+    __syncthreads();
+    for (uint32_t off = 1; off < 1024u; off <<= 1)
+    {
+        uint32_t v = t >= off ? part[t - off] : 0u;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    uint32_t run = part[t] - sum;
+    for (uint32_t sc = b; sc < e; sc++)
+    {
+        uint32_t c0 = run, k = nch(sc), s0 = ptr[g.sc_v0[sc]];
+        sc_cb0[sc]  = c0;
+        for (uint32_t c = 0; c < k; c++)
+        {
+            cb_scene[c0 + c] = sc;
+            cb_slot0[c0 + c] = s0 + 256u * c;
+        }
+        run += k;
+    }
+    if (t == 1023u) sc_cb0[S] = part[1023];
+}
+
 // t_c = sum_j w_j n.p_j for every contact, with p given by pfun.
 template <class PF>
 LCSX_D void contact_t(const ContactView& c, float* __restrict__ tc, PF pfun)
@@ -519,11 +589,11 @@ LCSX_D void contact_t(const ContactView& c, float* __restrict__ tc, PF pfun)
 // One 64-slot chunk per wavefront, starting at `base`: slot contributions a t_c are summed over
 // each run of equal rows by a segmented shuffle scan (slots are in row order); the last lane of
 // a run returns true with the run's row and sum. Called by all 64 lanes.
-LCSX_D bool slot_chunk(const VertContacts& vc, uint32_t base, uint32_t& row, f3& sum)
+LCSX_D bool slot_chunk(const VertContacts& vc, uint32_t base, uint32_t end, uint32_t& row, f3& sum)
 {
     const uint32_t lane = lane_id();
     uint32_t       e    = base + lane;
-    bool           ok   = e < vc.nslots;
+    bool           ok   = e < end;
     row                 = 0xffffffffu;
     float vx = 0.0f, vy = 0.0f, vz = 0.0f;
     if (ok)
@@ -559,26 +629,44 @@ struct IterScalars
     float rz, beta;
     bool  stop;
 };
-LCSX_D IterScalars iter_scalars(const PcgArgs& a, float* sh)
+LCSX_D IterScalars iter_scalars(const PcgArgs& a, uint32_t scene, float* sh)
 {
-    IterScalars o;
-    o.rz          = block_reduce_all(a.part_rz, a.n_rz, sh);
-    float rz0     = a.k == 0 ? o.rz : a.s->rz0;
+    IterScalars     o;
+    const PcgState* st = a.s + scene;
+    o.rz          = block_reduce_ranges(a.part_rz, a.seg.sc_rb0[scene], a.seg.sc_rb0[scene + 1], 0u, 0u, sh);
+    float rz0     = a.k == 0 ? o.rz : st->rz0;
     bool  check   = (a.k % a.check_every) == 0u;
     o.stop        = !(o.rz == o.rz) || (check && (o.rz < a.abs_tol || o.rz <= a.rel2 * rz0));
-    float rz_old  = a.k == 0 ? 0.0f : a.s->rz[(a.k - 1u) & 1u];
+    float rz_old  = a.k == 0 ? 0.0f : st->rz[(a.k - 1u) & 1u];
     o.beta        = rz_old < 1e-10f ? 0.0f : o.rz / rz_old;
     return o;
 }
 
-__global__ __launch_bounds__(kB) void k_contact_t_pcg(PcgArgs a, ContactView c, float* __restrict__ tc)
+// t_c of the contacts whose first slot (slot 0, the contact's first vertex) lies in this
+// scene-aligned chunk of the slot list, with the scene's beta.
+__global__ __launch_bounds__(kB) void k_contact_t_pcg(PcgArgs a, ContactView c, VertContacts vc)
 {
     __shared__ float sh[4];
-    if (block_flag(&a.s->converged)) return;
-    IterScalars sc = iter_scalars(a, sh);
+    const uint32_t   ch = blockIdx.x;
+    if (ch >= a.seg.sc_cb0[a.seg.nscenes]) return;
+    const uint32_t scene = a.seg.cb_scene[ch];
+    if (block_flag(&a.s[scene].converged)) return;
+    IterScalars sc = iter_scalars(a, scene, sh);
     if (sc.stop) return;
-    const float beta = sc.beta;
-    contact_t(c, tc, [&](uint32_t v) { return ld3(a.z, v) + beta * ld3(a.p_old, v); });
+    const float    beta = sc.beta;
+    const uint32_t e    = a.seg.cb_slot0[ch] + threadIdx.x;
+    if (e >= vc.ptr[a.seg.sc_v0[scene + 1]]) return;
+    const uint32_t ent = vc.list[e];
+    if ((ent & 3u) != 0u) return;
+    const uint32_t k  = ent >> 2;
+    auto           pf = [&](uint32_t v) { return ld3(a.z, v) + beta * ld3(a.p_old, v); };
+    uint4          id = c.idx[k];
+    float4         nk = c.nk[k], w4 = c.w[k];
+    f3             nn = mk(nk.x, nk.y, nk.z);
+    float          t  = w4.x * dot(nn, pf(id.x & 0x7fffffffu));
+    t                 = t + w4.y * dot(nn, pf(id.y & 0x7fffffffu));
+    t                 = t + w4.z * dot(nn, pf(id.z & 0x7fffffffu));
+    vc.tc[k]          = t + w4.w * dot(nn, pf(id.w & 0x7fffffffu));
 }
 
 __global__ __launch_bounds__(kB) void k_contact_t_plain(ContactView c, const float* __restrict__ p, float* __restrict__ tc)
@@ -592,9 +680,10 @@ __global__ __launch_bounds__(kB) void k_contact_t_plain(ContactView c, const flo
 __global__ __launch_bounds__(kB) void k_pcg_init(PcgArgs a)
 {
     __shared__ float sh[4];
-    uint32_t         i   = blockIdx.x * kB + threadIdx.x;
+    const RowBlk     rb  = row_blk(a.seg);
+    const uint32_t   i   = rb.row;
     float            loc = 0.0f;
-    if (i < a.n)
+    if (rb.valid)
     {
         m3 D = ld_planes(a.diag, a.n, i);
         // 3x3 inverse by cofactors
@@ -628,11 +717,15 @@ __global__ __launch_bounds__(kB) void k_pcg_init(PcgArgs a)
     if (threadIdx.x == 0)
     {
         a.part_rz[blockIdx.x] = bs;
-        if (blockIdx.x == 0)
+        if (blockIdx.x == a.seg.sc_rb0[rb.scene])
         {
-            a.s->converged = 0u;
-            a.s->iters     = 0u;
-            a.s->nan       = 0u;
+            // a scene whose Newton loop has stopped starts converged: its blocks do no work
+            PcgState* st = a.s + rb.scene;
+            bool      on = a.seg.active == nullptr || a.seg.active[rb.scene] != 0;
+            st->converged = on ? 0u : 1u;
+            st->iters     = 0u;
+            st->nan       = 0u;
+            st->rz[0] = st->rz[1] = st->rz0 = 0.0f;
         }
     }
 }
@@ -646,29 +739,36 @@ __global__ __launch_bounds__(kB) void k_pcg_init(PcgArgs a)
 // sum into the contact accumulator qc (folded into q by the update). p.q is reduced over both
 // parts in the same launch and the last block sets alpha = r.z / p.q (precond_cg.cpp:24-31).
 template <bool LP>
-__global__ __launch_bounds__(kB) void k_spmv_pcg(PcgArgs a, SellView A, VertContacts vc, uint32_t row_blocks)
+__global__ __launch_bounds__(kB) void k_spmv_pcg(PcgArgs a, SellView A, VertContacts vc)
 {
     __shared__ float sh[4];
-    if (block_flag(&a.s->converged)) return;
-    IterScalars sc = iter_scalars(a, sh);
-    if (blockIdx.x == 0 && threadIdx.x == 0)
+    const uint32_t   n_rb  = a.seg.n_rb;
+    const bool       rows  = blockIdx.x < n_rb;
+    const uint32_t   ch    = blockIdx.x - n_rb;
+    if (!rows && ch >= a.seg.sc_cb0[a.seg.nscenes]) return;
+    const uint32_t scene = rows ? a.seg.rb_scene[blockIdx.x] : a.seg.cb_scene[ch];
+    PcgState*      st    = a.s + scene;
+    if (block_flag(&st->converged)) return;
+    IterScalars sc = iter_scalars(a, scene, sh);
+    if (rows && blockIdx.x == a.seg.sc_rb0[scene] && threadIdx.x == 0)
     {
-        a.s->rz[a.k & 1u] = sc.rz;
-        if (a.k == 0) a.s->rz0 = sc.rz;
+        st->rz[a.k & 1u] = sc.rz;
+        if (a.k == 0) st->rz0 = sc.rz;
         if (sc.stop)
         {
-            a.s->iters     = a.k;
-            a.s->nan       = (sc.rz == sc.rz) ? 0u : 1u;
-            a.s->converged = 1u;
+            st->iters     = a.k;
+            st->nan       = (sc.rz == sc.rz) ? 0u : 1u;
+            st->converged = 1u;
         }
     }
     if (sc.stop) return;
     const float beta = sc.beta;
     float       loc  = 0.0f;
-    if (blockIdx.x < row_blocks)
+    if (rows)
     {
-        uint32_t i = blockIdx.x * kB + threadIdx.x;
-        if (i < a.n)
+        const RowBlk rb = row_blk(a.seg);
+        const uint32_t i = rb.row;
+        if (rb.valid)
         {
             f3 pi = ld3(a.z, i) + beta * ld3(a.p_old, i);
             st3(a.p_new, i, pi);
@@ -703,7 +803,7 @@ __global__ __launch_bounds__(kB) void k_spmv_pcg(PcgArgs a, SellView A, VertCont
     {
         uint32_t row;
         f3       sum;
-        if (slot_chunk(vc, (blockIdx.x - row_blocks) * kB + (threadIdx.x & ~63u), row, sum))
+        if (slot_chunk(vc, a.seg.cb_slot0[ch] + (threadIdx.x & ~63u), vc.ptr[a.seg.sc_v0[scene + 1]], row, sum))
         {
             atomicAdd(&a.qc[3 * (size_t)row], sum.x);
             atomicAdd(&a.qc[3 * (size_t)row + 1], sum.y);
@@ -750,15 +850,16 @@ LCSX_D void two_sum(float a, float b, float& s, float& e)
 // true residual), z = Minv r, and the block's r.z summed pairwise with TwoSum, the rounding errors
 // carried alongside: part_rz[block] = sum + error (read by the next iteration), part_res[2 block]
 // = (sum, error) for the host's double total.
-__global__ __launch_bounds__(kB) void k_residual(uint32_t n, const float* __restrict__ b, const float* __restrict__ y,
+__global__ __launch_bounds__(kB) void k_residual(SegView g, const float* __restrict__ b, const float* __restrict__ y,
                                                   const float* __restrict__ minv, float* __restrict__ r,
                                                   float* __restrict__ z, float* __restrict__ part_rz,
                                                   float* __restrict__ part_res)
 {
     __shared__ float ss[kB], se[kB];
-    uint32_t         i = blockIdx.x * kB + threadIdx.x;
+    const RowBlk     rb = row_blk(g);
+    const uint32_t   i  = rb.row, n = g.sc_v0[g.nscenes];
     float            s = 0.0f, e = 0.0f;
-    if (i < n)
+    if (rb.valid)
     {
         f3 ri = ld3(b, i) - ld3(y, i);
         st3(r, i, ri);
@@ -815,7 +916,7 @@ __global__ __launch_bounds__(kB) void k_slots_plain(VertContacts vc, float* __re
 {
     uint32_t row;
     f3       sum;
-    if (slot_chunk(vc, blockIdx.x * kB + (threadIdx.x & ~63u), row, sum))
+    if (slot_chunk(vc, blockIdx.x * kB + (threadIdx.x & ~63u), vc.nslots, row, sum))
     {
         atomicAdd(&y[3 * (size_t)row], sum.x);
         atomicAdd(&y[3 * (size_t)row + 1], sum.y);
@@ -826,13 +927,17 @@ __global__ __launch_bounds__(kB) void k_slots_plain(VertContacts vc, float* __re
 __global__ __launch_bounds__(kB) void k_pcg_update(PcgArgs a)
 {
     __shared__ float sh[4];
-    if (block_flag(&a.s->converged)) return;
-    const float pq    = block_reduce_all(a.part_pq, a.n_pq, sh);
-    const float rz    = a.s->rz[a.k & 1u];
-    const float alpha = pq < 1e-10f ? 0.0f : rz / pq;
-    uint32_t    i     = blockIdx.x * kB + threadIdx.x;
-    float       loc   = 0.0f;
-    if (i < a.n)
+    const RowBlk rb = row_blk(a.seg);
+    PcgState*    st = a.s + rb.scene;
+    if (block_flag(&st->converged)) return;
+    const SegView& g     = a.seg;
+    const float    pq    = block_reduce_ranges(a.part_pq, g.sc_rb0[rb.scene], g.sc_rb0[rb.scene + 1],
+                                               g.n_rb + g.sc_cb0[rb.scene], g.n_rb + g.sc_cb0[rb.scene + 1], sh);
+    const float    rz    = st->rz[a.k & 1u];
+    const float    alpha = pq < 1e-10f ? 0.0f : rz / pq;
+    const uint32_t i     = rb.row;
+    float          loc   = 0.0f;
+    if (rb.valid)
     {
         f3 pn = ld3(a.p_new, i);
         st3(a.x, i, ld3(a.x, i) + alpha * pn);
@@ -854,11 +959,13 @@ __global__ __launch_bounds__(kB) void k_pcg_update(PcgArgs a)
 
 // ---------------------------------------------------------------- line search
 __global__ void k_apply_dx(uint32_t nv, const float* __restrict__ x_iter, const float* __restrict__ dx, float* __restrict__ x,
-                           const uint32_t* __restrict__ alpha_bits, float alpha_const)
+                           const SceneStatus* __restrict__ sst, float alpha_const, SegView g)
 {
     uint32_t i = blockIdx.x * kB + threadIdx.x;
     if (i >= nv) return;
-    float alpha = alpha_bits ? ord2f(*alpha_bits) : alpha_const;
+    uint32_t sc    = g.vert_scene[i];
+    float    alpha = sst ? ord2f(sst[sc].toi_bits) : alpha_const;
+    if (g.active && g.active[sc] == 0) alpha = 0.0f;
     st3(x, i, ld3(x_iter, i) + alpha * ld3(dx, i));
 }
 
@@ -866,9 +973,10 @@ __global__ void k_apply_dx(uint32_t nv, const float* __restrict__ x_iter, const 
 __global__ __launch_bounds__(kB) void k_ground_ccd_maxdx(LsArgs a)
 {
     __shared__ float sh[4][2];
-    uint32_t         i = blockIdx.x * kB + threadIdx.x;
+    const RowBlk     rb = row_blk(a.seg);
+    const uint32_t   i  = rb.row;
     float            toi = 1.0f, mdx = 0.0f;
-    if (i < a.nv)
+    if (rb.valid)
     {
         f3 d = ld3(a.dx, i);
         mdx  = length(d);
@@ -896,18 +1004,22 @@ __global__ __launch_bounds__(kB) void k_ground_ccd_maxdx(LsArgs a)
     {
         float t = fminf(fminf(sh[0][0], sh[1][0]), fminf(sh[2][0], sh[3][0]));
         float m = fmaxf(fmaxf(sh[0][1], sh[1][1]), fmaxf(sh[2][1], sh[3][1]));
-        if (t != 1.0f) atomicMin(&a.st->toi_bits, f2ord(t));
-        atomicMax(&a.st->maxdx_bits, __float_as_uint(m));
+        if (t != 1.0f) atomicMin(&a.sst[rb.scene].toi_bits, f2ord(t));
+        atomicMax(&a.sst[rb.scene].maxdx_bits, __float_as_uint(m));
     }
 }
 
 // fn_narrow_phase_{vf,ee}_ccd_query, narrow_phase.cpp:293-593: min over pairs of toi / 1.25.
 __global__ __launch_bounds__(kB) void k_ccd(CcdArgs a)
 {
-    __shared__ float sh[4];
+    __shared__ float    sh[4];
+    __shared__ uint32_t s_any;
     uint32_t         n    = a.rest ? min(*a.rest_count, a.pair_capacity) : min(*a.pair_count, a.pair_capacity);
-    float            tmin = kLineSearchMaxT;  // FP32 mode: smallest raw toi
-    float            bmin = 1.0f;             // FP64 mode: smallest step bound toi / 1.25
+    // smallest step bound of this thread's pairs, per run of pairs of one scene (pairs of one
+    // scene are mostly contiguous in the list)
+    uint32_t         cs   = kInvalid;
+    float            cmin = 1.0f;
+    if (threadIdx.x == 0) s_any = kInvalid;
     for (uint32_t idx = blockIdx.x * kB + threadIdx.x; idx < n; idx += gridDim.x * kB)
     {
         uint32_t p = a.rest ? a.rest_list[idx] : idx;
@@ -937,6 +1049,7 @@ __global__ __launch_bounds__(kB) void k_ccd(CcdArgs a)
         }
         float off  = a.offset[v[0]] + a.offset[a.kind == 0 ? v[1] : v[2]];
         bool  done = true;
+        float b    = 1.0f;
         if (a.fp64)
         {
             // double ACCD on the FP32 positions; its step bound toi / 1.25 rounded toward zero, so
@@ -949,60 +1062,87 @@ __global__ __launch_bounds__(kB) void k_ccd(CcdArgs a)
             }
             double t = fp64::accd_budget(d0, d1, (double)a.offset[v[0]] + (double)a.offset[a.kind == 0 ? v[1] : v[2]],
                                          a.kind, a.rest ? 0x7fffffff : a.budget, &done);
-            if (done) bmin = fminf(bmin, __double2float_rd(t / fp64::kMaxT));
+            if (done) b = __double2float_rd(t / fp64::kMaxT);
         }
         else
         {
+            // FP32: toi / 1.25 (the minimum of the quotients is the quotient of the minimum)
             float t = accd_budget(x0, x1, off, a.kind, a.rest ? 0x7fffffff : a.budget, &done);
-            if (done) tmin = fminf(tmin, t);
+            if (done && t != kLineSearchMaxT) b = t / kLineSearchMaxT;
         }
         uint32_t slot = wave_append(a.rest_count, !done);
         if (!done && slot < a.pair_capacity) a.rest_list[slot] = p;
+        uint32_t sc = a.vert_scene[v[0]];
+        if (sc != cs)
+        {
+            if (cs != kInvalid && cmin < 1.0f) atomicMin(&a.sst[cs].toi_bits, f2ord(cmin));
+            cs   = sc;
+            cmin = 1.0f;
+        }
+        cmin = fminf(cmin, b);
     }
-    // FP32: toi / 1.25 of the smallest raw toi (division by a positive constant keeps the order)
-    float bound = a.fp64 ? bmin : (tmin != kLineSearchMaxT ? tmin / kLineSearchMaxT : 1.0f);
-    bound       = wave_min(bound);
-    int w       = threadIdx.x >> 6;
-    if (lane_id() == 0) sh[w] = bound;
+    // one scene in the whole block (always with one scene): a block minimum and one atomic
     __syncthreads();
-    if (threadIdx.x == 0)
+    if (cs != kInvalid) s_any = cs;
+    __syncthreads();
+    const uint32_t s0 = s_any;
+    if (__syncthreads_and(cs == kInvalid || cs == s0))
     {
-        float t = fminf(fminf(sh[0], sh[1]), fminf(sh[2], sh[3]));
-        if (t < 1.0f) atomicMin(a.toi_bits, f2ord(t));
+        float bound = wave_min(cmin);
+        int   w     = threadIdx.x >> 6;
+        if (lane_id() == 0) sh[w] = bound;
+        __syncthreads();
+        if (threadIdx.x == 0)
+        {
+            float t = fminf(fminf(sh[0], sh[1]), fminf(sh[2], sh[3]));
+            if (s0 != kInvalid && t < 1.0f) atomicMin(&a.sst[s0].toi_bits, f2ord(t));
+        }
     }
+    else if (cs != kInvalid && cmin < 1.0f)
+        atomicMin(&a.sst[cs].toi_bits, f2ord(cmin));
 }
 
 // max |x - x_tilde| over pinned vertices (line_search, newton_solver.cpp:3860-3882)
 __global__ void k_pin_delta(uint32_t nfix, const uint32_t* __restrict__ ids, const float* __restrict__ x,
-                            const float* __restrict__ x_tilde, uint32_t* __restrict__ bits)
+                            const float* __restrict__ x_tilde, SceneStatus* __restrict__ sst,
+                            const uint32_t* __restrict__ vert_scene)
 {
-    uint32_t k = blockIdx.x * kB + threadIdx.x;
-    float    d = 0.0f;
+    uint32_t k  = blockIdx.x * kB + threadIdx.x;
+    float    d  = 0.0f;
+    uint32_t sc = kInvalid;
     if (k < nfix)
     {
         uint32_t i = ids[k];
         d          = length(ld3(x, i) - ld3(x_tilde, i));
+        sc         = vert_scene[i];
     }
-    d = wave_max(d);
-    if (lane_id() == 0) atomicMax(bits, __float_as_uint(d));
+    uint32_t s0 = __shfl(sc, 0);
+    if (__ballot(sc != kInvalid && sc != s0) == 0ull)
+    {
+        d = wave_max(d);
+        if (lane_id() == 0 && s0 != kInvalid) atomicMax(&sst[s0].pin_bits, __float_as_uint(d));
+    }
+    else if (sc != kInvalid)
+        atomicMax(&sst[sc].pin_bits, __float_as_uint(d));
 }
 
-__global__ void k_reset_ls(Status* st)
+__global__ void k_reset_ls(Status* st, SceneStatus* sst, uint32_t nscenes)
 {
-    st->n_vf_ccd = st->n_ee_ccd = 0;
-    st->toi_bits   = f2ord(1.0f);
-    st->maxdx_bits = 0;
-    st->pin_bits   = 0;
+    uint32_t i = blockIdx.x * kB + threadIdx.x;
+    if (i == 0) st->n_vf_ccd = st->n_ee_ccd = 0;
+    if (i < nscenes) sst[i] = SceneStatus{f2ord(1.0f), 0u, 0u, 0u};
 }
 
-__global__ void k_reset_status(Status* st)
+__global__ void k_reset_status(Status* st, SceneStatus* sst, uint32_t nscenes)
 {
-    st->n_vf = st->n_ee = st->n_contacts = st->overflow = 0;
-    st->n_vf_ccd = st->n_ee_ccd = 0;
-    st->toi_bits = f2ord(1.0f);
-    st->maxdx_bits = 0;
-    st->pin_bits   = 0;
-    st->nan        = 0;
+    uint32_t i = blockIdx.x * kB + threadIdx.x;
+    if (i == 0)
+    {
+        st->n_vf = st->n_ee = st->n_contacts = st->overflow = 0;
+        st->n_vf_ccd = st->n_ee_ccd = 0;
+        st->nan = 0;
+    }
+    if (i < nscenes) sst[i] = SceneStatus{f2ord(1.0f), 0u, 0u, 0u};
 }
 }  // namespace
 
@@ -1046,7 +1186,8 @@ void launch_dcd(const DcdArgs& a, int kind, unsigned grid, hipStream_t s)
     else k_dcd_ee<<<grid, kB, 0, s>>>(a);
 }
 void launch_contact_adjacency(const ContactView& c, uint32_t nv, unsigned grid, uint32_t* cnt, uint32_t* ptr,
-                              uint32_t* list, uint32_t* row, float4* coef, hipStream_t s)
+                              uint32_t* list, uint32_t* row, float4* coef, const SegView& g, uint32_t* sc_cb0,
+                              uint32_t* cb_scene, uint32_t* cb_slot0, hipStream_t s)
 {
     k_fill_u32<<<grid_for(nv + 1, kB), kB, 0, s>>>(cnt, nv + 1, 0u);
     k_vc_count<<<grid, kB, 0, s>>>(c, cnt);
@@ -1054,44 +1195,45 @@ void launch_contact_adjacency(const ContactView& c, uint32_t nv, unsigned grid, 
     k_fill_u32<<<grid_for(nv + 1, kB), kB, 0, s>>>(cnt, nv + 1, 0u);
     k_vc_fill<<<grid, kB, 0, s>>>(c, ptr, cnt, list, row);
     k_vc_coeffs<<<4 * grid, kB, 0, s>>>(c, ptr, nv, list, coef);
+    k_chunk_table<<<1, 1024, 0, s>>>(g, ptr, sc_cb0, cb_scene, cb_slot0);
 }
 void launch_contact_assemble(const ContactView& c, const VertContacts& vc, float* cgB, float* diag, uint32_t nv,
                              hipStream_t s)
 {
     k_contact_assemble<<<grid_for(nv, kB), kB, 0, s>>>(c, vc, cgB, diag, nv);
 }
-void launch_pcg_init(const PcgArgs& a, hipStream_t s) { k_pcg_init<<<grid_for(a.n, kB), kB, 0, s>>>(a); }
-void launch_pcg_contact_t(const PcgArgs& a, const ContactView& c, float* tc, unsigned grid, hipStream_t s)
+void launch_pcg_init(const PcgArgs& a, hipStream_t s) { k_pcg_init<<<a.seg.n_rb, kB, 0, s>>>(a); }
+void launch_pcg_contact_t(const PcgArgs& a, const ContactView& c, const VertContacts& vc, unsigned max_chunks,
+                          hipStream_t s)
 {
-    k_contact_t_pcg<<<grid, kB, 0, s>>>(a, c, tc);
+    if (max_chunks) k_contact_t_pcg<<<max_chunks, kB, 0, s>>>(a, c, vc);
 }
-void launch_pcg_spmv(const PcgArgs& a, const SellView& A, const VertContacts& vc, hipStream_t s, hipEvent_t start,
-                     hipEvent_t stop)
+void launch_pcg_spmv(const PcgArgs& a, const SellView& A, const VertContacts& vc, unsigned max_chunks, hipStream_t s,
+                     hipEvent_t start, hipEvent_t stop)
 {
-    uint32_t rb = grid_for(a.n, kB);
-    uint32_t g  = rb + (vc.ptr ? grid_for(vc.nslots, kB) : 0u);
+    uint32_t g = a.seg.n_rb + (vc.ptr ? max_chunks : 0u);
     if (A.val_h)
     {
         if (start)
-            hipExtLaunchKernelGGL(k_spmv_pcg<true>, dim3(g), dim3(kB), 0, s, start, stop, 0, a, A, vc, rb);
+            hipExtLaunchKernelGGL(k_spmv_pcg<true>, dim3(g), dim3(kB), 0, s, start, stop, 0, a, A, vc);
         else
-            k_spmv_pcg<true><<<g, kB, 0, s>>>(a, A, vc, rb);
+            k_spmv_pcg<true><<<g, kB, 0, s>>>(a, A, vc);
     }
     else if (start)
-        hipExtLaunchKernelGGL(k_spmv_pcg<false>, dim3(g), dim3(kB), 0, s, start, stop, 0, a, A, vc, rb);
+        hipExtLaunchKernelGGL(k_spmv_pcg<false>, dim3(g), dim3(kB), 0, s, start, stop, 0, a, A, vc);
     else
-        k_spmv_pcg<false><<<g, kB, 0, s>>>(a, A, vc, rb);
+        k_spmv_pcg<false><<<g, kB, 0, s>>>(a, A, vc);
 }
 void launch_pack_fp16(uint32_t nblocks, const float* val, __half* val_h, int8_t* val_e, hipStream_t s)
 {
     if (nblocks) k_pack_fp16<<<grid_for(nblocks, kB), kB, 0, s>>>(nblocks, val, val_h, val_e);
 }
-void launch_residual(uint32_t n, const float* b, const float* y, const float* minv, float* r, float* z, float* part_rz,
-                     float* part_res, hipStream_t s)
+void launch_residual(const SegView& g, const float* b, const float* y, const float* minv, float* r, float* z,
+                     float* part_rz, float* part_res, hipStream_t s)
 {
-    k_residual<<<grid_for(n, kB), kB, 0, s>>>(n, b, y, minv, r, z, part_rz, part_res);
+    k_residual<<<g.n_rb, kB, 0, s>>>(g, b, y, minv, r, z, part_rz, part_res);
 }
-void launch_pcg_update(const PcgArgs& a, hipStream_t s) { k_pcg_update<<<grid_for(a.n, kB), kB, 0, s>>>(a); }
+void launch_pcg_update(const PcgArgs& a, hipStream_t s) { k_pcg_update<<<a.seg.n_rb, kB, 0, s>>>(a); }
 void launch_spmv(const SellView& A, const ContactView& c, const VertContacts& vc, unsigned contact_grid, const float* p,
                  float* y, uint32_t n, hipStream_t s)
 {
@@ -1102,18 +1244,25 @@ void launch_spmv(const SellView& A, const ContactView& c, const VertContacts& vc
         k_slots_plain<<<grid_for(vc.nslots, kB), kB, 0, s>>>(vc, y);
     }
 }
-void launch_apply_dx(uint32_t nv, const float* x_iter, const float* dx, float* x, const uint32_t* alpha_bits,
-                     float alpha_const, hipStream_t s)
+void launch_apply_dx(uint32_t nv, const float* x_iter, const float* dx, float* x, const SceneStatus* sst,
+                     float alpha_const, const SegView& g, hipStream_t s)
 {
-    k_apply_dx<<<grid_for(nv, kB), kB, 0, s>>>(nv, x_iter, dx, x, alpha_bits, alpha_const);
+    k_apply_dx<<<grid_for(nv, kB), kB, 0, s>>>(nv, x_iter, dx, x, sst, alpha_const, g);
 }
-void launch_ground_ccd_maxdx(const LsArgs& a, hipStream_t s) { k_ground_ccd_maxdx<<<grid_for(a.nv, kB), kB, 0, s>>>(a); }
+void launch_ground_ccd_maxdx(const LsArgs& a, hipStream_t s) { k_ground_ccd_maxdx<<<a.seg.n_rb, kB, 0, s>>>(a); }
 void launch_ccd(const CcdArgs& a, unsigned grid, hipStream_t s) { k_ccd<<<grid, kB, 0, s>>>(a); }
-void launch_pin_delta(uint32_t nfix, const uint32_t* ids, const float* x, const float* x_tilde, uint32_t* bits, hipStream_t s)
+void launch_pin_delta(uint32_t nfix, const uint32_t* ids, const float* x, const float* x_tilde, SceneStatus* sst,
+                      const uint32_t* vert_scene, hipStream_t s)
 {
-    if (nfix) k_pin_delta<<<grid_for(nfix, kB), kB, 0, s>>>(nfix, ids, x, x_tilde, bits);
+    if (nfix) k_pin_delta<<<grid_for(nfix, kB), kB, 0, s>>>(nfix, ids, x, x_tilde, sst, vert_scene);
 }
-void launch_reset_status(Status* st, hipStream_t s) { k_reset_status<<<1, 1, 0, s>>>(st); }
-void launch_reset_line_search(Status* st, hipStream_t s) { k_reset_ls<<<1, 1, 0, s>>>(st); }
+void launch_reset_status(Status* st, SceneStatus* sst, uint32_t nscenes, hipStream_t s)
+{
+    k_reset_status<<<grid_for(nscenes, kB), kB, 0, s>>>(st, sst, nscenes);
+}
+void launch_reset_line_search(Status* st, SceneStatus* sst, uint32_t nscenes, hipStream_t s)
+{
+    k_reset_ls<<<grid_for(nscenes, kB), kB, 0, s>>>(st, sst, nscenes);
+}
 
 }  // namespace lcsx

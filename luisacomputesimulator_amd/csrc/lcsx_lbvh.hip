@@ -33,16 +33,31 @@ LCSX_HD uint32_t morton32(f3 p)
     return (expand_bits((uint32_t)x) << 2) | (expand_bits((uint32_t)y) << 1) | expand_bits((uint32_t)z);
 }
 
-// Leaf centres (lbvh.cpp:449-491) and the global box over leaves [0, covered): the reference's
-// second reduction pass is dispatched over ceil(n_verts / 256) blocks of leaves (lbvh.cpp:958, 968).
-__global__ __launch_bounds__(kBlock) void k_leaf_centers(uint32_t n, int arity, const uint32_t* __restrict__ prims,
-                                                          const float* __restrict__ x, uint32_t covered,
-                                                          float4* __restrict__ center, uint32_t* __restrict__ gbox)
+// Leaf centres (lbvh.cpp:449-491) and each segment's global box over its first cover leaves: the
+// reference's second reduction pass is dispatched over ceil(n_verts / 256) blocks of leaves
+// (lbvh.cpp:958, 968). seg_tab holds (first leaf, leaves, first inner node, cover) per segment.
+// A block whose leaves share one segment reduces in LDS and issues one atomic per bound; a
+// mixed block falls back to one atomic per wavefront, or per lane.
+LCSX_D void box_atomics(uint32_t* gbox, uint32_t g, const float v[6])
 {
+    for (int k = 0; k < 3; k++) atomicMin(&gbox[6 * g + k], f2ord(v[k]));
+    for (int k = 3; k < 6; k++) atomicMax(&gbox[6 * g + k], f2ord(v[k]));
+}
+
+__global__ __launch_bounds__(kBlock) void k_leaf_centers(uint32_t n, int arity, const uint32_t* __restrict__ prims,
+                                                          const float* __restrict__ x, const uint32_t* __restrict__ seg_of_leaf,
+                                                          const uint32_t* __restrict__ seg_tab, float4* __restrict__ center,
+                                                          uint32_t* __restrict__ gbox)
+{
+    __shared__ uint32_t g_first;
     uint32_t l   = blockIdx.x * kBlock + threadIdx.x;
     f3       lo  = mk(3.0e38f, 3.0e38f, 3.0e38f), hi = mk(-3.0e38f, -3.0e38f, -3.0e38f);
+    uint32_t g   = 0;
+    bool     in  = false;
+    if (threadIdx.x == 0) g_first = seg_of_leaf[blockIdx.x * kBlock];
     if (l < n)
     {
+        g = seg_of_leaf[l];
         f3 c, blo, bhi;
         if (arity == 3)
         {
@@ -59,49 +74,71 @@ __global__ __launch_bounds__(kBlock) void k_leaf_centers(uint32_t n, int arity, 
             bhi = vmax(p0, p1);
         }
         center[l] = make_float4(c.x, c.y, c.z, 0.0f);
-        if (l < covered)
+        if (l - seg_tab[4 * g] < seg_tab[4 * g + 3])
         {
             lo = blo;
             hi = bhi;
+            in = true;
         }
     }
-    float v[6] = {lo.x, lo.y, lo.z, hi.x, hi.y, hi.z};
-    for (int k = 0; k < 3; k++) v[k] = wave_min(v[k]);
-    for (int k = 3; k < 6; k++) v[k] = wave_max(v[k]);
-    __shared__ float s[4][6];
-    int              w = threadIdx.x >> 6;
-    if (lane_id() == 0)
-        for (int k = 0; k < 6; k++) s[w][k] = v[k];
     __syncthreads();
-    if (threadIdx.x < 6)
+    const uint32_t g0      = g_first;
+    const bool     uniform = __syncthreads_and(l >= n || g == g0) != 0;
+    float v[6] = {lo.x, lo.y, lo.z, hi.x, hi.y, hi.z};
+    if (uniform)
     {
-        int   k = threadIdx.x;
-        float r = s[0][k];
-        for (int ww = 1; ww < 4; ww++) r = k < 3 ? smin(r, s[ww][k]) : smax(r, s[ww][k]);
-        if (blockIdx.x * kBlock < covered)
+        for (int k = 0; k < 3; k++) v[k] = wave_min(v[k]);
+        for (int k = 3; k < 6; k++) v[k] = wave_max(v[k]);
+        __shared__ float sb[4][6];
+        int              w = threadIdx.x >> 6;
+        if (lane_id() == 0)
+            for (int k = 0; k < 6; k++) sb[w][k] = v[k];
+        __syncthreads();
+        if (threadIdx.x < 6)
         {
-            if (k < 3) atomicMin(&gbox[k], f2ord(r));
-            else atomicMax(&gbox[k], f2ord(r));
+            int   k = threadIdx.x;
+            float r = sb[0][k];
+            for (int ww = 1; ww < 4; ww++) r = k < 3 ? smin(r, sb[ww][k]) : smax(r, sb[ww][k]);
+            if (blockIdx.x * kBlock - seg_tab[4 * g0] < seg_tab[4 * g0 + 3])
+            {
+                if (k < 3) atomicMin(&gbox[6 * g0 + k], f2ord(r));
+                else atomicMax(&gbox[6 * g0 + k], f2ord(r));
+            }
         }
+        return;
     }
+    // mixed block: wavefronts of one segment reduce first
+    uint32_t gw    = __shfl(g, 0);
+    bool     wuni  = __ballot(l < n && g != gw) == 0ull;
+    if (wuni)
+    {
+        for (int k = 0; k < 3; k++) v[k] = wave_min(v[k]);
+        for (int k = 3; k < 6; k++) v[k] = wave_max(v[k]);
+        const bool any_in = __ballot(in) != 0ull;
+        if (lane_id() == 0 && any_in) box_atomics(gbox, gw, v);
+    }
+    else if (in)
+        box_atomics(gbox, g, v);
 }
 
-__global__ void k_init_gbox(uint32_t* gbox)
+__global__ void k_init_gbox(uint32_t* gbox, uint32_t nseg)
 {
-    int k = threadIdx.x;
-    if (k < 3) gbox[k] = 0xffffffffu;
-    else if (k < 6) gbox[k] = 0u;
+    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= 6 * nseg) return;
+    gbox[i] = (i % 6) < 3 ? 0xffffffffu : 0u;
 }
 
 // fn_compute_mortons, lbvh.cpp:507-523: key = morton32 of the normalised centre, value = leaf id.
 __global__ __launch_bounds__(kBlock) void k_morton(uint32_t n, const float4* __restrict__ center,
+                                                    const uint32_t* __restrict__ seg_of_leaf,
                                                     const uint32_t* __restrict__ gbox, uint32_t* __restrict__ keys,
                                                     uint32_t* __restrict__ vals)
 {
     uint32_t l = blockIdx.x * kBlock + threadIdx.x;
     if (l >= n) return;
-    f3     lo  = mk(ord2f(gbox[0]), ord2f(gbox[1]), ord2f(gbox[2]));
-    f3     hi  = mk(ord2f(gbox[3]), ord2f(gbox[4]), ord2f(gbox[5]));
+    const uint32_t* G = gbox + 6 * (size_t)seg_of_leaf[l];
+    f3     lo  = mk(ord2f(G[0]), ord2f(G[1]), ord2f(G[2]));
+    f3     hi  = mk(ord2f(G[3]), ord2f(G[4]), ord2f(G[5]));
     f3     ext = vmax(hi - lo, mk(1e-6f, 1e-6f, 1e-6f));
     f3     inv = mk(1.0f / ext.x, 1.0f / ext.y, 1.0f / ext.z);
     float4 c4  = center[l];
@@ -111,6 +148,7 @@ __global__ __launch_bounds__(kBlock) void k_morton(uint32_t n, const float4* __r
 }
 
 // ---------------------------------------------------------------- stable LSD radix sort
+// Histogram of the digit (dig >> shift) & 255 per tile.
 __global__ __launch_bounds__(kBlock) void k_radix_hist(const uint32_t* __restrict__ keys, uint32_t n, int shift,
                                                         uint32_t* __restrict__ hist, uint32_t nblocks)
 {
@@ -154,8 +192,12 @@ __global__ __launch_bounds__(1024) void k_scan_single(uint32_t* __restrict__ a, 
     }
 }
 
+// Stable scatter by the digit of dig (kin, or the segment keys hin): kin, vin and, when present,
+// hin move together.
 __global__ __launch_bounds__(kBlock) void k_radix_scatter(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
                                                            uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
+                                                           const uint32_t* __restrict__ hin, uint32_t* __restrict__ hout,
+                                                           const uint32_t* __restrict__ dig,
                                                            uint32_t n, int shift, const uint32_t* __restrict__ hist,
                                                            uint32_t nblocks)
 {
@@ -171,7 +213,7 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(const uint32_t* __rest
         uint32_t idx   = base + i * kBlock + t;
         bool     valid = idx < n;
         uint32_t key   = valid ? kin[idx] : 0u;
-        uint32_t d     = (key >> shift) & 255u;
+        uint32_t d     = ((valid ? dig[idx] : 0u) >> shift) & 255u;
         uint64_t peers = __ballot(valid);
         for (int b = 0; b < 8; b++)
         {
@@ -188,6 +230,7 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(const uint32_t* __rest
             for (uint32_t ww = 0; ww < w; ww++) off += s_cnt[ww][d];
             kout[off] = key;
             vout[off] = vin[idx];
+            if (hin) hout[off] = hin[idx];
         }
         __syncthreads();
         s_base[t] += s_cnt[0][t] + s_cnt[1][t] + s_cnt[2][t] + s_cnt[3][t];
@@ -195,33 +238,50 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(const uint32_t* __rest
     }
 }
 
-// fn_apply_sorted, lbvh.cpp:525-539 (+ the 64-bit keys the build reads)
-__global__ __launch_bounds__(kBlock) void k_leaves(uint32_t n, const uint32_t* __restrict__ skeys,
-                                                    const uint32_t* __restrict__ svals, uint64_t* __restrict__ morton_sorted,
+// fn_apply_sorted, lbvh.cpp:525-539 (+ the 64-bit keys the build reads: Morton key and the leaf's
+// index within its segment, as in a tree built over that segment alone)
+__global__ __launch_bounds__(kBlock) void k_leaves(uint32_t n, uint32_t ni, const uint32_t* __restrict__ skeys,
+                                                    const uint32_t* __restrict__ svals,
+                                                    const uint32_t* __restrict__ seg_of_leaf,
+                                                    const uint32_t* __restrict__ seg_tab, uint64_t* __restrict__ morton_sorted,
                                                     uint2* __restrict__ children, uint32_t* __restrict__ object_idx,
                                                     uint32_t* __restrict__ parent)
 {
     uint32_t l = blockIdx.x * kBlock + threadIdx.x;
     if (l >= n) return;
     uint32_t o       = svals[l];
-    morton_sorted[l] = ((uint64_t)skeys[l] << 32) | (uint64_t)o;
-    uint32_t ni      = n - 1;
+    uint32_t g       = seg_of_leaf[o];
+    morton_sorted[l] = ((uint64_t)skeys[l] << 32) | (uint64_t)(o - seg_tab[4 * g]);
     children[ni + l] = make_uint2(o, o);
     object_idx[ni + l] = o;
-    if (l == 0) parent[0] = kInvalid;
+    if (seg_tab[4 * g + 1] == 1u) parent[ni + l] = kInvalid;  // a one-leaf segment: the leaf is its root
+}
+
+__global__ __launch_bounds__(kBlock) void k_seg_keys(uint32_t n, const uint32_t* __restrict__ vals,
+                                                      const uint32_t* __restrict__ seg_of_leaf, uint32_t* __restrict__ khi)
+{
+    uint32_t l = blockIdx.x * kBlock + threadIdx.x;
+    if (l < n) khi[l] = seg_of_leaf[vals[l]];
 }
 
 LCSX_D int clz64(uint64_t v) { return v == 0 ? 64 : __clzll((long long)v); }
 
-// fn_build_inner_nodes with determineRange / findSplit, lbvh.cpp:204-291, 541-596
-__global__ __launch_bounds__(kBlock) void k_karras(uint32_t n, const uint64_t* __restrict__ m,
-                                                    uint32_t* __restrict__ parent, uint2* __restrict__ children,
-                                                    uint32_t* __restrict__ object_idx)
+// fn_build_inner_nodes with determineRange / findSplit, lbvh.cpp:204-291, 541-596, on the keys
+// of the inner node's segment: local leaf j is sorted position L + j (node ni + L + j), local
+// inner node j is node ib + j, and the segment's local node 0 is its root.
+__global__ __launch_bounds__(kBlock) void k_karras(uint32_t ni_, const uint64_t* __restrict__ mg,
+                                                    const uint32_t* __restrict__ inner_seg,
+                                                    const uint32_t* __restrict__ seg_tab, uint32_t* __restrict__ parent,
+                                                    uint2* __restrict__ children, uint32_t* __restrict__ object_idx)
 {
-    int nid = blockIdx.x * kBlock + threadIdx.x;
-    int ni  = (int)n - 1;
-    if (nid >= ni) return;
-    int  N  = (int)n;
+    int gid = blockIdx.x * kBlock + threadIdx.x;
+    if (gid >= (int)ni_) return;
+    const uint32_t  g   = inner_seg[gid];
+    const int       L   = (int)seg_tab[4 * g], N = (int)seg_tab[4 * g + 1], ib = (int)seg_tab[4 * g + 2];
+    const int       GI  = (int)ni_;
+    const uint64_t* m   = mg + L;
+    const int       nid = gid - ib;
+    const int       ni  = N - 1;
     auto cp = [&](uint64_t mi, int j) -> int { return (j >= 0 && j < N) ? clz64(mi ^ m[j]) : -1; };
     int  i = 0, j = ni;
     if (nid != 0)
@@ -257,24 +317,25 @@ __global__ __launch_bounds__(kBlock) void k_karras(uint32_t n, const uint64_t* _
             if (!(t > 1)) break;
         }
     }
-    int cl_ = (a == split) ? ni + split : split;
-    int cr_ = (b == split + 1) ? ni + split + 1 : split + 1;
-    if (cr_ >= ni)
+    int cl_ = (a == split) ? GI + L + split : ib + split;
+    int cr_ = (b == split + 1) ? GI + L + split + 1 : ib + split + 1;
+    if (cr_ >= GI)  // a leaf child comes first (lbvh.cpp:579-584)
     {
         int tmp = cl_;
         cl_     = cr_;
         cr_     = tmp;
     }
-    parent[cl_]     = (uint32_t)nid;
-    parent[cr_]     = (uint32_t)nid;
-    children[nid]   = make_uint2((uint32_t)cl_, (uint32_t)cr_);
-    object_idx[nid] = kInvalid;
+    parent[cl_]     = (uint32_t)gid;
+    parent[cr_]     = (uint32_t)gid;
+    children[gid]   = make_uint2((uint32_t)cl_, (uint32_t)cr_);
+    object_idx[gid] = kInvalid;
+    if (nid == 0) parent[gid] = kInvalid;
 }
 
 // update_*_leave_aabb (lbvh.cpp:636-683) + refit (lbvh.cpp:702-754) in one pass, for trees taller
 // than the level-ordered refit handles: the second child to arrive at an inner node (atomic
 // ticket) computes its box; fences publish boxes across XCD L2s.
-__global__ __launch_bounds__(kBlock) void k_refit(uint32_t n, int arity, const uint32_t* __restrict__ prims,
+__global__ __launch_bounds__(kBlock) void k_refit(uint32_t n, uint32_t ni, int arity, const uint32_t* __restrict__ prims,
                                                    const uint32_t* __restrict__ sorted_to_orig,
                                                    const float* __restrict__ x0, const float* __restrict__ x1,
                                                    const float* __restrict__ pad, const uint32_t* __restrict__ parent,
@@ -305,7 +366,6 @@ __global__ __launch_bounds__(kBlock) void k_refit(uint32_t n, int arity, const u
     }
     lo              = lo - mk(th, th, th);
     hi              = hi + mk(th, th, th);
-    uint32_t ni     = n - 1;
     uint32_t cur    = ni + l;
     NodeBox  box    = {make_float4(lo.x, lo.y, lo.z, 0.0f), make_float4(hi.x, hi.y, hi.z, 0.0f)};
     nodes[cur]      = box;
@@ -363,13 +423,12 @@ __global__ void k_fill_u32(uint32_t* a, uint32_t n, uint32_t v)
 // construction (with the ticket protocol); the top levels run in one workgroup.
 constexpr uint32_t kMaxLevels = 256;
 
-__global__ __launch_bounds__(kBlock) void k_height(uint32_t n, const uint32_t* __restrict__ parent,
+__global__ __launch_bounds__(kBlock) void k_height(uint32_t n, uint32_t ni, const uint32_t* __restrict__ parent,
                                                     const uint2* __restrict__ children, uint32_t* height,
                                                     uint32_t* counters)
 {
     uint32_t l = blockIdx.x * kBlock + threadIdx.x;
     if (l >= n) return;
-    uint32_t ni  = n - 1;
     uint32_t cur = ni + l;
     for (uint32_t depth = 0; depth < n; depth++)
     {
@@ -483,14 +542,14 @@ LCSX_D void inner_box(uint32_t p, uint32_t ni, const uint2* __restrict__ childre
     P[3]      = make_float4(B.hi.x, B.hi.y, B.hi.z, __uint_as_float(mb));
 }
 
-__global__ __launch_bounds__(kBlock) void k_refit_leaves(uint32_t n, int arity, const uint32_t* __restrict__ prims,
+__global__ __launch_bounds__(kBlock) void k_refit_leaves(uint32_t n, uint32_t ni, int arity, const uint32_t* __restrict__ prims,
                                                           const uint32_t* __restrict__ sorted_to_orig,
                                                           const float* __restrict__ x0, const float* __restrict__ x1,
                                                           const float* __restrict__ pad, NodeBox* __restrict__ nodes)
 {
     uint32_t l = blockIdx.x * kBlock + threadIdx.x;
     if (l >= n) return;
-    nodes[n - 1 + l] = leaf_box(l, arity, prims, sorted_to_orig, x0, x1, pad);
+    nodes[ni + l] = leaf_box(l, arity, prims, sorted_to_orig, x0, x1, pad);
 }
 
 __global__ __launch_bounds__(kBlock) void k_refit_level(const uint32_t* __restrict__ order, uint32_t b, uint32_t e,
@@ -550,7 +609,9 @@ LCSX_D void flush_pairs(const uint2* buf, uint32_t nbuf, uint32_t* count, uint32
 // step loads one packed inner node (both child boxes and codes); children are tested and inner
 // hits pushed in the reference's order. The wavefront loops until all its lanes are done, so
 // hits are staged in LDS and written with one atomic per kWaveBuf/2 or more pairs.
-__global__ __launch_bounds__(kQueryBlock) void k_query(QueryArgs a, const float4* __restrict__ packed, uint32_t n_inner)
+__global__ __launch_bounds__(kQueryBlock) void k_query(QueryArgs a, const float4* __restrict__ packed, uint32_t n_inner,
+                                                        const uint32_t* __restrict__ roots, const NodeBox* __restrict__ nodes,
+                                                        const uint2* __restrict__ children)
 {
     __shared__ uint32_t stack[kStack][kQueryBlock];
     __shared__ uint2    buf[kQueryBlock / 64][kWaveBuf];
@@ -584,8 +645,16 @@ __global__ __launch_bounds__(kQueryBlock) void k_query(QueryArgs a, const float4
             filt = a.edge_filter != 0;
         }
     }
+    uint32_t root = kInvalid;
+    if (active)
+    {
+        uint32_t g = 0;
+        if (a.vert_scene && a.mode != QueryBoxes) g = a.vert_scene[a.mode == QueryVerts ? q : a.edges[2 * q]];
+        root = roots[g];
+        if (root == kInvalid || (a.active && a.active[g] == 0)) active = false;
+    }
     int sp = 0;
-    if (active) stack[sp++][tid] = 0u;
+    if (active) stack[sp++][tid] = root;
     // each inner node is visited at most once per query in a well-formed tree; the cap (as the
     // reference's 100000-iteration guard) only bounds a corrupted one
     uint32_t visits = 0;
@@ -604,8 +673,26 @@ __global__ __launch_bounds__(kQueryBlock) void k_query(QueryArgs a, const float4
             else
             {
                 uint32_t      nd = stack[--sp][tid];
-                const float4* P  = packed + 4 * (size_t)nd;
-                float4        r0 = P[0], r1 = P[1], r2 = P[2], r3 = P[3];
+                float4        r0, r1, r2, r3;
+                if (nd < n_inner)
+                {
+                    const float4* P = packed + 4 * (size_t)nd;
+                    r0 = P[0];
+                    r1 = P[1];
+                    r2 = P[2];
+                    r3 = P[3];
+                }
+                else
+                {
+                    // a one-leaf segment's root: the leaf alone, as child 0 (child 1 never hits)
+                    NodeBox  B = nodes[nd];
+                    uint32_t o = children[nd].x;
+                    r0 = make_float4(B.lo.x, B.lo.y, B.lo.z, __uint_as_float(0x80000000u | o));
+                    r1 = make_float4(B.hi.x, B.hi.y, B.hi.z, __uint_as_float(o));
+                    const float inf = __int_as_float(0x7f800000);
+                    r2 = make_float4(inf, inf, inf, __uint_as_float(0x80000000u));
+                    r3 = make_float4(-inf, -inf, -inf, 0.0f);
+                }
                 uint32_t      c0 = __float_as_uint(r0.w), c1 = __float_as_uint(r2.w);
                 // with the edge filter (keep q < element) a subtree whose largest element id is
                 // <= q holds no pair: it is not descended (leaf: the largest id is the element)
@@ -659,13 +746,51 @@ __global__ __launch_bounds__(kQueryBlock) void k_query(QueryArgs a, const float4
 }
 }  // namespace
 
-void Lbvh::init(int arity_, const uint32_t* device_prims, uint32_t n_leaves)
+void Lbvh::init(int arity_, const uint32_t* device_prims, uint32_t n_leaves, const uint32_t* seg0, uint32_t nseg_)
 {
     arity = arity_;
     n     = n_leaves;
     prims = device_prims;
+    nseg  = std::max(1u, nseg_);
+    seg0_h.assign(nseg + 1, 0u);
+    if (seg0)
+        for (uint32_t g = 0; g <= nseg; g++) seg0_h[g] = seg0[g];
+    else
+        seg0_h[1] = n;
+    // inner-node bases, roots and per-leaf / per-inner-node segment ids
+    ib_h.assign(nseg, 0u);
+    roots_h.assign(nseg, kInvalid);
+    cover_h.assign(nseg, 0u);
+    std::vector<uint32_t> sol(n), isg;
+    uint32_t              nonempty = 0;
+    for (uint32_t g = 0; g < nseg; g++)
+    {
+        uint32_t L = seg0_h[g], ng = seg0_h[g + 1] - L;
+        for (uint32_t l = L; l < L + ng; l++) sol[l] = g;
+        ib_h[g] = L - nonempty;
+        if (ng) nonempty++;
+        for (uint32_t k = 0; k + 1 < ng; k++) isg.push_back(g);
+    }
+    ni = n - nonempty;
+    for (uint32_t g = 0; g < nseg; g++)
+    {
+        uint32_t ng = seg0_h[g + 1] - seg0_h[g];
+        if (ng >= 2) roots_h[g] = ib_h[g];
+        else if (ng == 1) roots_h[g] = ni + seg0_h[g];
+    }
     if (n == 0) return;
+    hipStream_t s0 = nullptr;
+    if (isg.empty()) isg.push_back(0u);
+    seg_of_leaf.upload(sol, s0);
+    inner_seg.upload(isg, s0);
+    roots.upload(roots_h, s0);
+    seg_tab.alloc(4 * (size_t)nseg);
     keys[0].alloc(n); keys[1].alloc(n); vals[0].alloc(n); vals[1].alloc(n);
+    if (nseg > 1)
+    {
+        khi[0].alloc(n);
+        khi[1].alloc(n);
+    }
     uint32_t nb = (n + kSortTile - 1) / kSortTile;
     hist.alloc(256 * (size_t)nb);
     center.alloc(n);
@@ -674,45 +799,70 @@ void Lbvh::init(int arity_, const uint32_t* device_prims, uint32_t n_leaves)
     object_idx.alloc(n_nodes());
     children.alloc(n_nodes());
     nodes.alloc(n_nodes());
-    packed.alloc(4 * (size_t)(n_inner() ? n_inner() : 1));
-    counters.alloc(n_inner() ? n_inner() : 1);
-    height.alloc(n_inner() ? n_inner() : 1);
-    order.alloc(n_inner() ? n_inner() : 1);
+    packed.alloc(4 * (size_t)(ni ? ni : 1));
+    counters.alloc(ni ? ni : 1);
+    height.alloc(ni ? ni : 1);
+    order.alloc(ni ? ni : 1);
     level_hist.alloc(kMaxLevels + 1);
     level_ptr.alloc(kMaxLevels + 2);
     level_cursor.alloc(kMaxLevels + 1);
-    gbox.alloc(6);
+    gbox.alloc(6 * (size_t)nseg);
     sorted_to_orig = vals[0].p;
+    LCSX_HIP(hipStreamSynchronize(s0));
 }
 
-void Lbvh::construct(const float* x, uint32_t n_verts_for_reduce, hipStream_t s)
+void Lbvh::construct(const float* x, const uint32_t* verts_per_seg, hipStream_t s)
 {
     if (n == 0) return;
-    uint64_t covered = std::min<uint64_t>(n, ((uint64_t)n_verts_for_reduce + 255u) / 256u * 256u);
-    k_init_gbox<<<1, 64, 0, s>>>(gbox.p);
-    k_leaf_centers<<<grid_for(n, kBlock), kBlock, 0, s>>>(n, arity, prims, x, (uint32_t)covered, center.p, gbox.p);
-    k_morton<<<grid_for(n, kBlock), kBlock, 0, s>>>(n, center.p, gbox.p, keys[0].p, vals[0].p);
+    std::vector<uint32_t>& tab = tab_h;  // kept: the copy below is asynchronous
+    tab.assign(4 * (size_t)nseg, 0u);
+    for (uint32_t g = 0; g < nseg; g++)
+    {
+        uint32_t ng = seg0_h[g + 1] - seg0_h[g];
+        cover_h[g]  = (uint32_t)std::min<uint64_t>(ng, ((uint64_t)verts_per_seg[g] + 255u) / 256u * 256u);
+        tab[4 * g] = seg0_h[g];
+        tab[4 * g + 1] = ng;
+        tab[4 * g + 2] = ib_h[g];
+        tab[4 * g + 3] = cover_h[g];
+    }
+    seg_tab.upload(tab, s);
+    k_init_gbox<<<grid_for(6 * nseg, kBlock), kBlock, 0, s>>>(gbox.p, nseg);
+    k_leaf_centers<<<grid_for(n, kBlock), kBlock, 0, s>>>(n, arity, prims, x, seg_of_leaf.p, seg_tab.p, center.p, gbox.p);
+    k_morton<<<grid_for(n, kBlock), kBlock, 0, s>>>(n, center.p, seg_of_leaf.p, gbox.p, keys[0].p, vals[0].p);
     uint32_t nb  = (n + kSortTile - 1) / kSortTile;
     int      cur = 0;
+    // four passes over the Morton key, then (several segments) passes over the segment id
     for (int shift = 0; shift < 32; shift += 8)
     {
         k_radix_hist<<<nb, kBlock, 0, s>>>(keys[cur].p, n, shift, hist.p, nb);
         k_scan_single<<<1, 1024, 0, s>>>(hist.p, 256 * nb);
-        k_radix_scatter<<<nb, kBlock, 0, s>>>(keys[cur].p, vals[cur].p, keys[1 - cur].p, vals[1 - cur].p, n, shift,
-                                               hist.p, nb);
+        k_radix_scatter<<<nb, kBlock, 0, s>>>(keys[cur].p, vals[cur].p, keys[1 - cur].p, vals[1 - cur].p, nullptr,
+                                               nullptr, keys[cur].p, n, shift, hist.p, nb);
         cur = 1 - cur;
     }
+    if (nseg > 1)
+    {
+        k_seg_keys<<<grid_for(n, kBlock), kBlock, 0, s>>>(n, vals[cur].p, seg_of_leaf.p, khi[cur].p);
+        for (int shift = 0; shift < 32 && ((nseg - 1) >> shift) != 0u; shift += 8)
+        {
+            k_radix_hist<<<nb, kBlock, 0, s>>>(khi[cur].p, n, shift, hist.p, nb);
+            k_scan_single<<<1, 1024, 0, s>>>(hist.p, 256 * nb);
+            k_radix_scatter<<<nb, kBlock, 0, s>>>(keys[cur].p, vals[cur].p, keys[1 - cur].p, vals[1 - cur].p,
+                                                   khi[cur].p, khi[1 - cur].p, khi[cur].p, n, shift, hist.p, nb);
+            cur = 1 - cur;
+        }
+    }
     sorted_to_orig = vals[cur].p;
-    k_leaves<<<grid_for(n, kBlock), kBlock, 0, s>>>(n, keys[cur].p, vals[cur].p, morton_sorted.p, children.p,
-                                                    object_idx.p, parent.p);
-    if (n > 1) k_karras<<<grid_for(n - 1, kBlock), kBlock, 0, s>>>(n, morton_sorted.p, parent.p, children.p, object_idx.p);
+    k_leaves<<<grid_for(n, kBlock), kBlock, 0, s>>>(n, ni, keys[cur].p, vals[cur].p, seg_of_leaf.p, seg_tab.p,
+                                                    morton_sorted.p, children.p, object_idx.p, parent.p);
+    if (ni) k_karras<<<grid_for(ni, kBlock), kBlock, 0, s>>>(ni, morton_sorted.p, inner_seg.p, seg_tab.p, parent.p,
+                                                            children.p, object_idx.p);
     LCSX_HIP(hipGetLastError());
     use_levels = false;
-    if (n < 2) return;
+    if (ni == 0) return;
     // node heights and the height-sorted order of inner nodes (one host read of the histogram)
-    const uint32_t ni = n - 1;
     k_fill_u32<<<grid_for(ni, kBlock), kBlock, 0, s>>>(counters.p, ni, 0u);
-    k_height<<<grid_for(n, kBlock), kBlock, 0, s>>>(n, parent.p, children.p, height.p, counters.p);
+    k_height<<<grid_for(n, kBlock), kBlock, 0, s>>>(n, ni, parent.p, children.p, height.p, counters.p);
     k_fill_u32<<<grid_for(kMaxLevels + 1, kBlock), kBlock, 0, s>>>(level_hist.p, kMaxLevels + 1, 0u);
     k_level_hist<<<grid_for(ni, kBlock), kBlock, 0, s>>>(ni, height.p, level_hist.p);
     std::vector<uint32_t> hist(kMaxLevels + 1);
@@ -740,10 +890,10 @@ void Lbvh::construct(const float* x, uint32_t n_verts_for_reduce, hipStream_t s)
 void Lbvh::refit(const float* x0, const float* x1, const float* pad, hipStream_t s)
 {
     if (n == 0) return;
-    if (use_levels)
+    if (use_levels || ni == 0)
     {
-        const uint32_t ni = n - 1;
-        k_refit_leaves<<<grid_for(n, kBlock), kBlock, 0, s>>>(n, arity, prims, sorted_to_orig, x0, x1, pad, nodes.p);
+        k_refit_leaves<<<grid_for(n, kBlock), kBlock, 0, s>>>(n, ni, arity, prims, sorted_to_orig, x0, x1, pad, nodes.p);
+        if (ni == 0) return;
         for (uint32_t h = 1; h < tail_level; h++)
         {
             uint32_t b = level_ptr_h[h], e = level_ptr_h[h + 1];
@@ -755,7 +905,7 @@ void Lbvh::refit(const float* x0, const float* x1, const float* pad, hipStream_t
         return;
     }
     k_fill_u32<<<grid_for((uint32_t)counters.n, kBlock), kBlock, 0, s>>>(counters.p, (uint32_t)counters.n, 0u);
-    k_refit<<<grid_for(n, kBlock), kBlock, 0, s>>>(n, arity, prims, sorted_to_orig, x0, x1, pad, parent.p, children.p,
+    k_refit<<<grid_for(n, kBlock), kBlock, 0, s>>>(n, ni, arity, prims, sorted_to_orig, x0, x1, pad, parent.p, children.p,
                                                    nodes.p, packed.p, counters.p);
     LCSX_HIP(hipGetLastError());
 }
@@ -763,8 +913,8 @@ void Lbvh::refit(const float* x0, const float* x1, const float* pad, hipStream_t
 void lbvh_query(const Lbvh& t, const QueryArgs& a, hipStream_t s)
 {
     if (t.n == 0 || a.n_queries == 0) return;
-    if (t.n < 2) return;  // a single leaf has no inner node to start from (the solver never queries one)
-    k_query<<<grid_for(a.n_queries, kQueryBlock), kQueryBlock, 0, s>>>(a, t.packed.p, t.n_inner());
+    k_query<<<grid_for(a.n_queries, kQueryBlock), kQueryBlock, 0, s>>>(a, t.packed.p, t.ni, t.roots.p, t.nodes.p,
+                                                                        t.children.p);
     LCSX_HIP(hipGetLastError());
 }
 

@@ -176,6 +176,40 @@ void Setup::build(const std::vector<MeshDesc>& meshes)
     ne = (uint32_t)(edges.size() / 2);
     nh = (uint32_t)(hinges.size() / 4);
 
+    // scene ranges (meshes arrive in non-decreasing scene order) and scene-aligned row blocks
+    nscenes = meshes.empty() ? 1u : meshes.back().scene + 1u;
+    sc_v0.assign(nscenes + 1, 0);
+    sc_f0.assign(nscenes + 1, 0);
+    sc_e0.assign(nscenes + 1, 0);
+    for (size_t mi = 0; mi < meshes.size(); mi++)
+    {
+        uint32_t sc = meshes[mi].scene;
+        sc_v0[sc + 1] += (uint32_t)(meshes[mi].x.size() / 3);
+    }
+    for (uint32_t f = 0; f < nf; f++) sc_f0[meshes[face_mesh[f]].scene + 1]++;
+    for (uint32_t e = 0; e < ne; e++) sc_e0[meshes[edge_mesh[e]].scene + 1]++;
+    for (uint32_t sc = 0; sc < nscenes; sc++)
+    {
+        sc_v0[sc + 1] += sc_v0[sc];
+        sc_f0[sc + 1] += sc_f0[sc];
+        sc_e0[sc + 1] += sc_e0[sc];
+    }
+    vert_scene.assign(nv, 0);
+    sc_rb0.assign(nscenes + 1, 0);
+    rb_scene.clear();
+    rb_row0.clear();
+    for (uint32_t sc = 0; sc < nscenes; sc++)
+    {
+        for (uint32_t i = sc_v0[sc]; i < sc_v0[sc + 1]; i++) vert_scene[i] = sc;
+        for (uint32_t r = sc_v0[sc]; r < sc_v0[sc + 1]; r += 256u)
+        {
+            rb_scene.push_back(sc);
+            rb_row0.push_back(r);
+        }
+        sc_rb0[sc + 1] = (uint32_t)rb_scene.size();
+    }
+    n_rb = (uint32_t)rb_scene.size();
+
     // areas: faces in float, vertices and edges accumulate face/3 in double
     face_area.resize(nf);
     std::vector<std::vector<uint32_t>> v_faces(nv);

@@ -18,6 +18,7 @@ struct MeshDesc
     std::vector<uint32_t>         pinned;  // local ids in selection order
     std::vector<lcsx_fixed_anim>  anim;
     std::vector<float>            explicit_targets;  // optional (3 per pinned vertex)
+    uint32_t                      scene = 0;  // batch member (meshes are added scene by scene)
 };
 
 constexpr int kSlice = 64;  // SELL slice height (one wavefront)
@@ -50,6 +51,13 @@ struct Setup
     // per SELL block: contributors (bit 31 hinge, bits 29-30 local a, 27-28 local b, 0-26 element)
     std::vector<uint32_t> blk_ptr, blk_contrib;
     uint32_t              max_degree = 0;
+    // batch of independent scenes: vertices, faces and edges of scene s are the ranges
+    // [sc_v0[s], sc_v0[s+1]) etc. Row blocks of 256 vertices never straddle two scenes, so every
+    // per-block partial sum belongs to one scene: block b covers rows [rb_row0[b], ..) of scene
+    // rb_scene[b], and scene s owns blocks [sc_rb0[s], sc_rb0[s+1]).
+    uint32_t              nscenes = 1;
+    std::vector<uint32_t> sc_v0, sc_f0, sc_e0, sc_rb0, vert_scene, rb_scene, rb_row0;
+    uint32_t              n_rb = 0;
 
     void build(const std::vector<MeshDesc>& meshes);
 };

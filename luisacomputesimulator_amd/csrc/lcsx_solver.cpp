@@ -85,7 +85,9 @@ void Solver::ev_put(hipEvent_t e) { g_pool->free_.push_back(e); }
 
 void Solver::read_status()
 {
+    host_sst.resize(S.nscenes);
     LCSX_HIP(hipMemcpyAsync(&host_status, status.p, sizeof(Status), hipMemcpyDeviceToHost, stream));
+    LCSX_HIP(hipMemcpyAsync(host_sst.data(), sst.p, S.nscenes * sizeof(SceneStatus), hipMemcpyDeviceToHost, stream));
     LCSX_HIP(hipStreamSynchronize(stream));
     flush_events();
 }
@@ -125,8 +127,11 @@ void Solver::alloc_collision(uint32_t vf, uint32_t ee, uint32_t nc)
     ccd_rest_vf.alloc(cap_vf);
     ccd_rest_ee.alloc(cap_ee);
     vc_list.alloc(4 * (size_t)cap_c);
-    // per-block partial sums of the SpMV: row blocks + slot blocks (4 slots per contact)
-    part_pq.alloc(grid_for(S.nv, 256) + (4 * (size_t)cap_c + 255) / 256 + 1);
+    // per-block partial sums of the SpMV: row blocks + scene-aligned slot chunks (4 slots per contact)
+    const size_t chunks = max_chunks(4 * cap_c) + 1;
+    part_pq.alloc(S.n_rb + chunks);
+    cb_scene_d.alloc(chunks);
+    cb_slot0_d.alloc(chunks);
     vc_row.alloc(4 * (size_t)cap_c);
     vc_coef.alloc(4 * (size_t)cap_c);
     vc_tc.alloc(cap_c);
@@ -152,24 +157,44 @@ void Solver::init()
     minv.alloc(9 * (size_t)nv);
     sell_val.alloc(9 * (size_t)std::max(S.nblocks, 1u));
     vc_qc.alloc(3 * (size_t)nv);
-    part_rz.alloc(grid_for(nv, 256) + 1);
+    part_rz.alloc(S.n_rb + 1);
+    // scene tables
+    const uint32_t ns = S.nscenes;
+    sc_v0_d.upload(S.sc_v0, stream);
+    sc_rb0_d.upload(S.sc_rb0, stream);
+    rb_scene_d.upload(S.rb_scene, stream);
+    rb_row0_d.upload(S.rb_row0, stream);
+    vert_scene_d.upload(S.vert_scene, stream);
+    sc_cb0_d.alloc(ns + 1);
+    sc_cb0_zero.alloc(ns + 1);
+    LCSX_HIP(hipMemsetAsync(sc_cb0_d.p, 0, (ns + 1) * sizeof(uint32_t), stream));
+    LCSX_HIP(hipMemsetAsync(sc_cb0_zero.p, 0, (ns + 1) * sizeof(uint32_t), stream));
+    active_h.assign(ns, 1);
+    active_d.upload(active_h, stream);
+    sst.alloc(ns);
+    host_sst.assign(ns, SceneStatus{});
+    host_pcg.assign(ns, PcgState{});
+    scene_stats.assign(ns, lcsx_stats{});
+    verts_per_scene.assign(ns, 0);
+    for (uint32_t sc = 0; sc < ns; sc++) verts_per_scene[sc] = S.sc_v0[sc + 1] - S.sc_v0[sc];
     if (prm.precision == 1)
     {
         sell_h.alloc(9 * (size_t)std::max(S.nblocks, 1u));
         sell_e.alloc(std::max(S.nblocks, 1u));
-        part_res.alloc(2 * (size_t)grid_for(nv, 256));
+        part_res.alloc(2 * (size_t)S.n_rb);
     }
     upload_state_constants();
-    tree_face.init(3, faces.p, S.nf);
-    tree_edge.init(2, edges.p, S.ne);
+    tree_face.init(3, faces.p, S.nf, S.sc_f0.data(), ns);
+    tree_edge.init(2, edges.p, S.ne, S.sc_e0.data(), ns);
     alloc_collision(24 * nv, 24 * S.ne, 12 * nv);
     vc_cnt.alloc(nv + 1);
     ccd_rest_count.alloc(2);
     vc_ptr.alloc(nv + 1);
     status.alloc(1);
-    pcg_state.alloc(1);
-    LCSX_HIP(hipMemsetAsync(pcg_state.p, 0, sizeof(PcgState), stream));
+    pcg_state.alloc(ns);
+    LCSX_HIP(hipMemsetAsync(pcg_state.p, 0, ns * sizeof(PcgState), stream));
     LCSX_HIP(hipMemsetAsync(status.p, 0, sizeof(Status), stream));
+    LCSX_HIP(hipMemsetAsync(sst.p, 0, ns * sizeof(SceneStatus), stream));
     host_status = Status{};
     LCSX_HIP(hipMemsetAsync(sell_val.p, 0, sell_val.n * sizeof(float), stream));
     explicit_targets.assign(3 * S.fixed_verts.size(), 0.0f);
@@ -241,6 +266,23 @@ VertContacts Solver::vert_contacts()
     return VertContacts{vc_ptr.p, vc_list.p, vc_row.p, vc_coef.p, vc_tc.p, 4u * nc};
 }
 // The SpMV's diagonal is the material one: contact diagonal blocks enter through the contact slots.
+SegView Solver::seg_view(bool with_chunks) const
+{
+    SegView g{};
+    g.nscenes    = S.nscenes;
+    g.n_rb       = S.n_rb;
+    g.sc_v0      = sc_v0_d.p;
+    g.sc_rb0     = sc_rb0_d.p;
+    g.rb_scene   = rb_scene_d.p;
+    g.rb_row0    = rb_row0_d.p;
+    g.vert_scene = vert_scene_d.p;
+    g.active     = active_d.p;
+    g.sc_cb0     = with_chunks ? sc_cb0_d.p : sc_cb0_zero.p;
+    g.cb_scene   = cb_scene_d.p;
+    g.cb_slot0   = cb_slot0_d.p;
+    return g;
+}
+
 SellView Solver::sell_view() const { return SellView{sl_off.p, sell_col.p, sell_val.p, diag_mat.p, S.nblocks, nullptr, nullptr}; }
 SellView Solver::sell_view_lp() const
 {
@@ -269,8 +311,8 @@ void Solver::frame_begin()
     }
     LCSX_HIP(hipMemcpyAsync(x_step.p, x.p, x.n * sizeof(float), hipMemcpyDeviceToDevice, stream));
     tic();
-    if (S.nf > 1) tree_face.construct(x_step.p, S.nv, stream);
-    if (S.ne > 1) tree_edge.construct(x_step.p, S.nv, stream);
+    tree_face.construct(x_step.p, verts_per_scene.data(), stream);
+    tree_edge.construct(x_step.p, verts_per_scene.data(), stream);
     toc("lbvh_build");
 }
 
@@ -282,7 +324,7 @@ void Solver::predict()
 void Solver::iter_begin()
 {
     LCSX_HIP(hipMemcpyAsync(x_iter.p, x.p, x.n * sizeof(float), hipMemcpyDeviceToDevice, stream));
-    launch_reset_status(status.p, stream);
+    launch_reset_status(status.p, sst.p, S.nscenes, stream);
 }
 
 void Solver::material()
@@ -326,10 +368,12 @@ void Solver::broad_phase(bool ccd)
     tic();
     const float* x0 = ccd ? x_iter.p : x.p;
     Status*      st = status.p;
-    if (S.nf > 1)
+    if (S.nf > 0)
     {
         tree_face.refit(x0, x.p, offset.p, stream);
         QueryArgs q{};
+        q.vert_scene = vert_scene_d.p;
+        q.active     = active_d.p;
         q.mode      = QueryVerts;
         q.n_queries = S.nv;
         q.x0        = x0;
@@ -342,10 +386,12 @@ void Solver::broad_phase(bool ccd)
         q.overflow  = &st->overflow;
         lbvh_query(tree_face, q, stream);
     }
-    if (S.ne > 1)
+    if (S.ne > 0)
     {
         tree_edge.refit(x0, x.p, offset.p, stream);
         QueryArgs q{};
+        q.vert_scene = vert_scene_d.p;
+        q.active     = active_d.p;
         q.mode      = QueryEdges;
         q.n_queries = S.ne;
         q.x0        = x0;
@@ -406,7 +452,7 @@ void Solver::contact_assemble()
     if (!prm.use_self_collision) return;
     tic();
     launch_contact_adjacency(contact_view(), S.nv, pair_grid(cap_c), vc_cnt.p, vc_ptr.p, vc_list.p, vc_row.p, vc_coef.p,
-                             stream);
+                             seg_view(true), sc_cb0_d.p, cb_scene_d.p, cb_slot0_d.p, stream);
     vc_ready = true;
     VertContacts vc{vc_ptr.p, vc_list.p, vc_row.p, vc_coef.p, vc_tc.p, 0u};  // assembly walks the per-vertex lists
     launch_contact_assemble(contact_view(), vc, cgB.p, diag.p, S.nv, stream);
@@ -414,12 +460,15 @@ void Solver::contact_assemble()
 }
 
 int Solver::pcg_run(PcgArgs& a, const SellView& A, const ContactView& cv, const VertContacts& vc, int max_iters,
-                    int replace_every, PcgState& hs)
+                    int replace_every)
 {
+    const uint32_t ns = S.nscenes;
+    const unsigned mc = vc.ptr ? max_chunks(vc.nslots) : 0u;
     launch_pcg_init(a, stream);
-    hs         = PcgState{};
-    int issued = 0;
-    int par    = 0;
+    host_pcg.assign(ns, PcgState{});
+    int  issued = 0;
+    int  par    = 0;
+    bool nan    = false;
     while (issued < max_iters)
     {
         // one host check per convergence-test interval (at least kPcgChunk launches), chunks
@@ -439,8 +488,8 @@ int Solver::pcg_run(PcgArgs& a, const SellView& A, const ContactView& cv, const 
                 e1 = ev_get();
                 spmv_events.push_back({e0, e1});
             }
-            if (vc.ptr) launch_pcg_contact_t(b, cv, vc.tc, pair_grid(cap_c), stream);
-            launch_pcg_spmv(b, A, vc, stream, e0, e1);
+            if (vc.ptr) launch_pcg_contact_t(b, cv, vc, mc, stream);
+            launch_pcg_spmv(b, A, vc, mc, stream, e0, e1);
             LCSX_HIP(hipGetLastError());
             launch_pcg_update(b, stream);
             par = 1 - par;
@@ -452,9 +501,9 @@ int Solver::pcg_run(PcgArgs& a, const SellView& A, const ContactView& cv, const 
             // r = b - A x with the FP32 blocks; the next iteration's r.z (and its stopping test)
             // is this residual's. After convergence x no longer changes and this is harmless.
             launch_spmv(sell_view(), cv, vc, pair_grid(cap_c), a.x, spmv_tmp.p, S.nv, stream);
-            launch_residual(S.nv, a.b, spmv_tmp.p, a.minv, a.r, a.z, a.part_rz, part_res.p, stream);
+            launch_residual(a.seg, a.b, spmv_tmp.p, a.minv, a.r, a.z, a.part_rz, part_res.p, stream);
         }
-        LCSX_HIP(hipMemcpyAsync(&hs, pcg_state.p, sizeof(PcgState), hipMemcpyDeviceToHost, stream));
+        LCSX_HIP(hipMemcpyAsync(host_pcg.data(), pcg_state.p, ns * sizeof(PcgState), hipMemcpyDeviceToHost, stream));
         std::vector<float> res;
         if (replace)
         {
@@ -463,25 +512,45 @@ int Solver::pcg_run(PcgArgs& a, const SellView& A, const ContactView& cv, const 
         }
         LCSX_HIP(hipStreamSynchronize(stream));
         flush_events();
-        if (!hs.converged && !res.empty())
+        bool done = true;
+        for (const PcgState& h : host_pcg)
+        {
+            done = done && h.converged;
+            nan  = nan || h.nan;
+        }
+        if (!done && !res.empty())
         {
             double t = 0.0;
             for (size_t k = 0; k < res.size(); k += 2) t += (double)res[k] + (double)res[k + 1];
             rz_true = t;
         }
-        if (hs.converged || hs.nan) break;
+        if (done || nan) break;
     }
-    if (hs.nan)
+    if (nan)
     {
         set_error("NaN/Inf in the PCG iteration");
         throw HipFailure();
     }
-    // iterations run: the stopping one when the test stopped the loop, else the cap
-    const uint32_t iters = hs.converged ? hs.iters : (uint32_t)issued;
-    account_spmv(iters, vc.nslots);
-    // r.z of the last evaluated iteration kept in hs.rz[0]
-    hs.rz[0] = hs.rz[(hs.converged ? hs.iters : (uint32_t)std::max(issued - 1, 0)) & 1u];
-    return (int)iters;
+    // per scene: iterations run (the stopping one when the test stopped it, else the cap) and
+    // r.z of its last evaluated iteration
+    int most = 0;
+    for (uint32_t sc = 0; sc < ns; sc++)
+    {
+        const PcgState& h  = host_pcg[sc];
+        const bool      on = active_h.empty() || active_h[sc];
+        const int       it = !on ? 0 : (h.converged ? (int)h.iters : issued);
+        const float     rl = h.rz[(h.converged ? h.iters : (uint32_t)std::max(issued - 1, 0)) & 1u];
+        if (on)
+        {
+            lcsx_stats& st = scene_stats[sc];
+            st.pcg_iters   = it;
+            st.pcg_iters_total += it;
+            st.pcg_rel     = h.rz0 > 0.0f ? std::sqrt(rl / h.rz0) : 0.0f;
+        }
+        most = std::max(most, it);
+    }
+    account_spmv((uint32_t)most, vc.nslots);
+    return most;
 }
 
 // Timed SpMV launches: only the first `iters` did work (later ones exit on the converged flag).
@@ -511,14 +580,17 @@ void Solver::account_spmv(uint32_t iters, uint32_t nslots)
 }
 
 // ConjugateGradientSolver (pcg_solver.cpp:15-197), with the stopping test of the reference on
-// r.z. FP32: every block in FP32. Mixed precision: the SpMV inside the iteration reads FP16
-// blocks (half the bytes); x, r, z, p stay FP32, and after every kMixedSweep iterations the
-// recursive residual is replaced by the true residual b - A x computed with the FP32 blocks and
-// a compensated r.z (residual replacement: the iteration keeps its search direction, so the
-// Krylov progress made so far is not thrown away as a restarted refinement would).
+// r.z, per scene of the batch. FP32: every block in FP32. Mixed precision: the SpMV inside the
+// iteration reads FP16 blocks (half the bytes); x, r, z, p stay FP32, and after every
+// kMixedSweep iterations the recursive residual is replaced by the true residual b - A x computed
+// with the FP32 blocks and a compensated r.z (residual replacement: the iteration keeps its search
+// direction, so the Krylov progress made so far is not thrown away as a restarted refinement
+// would).
 void Solver::pcg()
 {
     tic();
+    ContactView  cv = contact_view();
+    VertContacts vc = vert_contacts();
     PcgArgs a{};
     a.n        = S.nv;
     a.b        = cgB.p;
@@ -536,23 +608,16 @@ void Solver::pcg()
     a.rel2     = prm.pcg_rel_tol * prm.pcg_rel_tol;
     a.abs_tol  = prm.pcg_abs_tol;
     a.check_every = (uint32_t)std::max(1, prm.pcg_check_every);
-    ContactView  cv = contact_view();
-    VertContacts vc = vert_contacts();
-    a.qc            = vc.ptr ? vc_qc.p : nullptr;
-    a.n_rz          = grid_for(S.nv, 256);                                   // init / update blocks
-    a.n_pq          = a.n_rz + (vc.ptr ? grid_for(vc.nslots, 256) : 0u);    // SpMV row + slot blocks
-    PcgState   hs{};
+    a.qc       = vc.ptr ? vc_qc.p : nullptr;
+    a.seg      = seg_view(vc.ptr != nullptr);
     const bool mixed = prm.precision == 1;
     // replacements at multiples of the test interval, so the test after one sees the true residual
     const int every = (int)a.check_every;
     const int repl  = mixed ? every * ((kMixedSweep + every - 1) / every) : 0;
     rz_true         = -1.0;
-    const int iters = pcg_run(a, mixed ? sell_view_lp() : sell_view(), cv, vc, prm.pcg_iter_count, repl, hs);
+    pcg_run(a, mixed ? sell_view_lp() : sell_view(), cv, vc, prm.pcg_iter_count, repl);
     toc("pcg");
-    stats.pcg_iters = iters;
-    stats.pcg_iters_total += iters;
-    stats.pcg_rel   = hs.rz0 > 0.0f ? std::sqrt(hs.rz[0] / hs.rz0) : 0.0f;
-    stats.pcg_rz_true = rz_true;
+    fold_stats();
 }
 
 void Solver::ccd()
@@ -568,6 +633,8 @@ void Solver::ccd()
     l.floor_y   = prm.floor_y;
     l.use_floor = prm.use_floor && prm.use_ccd_linesearch;
     l.st        = status.p;
+    l.sst       = sst.p;
+    l.seg       = seg_view(false);
     launch_ground_ccd_maxdx(l, stream);
     if (prm.use_ccd_linesearch && prm.use_self_collision)
     {
@@ -577,7 +644,8 @@ void Solver::ccd()
         c.faces    = faces.p;
         c.edges    = edges.p;
         c.offset   = offset.p;
-        c.toi_bits = &status.p->toi_bits;
+        c.sst      = sst.p;
+        c.vert_scene = vert_scene_d.p;
         c.fp64     = prm.precision == 1;
         c.budget   = kCcdFirstPassIters;
         launch_fill_u32(ccd_rest_count.p, 2, 0u, stream);
@@ -616,12 +684,13 @@ bool Solver::grow_if_needed(bool ccd_only)
 // NewtonSolver::line_search, newton_solver.cpp:3685-3884 (CCD step bound, no energy backtracking)
 void Solver::line_search()
 {
-    launch_reset_line_search(status.p, stream);
-    launch_apply_dx(S.nv, x_iter.p, dx.p, x.p, nullptr, 1.0f, stream);
+    const SegView g = seg_view(false);
+    launch_reset_line_search(status.p, sst.p, S.nscenes, stream);
+    launch_apply_dx(S.nv, x_iter.p, dx.p, x.p, nullptr, 1.0f, g, stream);
     if (prm.use_ccd_linesearch && prm.use_self_collision) broad_phase(true);
     ccd();
-    launch_apply_dx(S.nv, x_iter.p, dx.p, x.p, &status.p->toi_bits, 1.0f, stream);
-    launch_pin_delta((uint32_t)S.fixed_verts.size(), fixed_ids.p, x.p, x_tilde.p, &status.p->pin_bits, stream);
+    launch_apply_dx(S.nv, x_iter.p, dx.p, x.p, sst.p, 1.0f, g, stream);
+    launch_pin_delta((uint32_t)S.fixed_verts.size(), fixed_ids.p, x.p, x_tilde.p, sst.p, vert_scene_d.p, stream);
 }
 
 bool Solver::newton_iteration()
@@ -644,19 +713,57 @@ bool Solver::newton_iteration()
         LCSX_HIP(hipMemcpyAsync(x.p, x_iter.p, x.n * sizeof(float), hipMemcpyDeviceToDevice, stream));
         return false;
     }
-    stats.toi        = ord2f(host_status.toi_bits);
-    stats.max_dx     = __builtin_bit_cast(float, host_status.maxdx_bits);
-    stats.pin_delta  = S.fixed_verts.empty() ? -1e9f : __builtin_bit_cast(float, host_status.pin_bits);
+    std::vector<uint32_t> pin_scene(S.nscenes, 0u);
+    for (uint32_t v : S.fixed_verts) pin_scene[S.vert_scene[v]] = 1u;
+    for (uint32_t sc = 0; sc < S.nscenes; sc++)
+    {
+        if (!active_h[sc]) continue;
+        lcsx_stats& st = scene_stats[sc];
+        st.toi         = ord2f(host_sst[sc].toi_bits);
+        st.max_dx      = __builtin_bit_cast(float, host_sst[sc].maxdx_bits);
+        st.pin_delta   = pin_scene[sc] ? __builtin_bit_cast(float, host_sst[sc].pin_bits) : -1e9f;
+        if (!(st.toi > 0.0f && st.toi <= 1.0f))
+        {
+            set_error("invalid CCD step bound " + std::to_string(st.toi) + " in scene " + std::to_string(sc));
+            throw HipFailure();
+        }
+    }
     stats.num_pairs_vf = (int)host_status.n_vf;
     stats.num_pairs_ee = (int)host_status.n_ee;
     stats.num_contacts = (int)host_status.n_contacts;
     stats.broadphase_overflow |= host_status.overflow ? 1 : 0;
-    if (!(stats.toi > 0.0f && stats.toi <= 1.0f))
-    {
-        set_error("invalid CCD step bound " + std::to_string(stats.toi));
-        throw HipFailure();
-    }
+    fold_stats();
     return true;
+}
+
+// Batch totals into stats: sums of the per-scene iteration counts, the smallest step bound and the
+// largest |dx| and pin distance of the scenes still iterating (one scene: that scene's values).
+void Solver::fold_stats()
+{
+    int   pcg_last = 0, pcg_total = 0, newton = 0, conv = 1;
+    float toi = 1.0f, mdx = 0.0f, pin = -1e9f, rel = 0.0f;
+    for (uint32_t sc = 0; sc < S.nscenes; sc++)
+    {
+        const lcsx_stats& st = scene_stats[sc];
+        pcg_total += st.pcg_iters_total;
+        newton += st.newton_iters;
+        conv = conv && st.converged;
+        if (!active_h.empty() && !active_h[sc]) continue;
+        pcg_last = std::max(pcg_last, st.pcg_iters);
+        toi      = std::min(toi, st.toi);
+        mdx      = std::max(mdx, st.max_dx);
+        pin      = std::max(pin, st.pin_delta);
+        rel      = std::max(rel, st.pcg_rel);
+    }
+    stats.pcg_iters       = pcg_last;
+    stats.pcg_iters_total = pcg_total;
+    stats.pcg_rel         = rel;
+    stats.toi             = toi;
+    stats.max_dx          = mdx;
+    stats.pin_delta       = pin;
+    stats.newton_iters    = newton;
+    stats.converged       = conv;
+    stats.pcg_rz_true     = rz_true;
 }
 
 void Solver::frame_end()
@@ -667,33 +774,67 @@ void Solver::frame_end()
     stats.frame = frame;
 }
 
+// physics_step_GPU (newton_solver.cpp:4051-4345), each scene of the batch with its own Newton
+// loop: a scene stops on `converged || (iter >= nonlinear_iter_count && dirichlet)` or at the
+// iteration cap; stopped scenes keep their positions (their step is 0) while the others iterate.
 void Solver::step()
 {
     frame_begin();
     predict();
-    bool converged = false, dirichlet = false;
-    int  it        = 0;
-    stats.pcg_iters_total     = 0;
+    const uint32_t       ns = S.nscenes;
+    std::vector<uint8_t> conv(ns, 0), dir(ns, 0);
+    active_h.assign(ns, 1);
+    for (auto& st : scene_stats)
+    {
+        st.pcg_iters_total     = 0;
+        st.broadphase_overflow = 0;
+        st.newton_iters        = 0;
+        st.converged           = 0;
+    }
     stats.broadphase_overflow = 0;
+    int it = 0;
     for (it = 0; it < prm.newton_max_iters; it++)
     {
-        if (converged || (it >= prm.nonlinear_iter_count && dirichlet)) break;
+        bool any = false;
+        for (uint32_t sc = 0; sc < ns; sc++)
+        {
+            if (!active_h[sc]) continue;
+            if (conv[sc] || (it >= prm.nonlinear_iter_count && dir[sc]))
+            {
+                active_h[sc]                 = 0;
+                scene_stats[sc].newton_iters = it;
+            }
+            else
+                any = true;
+        }
+        if (!any) break;
+        active_d.upload(active_h, stream);
         while (!newton_iteration())
         {
         }
-        if (stats.max_dx < prm.newton_tol * prm.implicit_dt)
+        for (uint32_t sc = 0; sc < ns; sc++)
         {
-            converged = dirichlet = true;
-        }
-        else
-        {
-            converged = false;
-            dirichlet = stats.pin_delta < 1e-3f;
+            if (!active_h[sc]) continue;
+            const lcsx_stats& st = scene_stats[sc];
+            if (st.max_dx < prm.newton_tol * prm.implicit_dt)
+                conv[sc] = dir[sc] = 1;
+            else
+            {
+                conv[sc] = 0;
+                dir[sc]  = st.pin_delta < 1e-3f;
+            }
         }
     }
-    stats.newton_iters = it;
-    stats.converged    = converged ? 1 : 0;
+    for (uint32_t sc = 0; sc < ns; sc++)
+    {
+        if (active_h[sc]) scene_stats[sc].newton_iters = it;
+        scene_stats[sc].converged = conv[sc];
+    }
+    active_h.assign(ns, 1);  // every scene's stats count in the totals
+    active_d.upload(active_h, stream);
+    fold_stats();
     frame_end();
+    for (auto& st : scene_stats) st.frame = frame;
     LCSX_HIP(hipStreamSynchronize(stream));
 }
 

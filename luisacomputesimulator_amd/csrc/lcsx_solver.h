@@ -53,8 +53,19 @@ struct Solver
     DBuf<uint32_t> ccd_rest_vf, ccd_rest_ee, ccd_rest_count;  // CCD pairs left for the full-budget pass
     bool           vc_ready = false;
     DBuf<Status>   status;
-    DBuf<PcgState> pcg_state;
+    DBuf<PcgState> pcg_state;  // per scene
     Status         host_status{};
+    // batch of scenes: segmentation tables (lcsx_kernels.h SegView), the Newton loops' active
+    // mask, per-scene line-search results and statistics
+    DBuf<uint32_t>           sc_v0_d, sc_rb0_d, rb_scene_d, rb_row0_d, vert_scene_d, sc_cb0_d, sc_cb0_zero,
+                             cb_scene_d, cb_slot0_d;
+    DBuf<uint8_t>            active_d;
+    std::vector<uint8_t>     active_h;
+    DBuf<SceneStatus>        sst;
+    std::vector<SceneStatus> host_sst;
+    std::vector<PcgState>    host_pcg;
+    std::vector<lcsx_stats>  scene_stats;
+    std::vector<uint32_t>    verts_per_scene;
     int            last_bp = 0;  // 0 none, 1 DCD, 2 CCD
     int            pcg_parity = 0;
     std::vector<float> explicit_targets;  // per fixed vertex (3), when set
@@ -90,13 +101,16 @@ struct Solver
     bool grow_if_needed(bool ccd_only);
 
     ContactView contact_view() const;
+    SegView     seg_view(bool with_chunks) const;  // with_chunks: the contact-slot chunk tables
+    unsigned    max_chunks(uint32_t nslots) const { return nslots / 256u + S.nscenes; }
+    void        fold_stats();                       // per-scene statistics into stats
     SellView    sell_view() const;
     SellView    sell_view_lp() const;  // FP16 blocks (mixed precision)
     // PCG iterations on a.b from x = 0, at most max_iters; with replace_every > 0 the recursive
     // residual is replaced by the true one (FP32 blocks) after every replace_every iterations.
     // Returns the iterations run.
     int         pcg_run(PcgArgs& a, const SellView& A, const ContactView& cv, const VertContacts& vc, int max_iters,
-                        int replace_every, PcgState& hs);
+                        int replace_every);
     void        account_spmv(uint32_t iters, uint32_t nslots);
     VertContacts vert_contacts();
     unsigned    pair_grid(uint32_t cap) const;

@@ -260,6 +260,56 @@ def twisting_sheet(n: int = 513, omega_deg: float = 180.0) -> Scene:
     return Scene(f"twisting_sheet_{n}", [mesh], Params(use_floor=False, gravity=(0.0, 0.0, 0.0)), frames=120)
 
 
+def icosphere(radius: float, subdiv: int):
+    """Vertices (float32) and outward triangles of a subdivided icosahedron."""
+    t = (1.0 + 5.0 ** 0.5) / 2.0
+    v = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+         (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    f = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2), (10, 7, 6),
+         (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11), (6, 2, 10),
+         (8, 6, 7), (9, 8, 1)]
+    verts = [np.array(p, dtype=np.float64) / np.linalg.norm(p) for p in v]
+    for _ in range(subdiv):
+        mid = {}
+
+        def m(a, b):
+            k = (min(a, b), max(a, b))
+            if k not in mid:
+                p = verts[a] + verts[b]
+                verts.append(p / np.linalg.norm(p))
+                mid[k] = len(verts) - 1
+            return mid[k]
+        nf = []
+        for a, b, c in f:
+            ab, bc, ca = m(a, b), m(b, c), m(c, a)
+            nf += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        f = nf
+    return (np.array(verts) * radius).astype(np.float32), np.array(f, dtype=np.int32)
+
+
+def cloth_on_sphere(n: int, offset_x: float = 0.0, offset_z: float = 0.0, seed: int = 0, sphere_subdiv: int = 3,
+                    height: float = 0.5) -> Scene:
+    """One scene of the batched workload: an n x n, 1.0 x 1.0 m cloth (the square cloth's material,
+    dt and jitter, no pins) at `height` (0.5 m), dropped onto a static sphere of radius 0.3 m centred
+    at (offset_x, 0, offset_z) (every sphere vertex pinned in place), ground plane y = 0."""
+    cloth = square_cloth(n, seed=seed, height=height)
+    cm = cloth.meshes[0]
+    cloth_mesh = Mesh("cloth", cm.verts, cm.tris, cm.material, [])
+    sv, st = icosphere(0.3, sphere_subdiv)
+    sv = sv + np.array([offset_x, 0.0, offset_z], dtype=np.float32)
+    sphere = Mesh("sphere", sv, st, cm.material, [FixedPoints("All", 0.001)])
+    return Scene(f"cloth_on_sphere_{n}", [cloth_mesh, sphere], cloth.params, frames=60)
+
+
+def sphere_batch(count: int = 256, seed: int = 0, n_min: int = 17, n_max: int = 161) -> List[Scene]:
+    """The batched workload: `count` cloth-on-sphere scenes with grid side n_i uniform in
+    [n_min, n_max] and sphere offsets uniform in (-0.1, 0.1) m in x and z, all drawn from `seed`."""
+    rng = np.random.default_rng(seed)
+    ns = rng.integers(n_min, n_max + 1, size=count)
+    off = rng.uniform(-0.1, 0.1, size=(count, 2))
+    return [cloth_on_sphere(int(ns[k]), float(off[k, 0]), float(off[k, 1]), seed=seed + k) for k in range(count)]
+
+
 def small_two_sheets(n: int = 9, gap: float = 2.5e-3) -> Scene:
     """Two small stacked sheets a few millimetres apart (self-contact exercise): the lower one is
     pinned at its four corners, the upper one falls onto it."""
