@@ -21,6 +21,11 @@ The JSON line carries:
                 launches (hipExtLaunchKernelGGL) on the solver stream; peak = 8 TB/s HBM3E. The
                 FP16 launch moves 23 B per off-diagonal block (18 B values, 1 B exponent, 4 B
                 column) and the same per row.
+  batched       the batched form on 256 cloth-on-sphere scenes (grid side n ~ U{17..161},
+                sphere r = 0.3 m offset U(-0.1, 0.1) m in x and z, seed 0; 2.81M vertices): all
+                scenes in one solver (one launch per stage) against the same scenes stepped one by
+                one, each from the batch's state after the warm-up frames (the cloths reach the
+                spheres at about frame 12). Scene-steps/s = scenes x frames / seconds.
   cpu_baseline  the CPU oracle (C++ restatement of the reference path, 1 thread) timed on
                 frame 0 of the same workload on this host.
 """
@@ -78,6 +83,61 @@ def profile_pcg(g, copy_mib: int = 256) -> None:
     print(f"[bench] profile-pcg: copied {copy_mib} MiB, PCG {st['pcg_iters']} iterations", file=sys.stderr, flush=True)
 
 
+def batched_leg(args, precision, sequential):
+    """Batched vs one-by-one scene-steps/s on the cloth-on-sphere batch."""
+    from luisacomputesimulator_amd import NewtonSolver, hiprt, scenes
+
+    scs = scenes.sphere_batch(args.batch_scenes, seed=0)
+    for sc in scs:
+        sc.params.precision = precision
+    t = time.perf_counter()
+    b = NewtonSolver.from_scenes(scs)
+    print(f"[bench] batch {precision}: {len(scs)} scenes, {b.num_verts} vertices, setup {time.perf_counter() - t:.1f} s",
+          file=sys.stderr, flush=True)
+    for f in range(args.batch_warmup):
+        tw = time.perf_counter()
+        b.physics_step_gpu(1)
+        st = b.stats()
+        print(f"[bench] batch {precision} warmup frame {f}: {st['newton_iters']} scene-Newton, {st['num_contacts']} contacts, "
+              f"{time.perf_counter() - tw:.2f} s", file=sys.stderr, flush=True)
+    x0, v0 = b.get_simulation_results(), b.get_velocities()
+    ranges = [b.scene_range(g) for g in range(len(scs))]
+    hiprt.synchronize()
+    newton = 0
+    t0 = time.perf_counter()
+    for f in range(args.batch_steps):
+        b.physics_step_gpu(1)
+        newton += b.stats()["newton_iters"]
+    hiprt.synchronize()
+    el = time.perf_counter() - t0
+    out = {"scene_steps_per_s": len(scs) * args.batch_steps / el, "ms_per_frame": 1e3 * el / args.batch_steps,
+           "scene_newton_iters": newton, "contacts_last_iter": b.stats()["num_contacts"]}
+    print(f"[bench] batch {precision} timed: {out}", file=sys.stderr, flush=True)
+    del b
+    if sequential:
+        total, newton_s = 0.0, 0
+        for g, sc in enumerate(scs):
+            a, n = ranges[g]
+            one = NewtonSolver.from_scene(sc)
+            one.set_state(x0[a:a + n], v0[a:a + n])
+            one.physics_step_gpu(1)  # warm-up frame (buffer sizes), then back to the batch's state
+            one.set_state(x0[a:a + n], v0[a:a + n])
+            hiprt.synchronize()
+            t0 = time.perf_counter()
+            for f in range(args.batch_steps):
+                one.physics_step_gpu(1)
+                newton_s += one.stats()["newton_iters"]
+            hiprt.synchronize()
+            total += time.perf_counter() - t0
+            del one
+            if g % 64 == 63:
+                print(f"[bench] one-by-one: {g + 1} scenes, {total:.2f} s", file=sys.stderr, flush=True)
+        out["one_by_one_scene_steps_per_s"] = len(scs) * args.batch_steps / total
+        out["one_by_one_scene_newton_iters"] = newton_s
+        out["speedup_vs_one_by_one"] = out["scene_steps_per_s"] / out["one_by_one_scene_steps_per_s"]
+    return out
+
+
 def cpu_baseline(scene):
     """The CPU oracle (FP32, one thread) on frame 0 of the same scene: Newton iterations per second."""
     import oracle
@@ -107,6 +167,9 @@ def main():
                     help="fp32: the reference's precision (the headline value); mixed: FP16 blocks in the PCG "
                          "with FP32 residual replacement, FP64 CCD/barrier; both: FP32 headline plus a mixed "
                          "run of the same frames")
+    ap.add_argument("--batch-scenes", type=int, default=256, help="batched leg: number of scenes (0: skip)")
+    ap.add_argument("--batch-steps", type=int, default=3, help="batched leg: timed frames")
+    ap.add_argument("--batch-warmup", type=int, default=16, help="batched leg: frames before timing")
     ap.add_argument("--profile-pcg", action="store_true",
                     help="after the warm-up frames run one calibration copy and one PCG solve, then exit "
                          "(workload for rocprofv3 counter runs)")
@@ -219,6 +282,12 @@ def main():
                          "spmv_avg_launch_ms": m["spmv_ms"], "spmv_bytes_per_launch": m["nbytes"],
                          "spmv_achieved_gbs": m["achieved"],
                          "pcg_ms": round(m["kt"].get("pcg", (float("nan"), 0))[0], 4)}
+    if args.batch_scenes > 0:
+        line["batched"] = {"workload": f"{args.batch_scenes} cloth-on-sphere scenes, n ~ U{{17..161}}, seed 0",
+                           "unit": "scene-steps/s", "frames": args.batch_steps, "warmup_frames": args.batch_warmup,
+                           "fp32": batched_leg(args, "fp32", sequential=True)}
+        if "mixed" in precisions:
+            line["batched"]["mixed"] = batched_leg(args, "mixed", sequential=False)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(sc)
     print(json.dumps(line), flush=True)

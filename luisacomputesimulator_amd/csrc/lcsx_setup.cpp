@@ -220,15 +220,18 @@ void Setup::build(const std::vector<MeshDesc>& meshes)
         for (int j = 0; j < 3; j++) v_faces[t[j]].push_back(f);
     }
     {
-        std::map<uint64_t, uint32_t> eid;
-        for (uint32_t e = 0; e < ne; e++) eid[((uint64_t)edges[2 * e] << 32) | edges[2 * e + 1]] = e;
+        // edges are sorted (lexicographic within each mesh, meshes in order): binary search, the
+        // last of equal keys (a non-manifold edge listed twice) as a key -> edge map would keep
+        std::vector<uint64_t> ekey(ne);
+        for (uint32_t e = 0; e < ne; e++) ekey[e] = ((uint64_t)edges[2 * e] << 32) | edges[2 * e + 1];
         std::vector<double> acc(ne, 0.0);
         for (uint32_t f = 0; f < nf; f++)
             for (int j = 0; j < 3; j++)
             {
                 uint32_t a = faces[3 * f + j], b = faces[3 * f + (j + 1) % 3];
-                auto     it = eid.find(((uint64_t)std::min(a, b) << 32) | std::max(a, b));
-                if (it != eid.end()) acc[it->second] += face_area[f] / 3.0;
+                uint64_t k  = ((uint64_t)std::min(a, b) << 32) | std::max(a, b);
+                auto     it = std::upper_bound(ekey.begin(), ekey.end(), k);
+                if (it != ekey.begin() && *(it - 1) == k) acc[(it - 1) - ekey.begin()] += face_area[f] / 3.0;
             }
         edge_area.resize(ne);
         for (uint32_t e = 0; e < ne; e++) edge_area[e] = (float)acc[e];
@@ -386,31 +389,35 @@ void Setup::build(const std::vector<MeshDesc>& meshes)
         }
     }
     // contributors per SELL block: faces of the row vertex in face order, then hinges in order
-    std::vector<std::vector<uint32_t>> contrib(nblocks);
+    // (counted, then filled in the same visiting order)
     auto slot_of = [&](uint32_t r, uint32_t c) {
         const auto& l = adj[r];
         uint32_t    k = (uint32_t)(std::lower_bound(l.begin(), l.end(), c) - l.begin());
         return (sl_off[r / kSlice] + k) * kSlice + (r % kSlice);
     };
-    for (uint32_t r = 0; r < nv; r++)
-    {
-        for (uint32_t p = v_sf_ptr[r]; p < v_sf_ptr[r + 1]; p++)
+    auto visit = [&](auto&& emit) {
+        for (uint32_t r = 0; r < nv; r++)
         {
-            uint32_t f = v_sf[p] >> 2, a = v_sf[p] & 3;
-            for (uint32_t b = 0; b < 3; b++)
-                if (b != a) contrib[slot_of(r, sf[3 * f + b])].push_back((a << 29) | (b << 27) | f);
+            for (uint32_t p = v_sf_ptr[r]; p < v_sf_ptr[r + 1]; p++)
+            {
+                uint32_t f = v_sf[p] >> 2, a = v_sf[p] & 3;
+                for (uint32_t b = 0; b < 3; b++)
+                    if (b != a) emit(slot_of(r, sf[3 * f + b]), (a << 29) | (b << 27) | f);
+            }
+            for (uint32_t p = v_bh_ptr[r]; p < v_bh_ptr[r + 1]; p++)
+            {
+                uint32_t e = v_bh[p] >> 2, a = v_bh[p] & 3;
+                for (uint32_t b = 0; b < 4; b++)
+                    if (b != a) emit(slot_of(r, bh[4 * e + b]), 0x80000000u | (a << 29) | (b << 27) | e);
+            }
         }
-        for (uint32_t p = v_bh_ptr[r]; p < v_bh_ptr[r + 1]; p++)
-        {
-            uint32_t e = v_bh[p] >> 2, a = v_bh[p] & 3;
-            for (uint32_t b = 0; b < 4; b++)
-                if (b != a) contrib[slot_of(r, bh[4 * e + b])].push_back(0x80000000u | (a << 29) | (b << 27) | e);
-        }
-    }
+    };
     blk_ptr.assign(nblocks + 1, 0);
-    for (uint32_t b = 0; b < nblocks; b++) blk_ptr[b + 1] = blk_ptr[b] + (uint32_t)contrib[b].size();
-    blk_contrib.reserve(blk_ptr[nblocks]);
-    for (auto& c : contrib) blk_contrib.insert(blk_contrib.end(), c.begin(), c.end());
+    visit([&](uint32_t blk, uint32_t) { blk_ptr[blk + 1]++; });
+    for (uint32_t b = 0; b < nblocks; b++) blk_ptr[b + 1] += blk_ptr[b];
+    blk_contrib.assign(blk_ptr[nblocks], 0);
+    std::vector<uint32_t> fill(blk_ptr.begin(), blk_ptr.end() - 1);
+    visit([&](uint32_t blk, uint32_t code) { blk_contrib[fill[blk]++] = code; });
 }
 
 }  // namespace lcsx
