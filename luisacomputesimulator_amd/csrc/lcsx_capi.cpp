@@ -437,6 +437,7 @@ lcsx_status lcsx_solver_get_velocities(lcsx_solver* s, float* out, size_t n)
 lcsx_status lcsx_solver_set_state(lcsx_solver* s, const float* x, const float* v, size_t n)
 {
     if (!s || !s->s.inited || n != 3 * (size_t)s->s.S.nv) return arg_error("invalid arguments");
+    s->s.ccd_pairs_fresh = false;
     LCSX_GUARD(if (x) s->s.x.upload(x, n, s->s.stream); if (v) s->s.v.upload(v, n, s->s.stream);
                LCSX_HIP(hipStreamSynchronize(s->s.stream)); return LCSX_OK;)
 }
@@ -527,6 +528,7 @@ lcsx_status lcsx_solver_write(lcsx_solver* s, const char* name, const void* in, 
     Solver&      S = s->s;
     std::string  n = name;
     DBuf<float>* b = nullptr;
+    S.ccd_pairs_fresh = false;
     if (n == "x") b = &S.x;
     else if (n == "v") b = &S.v;
     else if (n == "x_step") b = &S.x_step;

@@ -88,10 +88,11 @@ struct Lbvh
     // per inner node: both child boxes with the child codes in the .w lanes (inner child: its
     // node id; leaf child: 0x80000000 | element id), so a traversal step is one 64 B load
     DBuf<float4>      packed;
-    // level-ordered refit: node heights, inner nodes sorted by height, level offsets
-    DBuf<uint32_t>        height, order, level_hist, level_ptr, level_cursor;
+    // level-ordered refit: node depths (pointer jumping: anc / height / counters as buffers), inner
+    // nodes sorted by depth, level offsets
+    DBuf<uint32_t>        height, order, level_hist, level_ptr, level_cursor, anc[2], scan_tmp;
     std::vector<uint32_t> level_ptr_h;
-    uint32_t              levels = 0, tail_level = 0;  // highest height; first height refit in one workgroup
+    uint32_t              levels = 0, tail_level = 0;  // deepest level; deepest level refit in one workgroup
     bool                  use_levels = false;          // false: one-pass ticket refit (very tall trees)
     uint32_t*         sorted_to_orig = nullptr;  // points into vals[?] after construct
 
@@ -105,6 +106,11 @@ struct Lbvh
     uint32_t n_inner() const { return ni; }
     uint32_t n_nodes() const { return n + ni; }
 };
+
+// Exclusive scan of n counts into out[0..n] (out[n] = total; out may be in), three launches.
+// tmp holds scan_tmp_words(n) words.
+uint32_t scan_tmp_words(uint32_t n);
+void     launch_scan_excl_u32(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* tmp, hipStream_t s);
 
 // Query modes for the solver's broad phase.
 enum QueryMode

@@ -3,6 +3,7 @@
 // Reference counterparts are cited per kernel (paths relative to the reference root).
 #include <hip/hip_ext.h>
 #include "lcsx_fp64.h"
+#include "lcsx_internal.h"
 #include "lcsx_kernels.h"
 #include "lcsx_physics.h"
 
@@ -437,33 +438,84 @@ __global__ __launch_bounds__(kB) void k_vc_count(ContactView c, uint32_t* __rest
     }
 }
 
-// Exclusive scan of n counts into out[0..n] (out[n] = total), one block of 1024 threads: each
-// thread scans a contiguous chunk, chunk totals are scanned in LDS.
-__global__ __launch_bounds__(1024) void k_scan_excl(const uint32_t* __restrict__ in, uint32_t* __restrict__ out, uint32_t n)
+// Exclusive scan of n 32-bit counts into out[0..n] (out[n] = total; out may equal in) in three
+// launches: tile sums, a scan of the tile sums in one workgroup, then each tile rescanned from its
+// offset. Tiles of kScanTile values, 1024 threads x 4 consecutive values.
+constexpr uint32_t kScanTile = 4096;
+
+// Exclusive scan over the block of one value per thread (1024 threads); returns the block total.
+LCSX_D uint32_t block_scan_1024(uint32_t v, uint32_t& excl)
 {
-    __shared__ uint32_t part[1024];
-    const uint32_t t   = threadIdx.x;
-    const uint32_t per = (n + 1023u) / 1024u;
-    const uint32_t b   = min(n, t * per), e = min(n, b + per);
-    uint32_t       s   = 0;
-    for (uint32_t i = b; i < e; i++) s += in[i];
-    part[t] = s;
-    __syncthreads();
-    for (uint32_t off = 1; off < 1024u; off <<= 1)
+    __shared__ uint32_t wsum[16];
+    const uint32_t      lane = lane_id(), w = threadIdx.x >> 6;
+    uint32_t            x    = v;
+    for (int off = 1; off < 64; off <<= 1)
     {
-        uint32_t v = t >= off ? part[t - off] : 0u;
-        __syncthreads();
-        part[t] += v;
-        __syncthreads();
+        uint32_t y = __shfl_up(x, off);
+        if ((int)lane >= off) x += y;
     }
-    uint32_t run = part[t] - s;
+    if (lane == 63u) wsum[w] = x;
+    __syncthreads();
+    uint32_t before = 0, total = 0;
+    for (uint32_t k = 0; k < 16u; k++)
+    {
+        if (k < w) before += wsum[k];
+        total += wsum[k];
+    }
+    __syncthreads();
+    excl = before + x - v;
+    return total;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_tiles(const uint32_t* __restrict__ in, uint32_t n, uint32_t* __restrict__ part)
+{
+    const uint32_t base = blockIdx.x * kScanTile + 4u * threadIdx.x;
+    uint32_t       v    = 0;
+    for (uint32_t k = 0; k < 4u; k++)
+        if (base + k < n) v += in[base + k];
+    uint32_t excl;
+    uint32_t tot = block_scan_1024(v, excl);
+    if (threadIdx.x == 0) part[blockIdx.x] = tot;
+}
+
+// part[0..m) -> exclusive offsets, part[m] = total (one workgroup, m <= 1024 * chunk)
+__global__ __launch_bounds__(1024) void k_scan_parts(uint32_t* __restrict__ part, uint32_t m)
+{
+    const uint32_t per = (m + 1023u) / 1024u;
+    const uint32_t b = min(m, threadIdx.x * per), e = min(m, b + per);
+    uint32_t       v = 0;
+    for (uint32_t i = b; i < e; i++) v += part[i];
+    uint32_t excl;
+    uint32_t tot = block_scan_1024(v, excl);
     for (uint32_t i = b; i < e; i++)
     {
-        uint32_t x = in[i];
-        out[i]     = run;
-        run += x;
+        uint32_t x = part[i];
+        part[i]    = excl;
+        excl += x;
     }
-    if (t == 1023u) out[n] = part[1023];
+    if (threadIdx.x == 0) part[m] = tot;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_apply(const uint32_t* in, uint32_t n, const uint32_t* __restrict__ part,
+                                                      uint32_t* out)
+{
+    const uint32_t base = blockIdx.x * kScanTile + 4u * threadIdx.x;
+    uint32_t       x[4], v = 0;
+    for (uint32_t k = 0; k < 4u; k++)
+    {
+        x[k] = base + k < n ? in[base + k] : 0u;
+        v += x[k];
+    }
+    uint32_t excl;
+    block_scan_1024(v, excl);  // every read of this tile happens before the barrier inside
+    uint32_t run = part[blockIdx.x] + excl;
+    for (uint32_t k = 0; k < 4u; k++)
+        if (base + k < n)
+        {
+            out[base + k] = run;
+            run += x[k];
+        }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) out[n] = part[gridDim.x];
 }
 
 __global__ __launch_bounds__(kB) void k_vc_fill(ContactView c, const uint32_t* __restrict__ ptr, uint32_t* __restrict__ cursor,
@@ -1138,8 +1190,9 @@ __global__ void k_reset_status(Status* st, SceneStatus* sst, uint32_t nscenes)
     uint32_t i = blockIdx.x * kB + threadIdx.x;
     if (i == 0)
     {
+        // the CCD counts stay: the next DCD may run on the last line search's candidates (the
+        // line search resets them before its own broad phase)
         st->n_vf = st->n_ee = st->n_contacts = st->overflow = 0;
-        st->n_vf_ccd = st->n_ee_ccd = 0;
         st->nan = 0;
     }
     if (i < nscenes) sst[i] = SceneStatus{f2ord(1.0f), 0u, 0u, 0u};
@@ -1147,6 +1200,19 @@ __global__ void k_reset_status(Status* st, SceneStatus* sst, uint32_t nscenes)
 }  // namespace
 
 // ---------------------------------------------------------------- launchers
+uint32_t scan_tmp_words(uint32_t n) { return (n + kScanTile - 1) / kScanTile + 1; }
+void launch_scan_excl_u32(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* tmp, hipStream_t s)
+{
+    const uint32_t m = (n + kScanTile - 1) / kScanTile;
+    if (m == 0)
+    {
+        k_fill_u32<<<1, kB, 0, s>>>(out, 1, 0u);
+        return;
+    }
+    k_scan_tiles<<<m, 1024, 0, s>>>(in, n, tmp);
+    k_scan_parts<<<1, 1024, 0, s>>>(tmp, m);
+    k_scan_apply<<<m, 1024, 0, s>>>(in, n, tmp, out);
+}
 void launch_fill_u32(uint32_t* a, uint32_t n, uint32_t v, hipStream_t s)
 {
     if (n) k_fill_u32<<<grid_for(n, kB), kB, 0, s>>>(a, n, v);
@@ -1191,7 +1257,7 @@ void launch_contact_adjacency(const ContactView& c, uint32_t nv, unsigned grid, 
 {
     k_fill_u32<<<grid_for(nv + 1, kB), kB, 0, s>>>(cnt, nv + 1, 0u);
     k_vc_count<<<grid, kB, 0, s>>>(c, cnt);
-    k_scan_excl<<<1, 1024, 0, s>>>(cnt, ptr, nv);
+    launch_scan_excl_u32(cnt, ptr, nv, list, s);  // list: scratch for the tile sums (refilled below)
     k_fill_u32<<<grid_for(nv + 1, kB), kB, 0, s>>>(cnt, nv + 1, 0u);
     k_vc_fill<<<grid, kB, 0, s>>>(c, ptr, cnt, list, row);
     k_vc_coeffs<<<4 * grid, kB, 0, s>>>(c, ptr, nv, list, coef);

@@ -165,33 +165,6 @@ __global__ __launch_bounds__(kBlock) void k_radix_hist(const uint32_t* __restric
     hist[threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
 }
 
-// Exclusive scan of `total` counters in one workgroup of 1024 threads.
-__global__ __launch_bounds__(1024) void k_scan_single(uint32_t* __restrict__ a, uint32_t total)
-{
-    __shared__ uint32_t part[1024];
-    uint32_t            t     = threadIdx.x;
-    uint32_t            chunk = (total + 1023) / 1024;
-    uint32_t            b = t * chunk, e = min(total, b + chunk);
-    uint32_t            sum = 0;
-    for (uint32_t i = b; i < e; i++) sum += a[i];
-    part[t] = sum;
-    __syncthreads();
-    for (uint32_t o = 1; o < 1024; o <<= 1)
-    {
-        uint32_t v = t >= o ? part[t - o] : 0u;
-        __syncthreads();
-        part[t] += v;
-        __syncthreads();
-    }
-    uint32_t run = part[t] - sum;
-    for (uint32_t i = b; i < e; i++)
-    {
-        uint32_t v = a[i];
-        a[i]       = run;
-        run += v;
-    }
-}
-
 // Stable scatter by the digit of dig (kin, or the segment keys hin): kin, vin and, when present,
 // hin move together.
 __global__ __launch_bounds__(kBlock) void k_radix_scatter(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
@@ -417,32 +390,39 @@ __global__ void k_fill_u32(uint32_t* a, uint32_t n, uint32_t v)
 
 // ---------------------------------------------------------------- level-ordered refit
 // Cross-workgroup hand-off inside one kernel needs device-scope fences, which on a multi-XCD
-// part write back and invalidate L2. The refit therefore runs one launch per node height
-// (leaves 0, inner 1 + max of its children): nodes of one height are independent, and kernel
-// boundaries order the levels. Heights and the height-sorted node order are computed once per
-// construction (with the ticket protocol); the top levels run in one workgroup.
-constexpr uint32_t kMaxLevels = 256;
+// part write back and invalidate L2. The refit therefore runs one launch per node depth, deepest
+// first: an inner node's children are one level deeper, so kernel boundaries order the levels.
+// Depths come from pointer jumping over the parent links (kDepthRounds launches, no atomics): after
+// round r each node holds its distance to its 2^r-th ancestor; depths of kMaxLevels or more land in
+// the overflow bin and keep the ticket refit. The top levels run in one workgroup.
+constexpr uint32_t kMaxLevels   = 256;
+constexpr int      kDepthRounds = 8;  // 2^8 = kMaxLevels
 
-__global__ __launch_bounds__(kBlock) void k_height(uint32_t n, uint32_t ni, const uint32_t* __restrict__ parent,
-                                                    const uint2* __restrict__ children, uint32_t* height,
-                                                    uint32_t* counters)
+__global__ __launch_bounds__(kBlock) void k_depth_init(uint32_t ni, const uint32_t* __restrict__ parent,
+                                                        uint32_t* __restrict__ anc, uint32_t* __restrict__ dep)
 {
-    uint32_t l = blockIdx.x * kBlock + threadIdx.x;
-    if (l >= n) return;
-    uint32_t cur = ni + l;
-    for (uint32_t depth = 0; depth < n; depth++)
+    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= ni) return;
+    uint32_t p = parent[i];
+    anc[i]     = p;
+    dep[i]     = p == kInvalid ? 0u : 1u;
+}
+
+__global__ __launch_bounds__(kBlock) void k_depth_jump(uint32_t ni, const uint32_t* __restrict__ anc_in,
+                                                        const uint32_t* __restrict__ dep_in, uint32_t* __restrict__ anc_out,
+                                                        uint32_t* __restrict__ dep_out)
+{
+    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= ni) return;
+    uint32_t a = anc_in[i], d = dep_in[i];
+    if (a == kInvalid)
     {
-        uint32_t p = parent[cur];
-        if (p == kInvalid || p >= ni) break;
-        __threadfence();
-        if (atomicAdd(&counters[p], 1u) == 0u) break;
-        __threadfence();
-        uint2    c  = children[p];
-        uint32_t ha = c.x >= ni ? 0u : height[c.x];
-        uint32_t hb = c.y >= ni ? 0u : height[c.y];
-        height[p]   = 1u + max(ha, hb);
-        cur         = p;
+        anc_out[i] = a;
+        dep_out[i] = d;
+        return;
     }
+    anc_out[i] = anc_in[a];
+    dep_out[i] = min(d + dep_in[a], kMaxLevels);
 }
 
 // hist[h] for h < kMaxLevels; taller nodes are counted in hist[kMaxLevels]. Per-block LDS
@@ -560,12 +540,12 @@ __global__ __launch_bounds__(kBlock) void k_refit_level(const uint32_t* __restri
     if (k < e) inner_box(order[k], ni, children, nodes, packed);
 }
 
-// Levels h0 .. h1 (inclusive) in one workgroup, a barrier between levels.
+// Depths h1 down to 0 in one workgroup, a barrier between levels.
 __global__ __launch_bounds__(1024) void k_refit_tail(const uint32_t* __restrict__ order, const uint32_t* __restrict__ level_ptr,
-                                                      uint32_t h0, uint32_t h1, uint32_t ni,
-                                                      const uint2* __restrict__ children, NodeBox* nodes, float4* packed)
+                                                      uint32_t h1, uint32_t ni, const uint2* __restrict__ children,
+                                                      NodeBox* nodes, float4* packed)
 {
-    for (uint32_t h = h0; h <= h1; h++)
+    for (int h = (int)h1; h >= 0; h--)
     {
         for (uint32_t k = level_ptr[h] + threadIdx.x; k < level_ptr[h + 1]; k += 1024u)
             inner_box(order[k], ni, children, nodes, packed);
@@ -792,7 +772,8 @@ void Lbvh::init(int arity_, const uint32_t* device_prims, uint32_t n_leaves, con
         khi[1].alloc(n);
     }
     uint32_t nb = (n + kSortTile - 1) / kSortTile;
-    hist.alloc(256 * (size_t)nb);
+    hist.alloc(256 * (size_t)nb + 1);
+    scan_tmp.alloc(scan_tmp_words(256 * nb));
     center.alloc(n);
     morton_sorted.alloc(n);
     parent.alloc(n_nodes());
@@ -802,6 +783,8 @@ void Lbvh::init(int arity_, const uint32_t* device_prims, uint32_t n_leaves, con
     packed.alloc(4 * (size_t)(ni ? ni : 1));
     counters.alloc(ni ? ni : 1);
     height.alloc(ni ? ni : 1);
+    anc[0].alloc(ni ? ni : 1);
+    anc[1].alloc(ni ? ni : 1);
     order.alloc(ni ? ni : 1);
     level_hist.alloc(kMaxLevels + 1);
     level_ptr.alloc(kMaxLevels + 2);
@@ -835,7 +818,7 @@ void Lbvh::construct(const float* x, const uint32_t* verts_per_seg, hipStream_t 
     for (int shift = 0; shift < 32; shift += 8)
     {
         k_radix_hist<<<nb, kBlock, 0, s>>>(keys[cur].p, n, shift, hist.p, nb);
-        k_scan_single<<<1, 1024, 0, s>>>(hist.p, 256 * nb);
+        launch_scan_excl_u32(hist.p, hist.p, 256 * nb, scan_tmp.p, s);
         k_radix_scatter<<<nb, kBlock, 0, s>>>(keys[cur].p, vals[cur].p, keys[1 - cur].p, vals[1 - cur].p, nullptr,
                                                nullptr, keys[cur].p, n, shift, hist.p, nb);
         cur = 1 - cur;
@@ -846,7 +829,7 @@ void Lbvh::construct(const float* x, const uint32_t* verts_per_seg, hipStream_t 
         for (int shift = 0; shift < 32 && ((nseg - 1) >> shift) != 0u; shift += 8)
         {
             k_radix_hist<<<nb, kBlock, 0, s>>>(khi[cur].p, n, shift, hist.p, nb);
-            k_scan_single<<<1, 1024, 0, s>>>(hist.p, 256 * nb);
+            launch_scan_excl_u32(hist.p, hist.p, 256 * nb, scan_tmp.p, s);
             k_radix_scatter<<<nb, kBlock, 0, s>>>(keys[cur].p, vals[cur].p, keys[1 - cur].p, vals[1 - cur].p,
                                                    khi[cur].p, khi[1 - cur].p, khi[cur].p, n, shift, hist.p, nb);
             cur = 1 - cur;
@@ -860,15 +843,19 @@ void Lbvh::construct(const float* x, const uint32_t* verts_per_seg, hipStream_t 
     LCSX_HIP(hipGetLastError());
     use_levels = false;
     if (ni == 0) return;
-    // node heights and the height-sorted order of inner nodes (one host read of the histogram)
-    k_fill_u32<<<grid_for(ni, kBlock), kBlock, 0, s>>>(counters.p, ni, 0u);
-    k_height<<<grid_for(n, kBlock), kBlock, 0, s>>>(n, ni, parent.p, children.p, height.p, counters.p);
+    // node depths and the depth-sorted order of inner nodes (one host read of the histogram)
+    k_depth_init<<<grid_for(ni, kBlock), kBlock, 0, s>>>(ni, parent.p, anc[0].p, height.p);
+    uint32_t* dep[2] = {height.p, counters.p};
+    for (int r = 0; r < kDepthRounds; r++)
+        k_depth_jump<<<grid_for(ni, kBlock), kBlock, 0, s>>>(ni, anc[r & 1].p, dep[r & 1], anc[1 - (r & 1)].p,
+                                                             dep[1 - (r & 1)]);
+    uint32_t* depth = dep[kDepthRounds & 1];
     k_fill_u32<<<grid_for(kMaxLevels + 1, kBlock), kBlock, 0, s>>>(level_hist.p, kMaxLevels + 1, 0u);
-    k_level_hist<<<grid_for(ni, kBlock), kBlock, 0, s>>>(ni, height.p, level_hist.p);
+    k_level_hist<<<grid_for(ni, kBlock), kBlock, 0, s>>>(ni, depth, level_hist.p);
     std::vector<uint32_t> hist(kMaxLevels + 1);
     LCSX_HIP(hipMemcpyAsync(hist.data(), level_hist.p, hist.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     LCSX_HIP(hipStreamSynchronize(s));
-    if (hist[kMaxLevels] != 0) return;  // taller than kMaxLevels: keep the ticket refit
+    if (hist[kMaxLevels] != 0) return;  // deeper than kMaxLevels: keep the ticket refit
     level_ptr_h.assign(kMaxLevels + 2, 0u);
     levels = 0;
     for (uint32_t h = 0; h <= kMaxLevels; h++)
@@ -876,13 +863,16 @@ void Lbvh::construct(const float* x, const uint32_t* verts_per_seg, hipStream_t 
         level_ptr_h[h + 1] = level_ptr_h[h] + hist[h];
         if (hist[h]) levels = h;
     }
-    // the top levels, together at most kTailNodes nodes, are refit by one workgroup
+    // the top levels (depth <= tail_level), together at most kTailNodes nodes, are refit by one
+    // workgroup; tail_level = kInvalid when the roots alone are more
     constexpr uint32_t kTailNodes = 8192;
-    tail_level = levels;
-    while (tail_level > 1 && level_ptr_h[levels + 1] - level_ptr_h[tail_level - 1] <= kTailNodes) tail_level--;
+    tail_level = kInvalid;
+    while ((tail_level == kInvalid ? 0u : tail_level + 1) <= levels &&
+           level_ptr_h[(tail_level == kInvalid ? 0u : tail_level + 1) + 1] <= kTailNodes)
+        tail_level = tail_level == kInvalid ? 0u : tail_level + 1;
     LCSX_HIP(hipMemcpyAsync(level_ptr.p, level_ptr_h.data(), level_ptr_h.size() * sizeof(uint32_t), hipMemcpyHostToDevice, s));
     k_fill_u32<<<grid_for(kMaxLevels + 1, kBlock), kBlock, 0, s>>>(level_cursor.p, kMaxLevels + 1, 0u);
-    k_level_scatter<<<grid_for(ni, kBlock), kBlock, 0, s>>>(ni, height.p, level_ptr.p, level_cursor.p, order.p);
+    k_level_scatter<<<grid_for(ni, kBlock), kBlock, 0, s>>>(ni, depth, level_ptr.p, level_cursor.p, order.p);
     LCSX_HIP(hipGetLastError());
     use_levels = true;
 }
@@ -894,13 +884,14 @@ void Lbvh::refit(const float* x0, const float* x1, const float* pad, hipStream_t
     {
         k_refit_leaves<<<grid_for(n, kBlock), kBlock, 0, s>>>(n, ni, arity, prims, sorted_to_orig, x0, x1, pad, nodes.p);
         if (ni == 0) return;
-        for (uint32_t h = 1; h < tail_level; h++)
+        const int top = tail_level == kInvalid ? -1 : (int)tail_level;
+        for (int h = (int)levels; h > top; h--)
         {
             uint32_t b = level_ptr_h[h], e = level_ptr_h[h + 1];
             if (e > b)
                 k_refit_level<<<grid_for(e - b, kBlock), kBlock, 0, s>>>(order.p, b, e, ni, children.p, nodes.p, packed.p);
         }
-        k_refit_tail<<<1, 1024, 0, s>>>(order.p, level_ptr.p, tail_level, levels, ni, children.p, nodes.p, packed.p);
+        if (top >= 0) k_refit_tail<<<1, 1024, 0, s>>>(order.p, level_ptr.p, (uint32_t)top, ni, children.p, nodes.p, packed.p);
         LCSX_HIP(hipGetLastError());
         return;
     }

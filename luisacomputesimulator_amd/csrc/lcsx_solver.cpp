@@ -125,6 +125,7 @@ void Solver::alloc_collision(uint32_t vf, uint32_t ee, uint32_t nc)
     c_w.alloc(cap_c);
     c_aux.alloc(cap_c);
     ccd_rest_vf.alloc(cap_vf);
+    ccd_pairs_fresh = false;
     ccd_rest_ee.alloc(cap_ee);
     vc_list.alloc(4 * (size_t)cap_c);
     // per-block partial sums of the SpMV: row blocks + scene-aligned slot chunks (4 slots per contact)
@@ -292,6 +293,7 @@ SellView Solver::sell_view_lp() const
 // ------------------------------------------------------------------ stages
 void Solver::frame_begin()
 {
+    ccd_pairs_fresh = false;  // the last line search may have covered only some scenes of a batch
     const float t   = (float)frame * prm.implicit_dt;
     const size_t nf = S.fixed_verts.size();
     if (nf)
@@ -409,7 +411,7 @@ void Solver::broad_phase(bool ccd)
     toc(ccd ? "broadphase_ccd" : "broadphase_dcd");
 }
 
-void Solver::dcd()
+void Solver::dcd(bool from_ccd_pairs)
 {
     vc_ready = false;  // new contact set: the per-vertex lists are rebuilt by contact_assemble
     tic();
@@ -424,13 +426,13 @@ void Solver::dcd()
     a.fp64   = prm.precision == 1;
     a.out    = out;
     a.pairs = pairs_vf.p;
-    a.pair_count = &status.p->n_vf;
+    a.pair_count = from_ccd_pairs ? &status.p->n_vf_ccd : &status.p->n_vf;
     a.pair_capacity = cap_vf;
     a.area_a = vert_area.p;
     a.area_b = face_area.p;
     launch_dcd(a, 0, pair_grid(cap_vf), stream);
     a.pairs = pairs_ee.p;
-    a.pair_count = &status.p->n_ee;
+    a.pair_count = from_ccd_pairs ? &status.p->n_ee_ccd : &status.p->n_ee;
     a.pair_capacity = cap_ee;
     a.area_a = edge_area.p;
     a.area_b = edge_area.p;
@@ -438,13 +440,17 @@ void Solver::dcd()
     toc("dcd");
 }
 
-void Solver::update_contacts()
+// The DCD candidates of a Newton iteration after the first of a frame: the swept boxes of the
+// previous line search (x_iter to x_iter + dx, padded as the DCD boxes) contain the boxes at
+// x_iter + alpha dx for any alpha in [0, 1], so its candidate pairs include every pair the DCD
+// broad phase would find; the narrow phase keeps exactly the same contacts (its distance test
+// decides). Skips one of the two broad phases per iteration.
+void Solver::update_contacts(bool reuse_ccd_pairs)
 {
-    if (prm.use_self_collision)
-    {
-        broad_phase(false);
-        dcd();
-    }
+    if (!prm.use_self_collision) return;
+    dcd_reused = reuse_ccd_pairs && ccd_pairs_fresh;
+    if (!dcd_reused) broad_phase(false);
+    dcd(dcd_reused);
 }
 
 void Solver::contact_assemble()
@@ -688,6 +694,7 @@ void Solver::line_search()
     launch_reset_line_search(status.p, sst.p, S.nscenes, stream);
     launch_apply_dx(S.nv, x_iter.p, dx.p, x.p, nullptr, 1.0f, g, stream);
     if (prm.use_ccd_linesearch && prm.use_self_collision) broad_phase(true);
+    ccd_pairs_fresh = prm.use_ccd_linesearch && prm.use_self_collision;
     ccd();
     launch_apply_dx(S.nv, x_iter.p, dx.p, x.p, sst.p, 1.0f, g, stream);
     launch_pin_delta((uint32_t)S.fixed_verts.size(), fixed_ids.p, x.p, x_tilde.p, sst.p, vert_scene_d.p, stream);
@@ -697,7 +704,7 @@ bool Solver::newton_iteration()
 {
     iter_begin();
     material();
-    update_contacts();
+    update_contacts(true);
     contact_assemble();
     pcg();
     line_search();
@@ -728,8 +735,8 @@ bool Solver::newton_iteration()
             throw HipFailure();
         }
     }
-    stats.num_pairs_vf = (int)host_status.n_vf;
-    stats.num_pairs_ee = (int)host_status.n_ee;
+    stats.num_pairs_vf = (int)(dcd_reused ? host_status.n_vf_ccd : host_status.n_vf);
+    stats.num_pairs_ee = (int)(dcd_reused ? host_status.n_ee_ccd : host_status.n_ee);
     stats.num_contacts = (int)host_status.n_contacts;
     stats.broadphase_overflow |= host_status.overflow ? 1 : 0;
     fold_stats();

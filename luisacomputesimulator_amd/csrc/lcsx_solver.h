@@ -52,6 +52,8 @@ struct Solver
     DBuf<float>    vc_qc;                    // contact part of the SpMV (3 per vertex)
     DBuf<uint32_t> ccd_rest_vf, ccd_rest_ee, ccd_rest_count;  // CCD pairs left for the full-budget pass
     bool           vc_ready = false;
+    bool           ccd_pairs_fresh = false;  // pairs_* hold the last line search's swept candidates
+    bool           dcd_reused      = false;  // the last DCD ran on them
     DBuf<Status>   status;
     DBuf<PcgState> pcg_state;  // per scene
     Status         host_status{};
@@ -87,8 +89,8 @@ struct Solver
     void iter_begin();
     void material();
     void broad_phase(bool ccd);
-    void dcd();
-    void update_contacts();
+    void dcd(bool from_ccd_pairs = false);
+    void update_contacts(bool reuse_ccd_pairs = false);
     void contact_assemble();
     void pcg();
     void ccd();
