@@ -53,6 +53,10 @@ typedef struct lcsx_material
     float density;                /* kg/m^3; mass = area * thickness * density when mass==0 */
     float mass;                   /* total mesh mass, 0 = from density                      */
     float d_hat;                  /* barrier activation distance, m                         */
+    int   body;                   /* 0 cloth; 1 rigid shell (affine body, surface mass); 2 rigid
+                                     solid (affine body of a closed surface, contact thickness 0)
+                                     (RigidMaterial, physical_material.h:69-77)              */
+    float rigid_stiffness;        /* RigidMaterial::stiffness: orthogonality kappa          */
 } lcsx_material;
 
 /* FixedPointAnimationInfo, Solver/Initializer/init_mesh_data.h:39-93. */

@@ -18,7 +18,7 @@ from typing import Dict, List, Optional
 import numpy as np
 
 from . import scenes  # noqa: F401
-from .scenes import FIXED_METHODS, Anim, Material, Params, Scene
+from .scenes import BODY_TYPES, FIXED_METHODS, Anim, Material, Params, Scene
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "liblcsx.so")
@@ -32,7 +32,7 @@ class LcsxError(RuntimeError):
 class lcsx_material(C.Structure):
     _fields_ = [("thickness", C.c_float), ("youngs_modulus", C.c_float), ("poisson_ratio", C.c_float),
                 ("area_bending_stiffness", C.c_float), ("density", C.c_float), ("mass", C.c_float),
-                ("d_hat", C.c_float)]
+                ("d_hat", C.c_float), ("body", C.c_int), ("rigid_stiffness", C.c_float)]
 
 
 class lcsx_fixed_anim(C.Structure):
@@ -147,6 +147,8 @@ def material_to_c(m: Material) -> lcsx_material:
     c = lcsx_material()
     c.thickness, c.youngs_modulus, c.poisson_ratio = m.thickness, m.youngs_modulus, m.poisson_ratio
     c.area_bending_stiffness, c.density, c.mass, c.d_hat = m.area_bending_stiffness, m.density, m.mass, m.d_hat
+    c.body = BODY_TYPES[m.body]
+    c.rigid_stiffness = m.rigid_stiffness
     return c
 
 

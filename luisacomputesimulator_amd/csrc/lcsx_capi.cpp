@@ -309,6 +309,8 @@ void lcsx_material_default(lcsx_material* m)
     m->density                = 1e3f;
     m->mass                   = 0.0f;
     m->d_hat                  = 2e-3f;
+    m->body                   = 0;
+    m->rigid_stiffness        = 1e6f;
 }
 
 lcsx_status lcsx_solver_create(const lcsx_params* params, int device, lcsx_solver** out)

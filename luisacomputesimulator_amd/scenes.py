@@ -48,9 +48,16 @@ class Anim:
                 *map(float, self.rot_axis), float(self.rot_deg_per_s), 0.0]
 
 
+BODY_TYPES = {"cloth": 0, "rigid_shell": 1, "rigid_solid": 2}
+
+
 @dataclasses.dataclass
 class Material:
-    """ClothMaterial (Solver/SimulationCore/physical_material.h:45-63)."""
+    """ClothMaterial / RigidMaterial (Solver/SimulationCore/physical_material.h:45-77). body:
+    "cloth"; "rigid_shell" (an affine body whose mass is carried by its surface, area x thickness
+    x density, the RigidMaterial default); "rigid_solid" (a closed surface bounding a solid of
+    the given density, contact thickness 0). rigid_stiffness: RigidMaterial::stiffness, the
+    orthogonality stiffness kappa (times the body volume)."""
     thickness: float = 2e-3
     youngs_modulus: float = 1e5
     poisson_ratio: float = 0.25
@@ -58,10 +65,12 @@ class Material:
     density: float = 1e3
     mass: float = 0.0
     d_hat: float = 2e-3
+    body: str = "cloth"
+    rigid_stiffness: float = 1e6
 
     def pack(self) -> List[float]:
         return [self.thickness, self.youngs_modulus, self.poisson_ratio, self.area_bending_stiffness,
-                self.density, self.mass, self.d_hat]
+                self.density, self.mass, self.d_hat, float(BODY_TYPES[self.body]), self.rigid_stiffness]
 
 
 @dataclasses.dataclass
@@ -258,6 +267,28 @@ def twisting_sheet(n: int = 513, omega_deg: float = 180.0) -> Scene:
     mesh = Mesh("sheet", verts, staggered_ring_triangles(n, n, wrap=False), mat,
                 [FixedPoints("Left", 0.001, left), FixedPoints("Right", 0.001, right)])
     return Scene(f"twisting_sheet_{n}", [mesh], Params(use_floor=False, gravity=(0.0, 0.0, 0.0)), frames=120)
+
+
+def rotation_matrix(angles) -> np.ndarray:
+    """Rotation by the Euler angles (x, then y, then z, radians) as a 3x3 matrix."""
+    ax, ay, az = angles
+    cx, sx, cy, sy, cz, sz = np.cos(ax), np.sin(ax), np.cos(ay), np.sin(ay), np.cos(az), np.sin(az)
+    Rx = np.array([[1, 0, 0], [0, cx, -sx], [0, sx, cx]])
+    Ry = np.array([[cy, 0, sy], [0, 1, 0], [-sy, 0, cy]])
+    Rz = np.array([[cz, -sz, 0], [sz, cz, 0], [0, 0, 1]])
+    return Rz @ Ry @ Rx
+
+
+def box_mesh(center, side: float, R=None):
+    """A cube of the given side (8 vertices, 12 outward triangles), rotated by R about its centre."""
+    c = np.array([[x, y, z] for x in (-0.5, 0.5) for y in (-0.5, 0.5) for z in (-0.5, 0.5)]) * side
+    if R is not None:
+        c = c @ np.asarray(R).T
+    v = (c + np.asarray(center)).astype(np.float32)
+    # vertex k = 4 x + 2 y + z (bits of the corner)
+    t = np.array([[0, 1, 3], [0, 3, 2], [4, 6, 7], [4, 7, 5], [0, 4, 5], [0, 5, 1],
+                  [2, 3, 7], [2, 7, 6], [0, 2, 6], [0, 6, 4], [1, 5, 7], [1, 7, 3]], dtype=np.int32)
+    return v, t
 
 
 def icosphere(radius: float, subdiv: int):

@@ -144,7 +144,7 @@ int run_stage(Scene<T>& s, const std::string& st)
     if (st == "step") { s.step(); return 0; }
     if (st == "spmv_dx")
     {
-        std::vector<V3<T>> y(s.nv);
+        std::vector<V3<T>> y(s.nr);
         s.spmv(s.dx, y);
         s.dx = y;
         return 0;
@@ -187,7 +187,8 @@ int orc_set_params(void* p, const double* a, int n)
     return 0;
 }
 
-// material order: thickness, youngs, poisson, bending, density, mass, d_hat
+// material order: thickness, youngs, poisson, bending, density, mass, d_hat[, body (0 cloth,
+// 1 rigid shell, 2 rigid solid), rigid stiffness]
 int orc_add_mesh(void* p, const float* x, int nv, const int32_t* tris, int nt, const double* mat, int nm)
 {
     if (nm < 7) return -1;
@@ -197,6 +198,11 @@ int orc_add_mesh(void* p, const float* x, int nv, const int32_t* tris, int nt, c
     for (size_t i = 0; i < m.tris.size(); i++) m.tris[i] = (uint32_t)tris[i];
     m.mat.thickness = mat[0]; m.mat.youngs = mat[1]; m.mat.poisson = mat[2]; m.mat.bending = mat[3];
     m.mat.density = mat[4]; m.mat.mass = mat[5]; m.mat.d_hat = mat[6];
+    if (nm > 8)
+    {
+        m.mat.body        = (int)mat[7];
+        m.mat.rigid_kappa = mat[8];
+    }
     Handle* h = (Handle*)p;
     h->s32.meshes.push_back(m);
     h->s64.meshes.push_back(m);
@@ -504,5 +510,49 @@ int64_t orc_lbvh_query(void* t, const float* boxes, uint32_t nq, int edge_filter
         });
     }
     return ok ? k : -k - 1;
+}
+
+// Affine-body terms of one body without ground contact: inertia (q, q_tilde, 4x4 mass matrix,
+// h^-2, fixed flag, k_D) plus orthogonality (stiffness kappa V). out_g: 12 (energy gradient,
+// four 3-blocks), out_H: 16 3x3 blocks row-major in the reference's block order.
+void orc_abd_terms(int precision, const double* q, const double* qt, const double* M, double h2inv, int fixed,
+                   double kD, double stiff, double* out_g, double* out_H)
+{
+    auto run = [&](auto zero) {
+        using T = decltype(zero);
+        V3<T> qq[4], tt[4];
+        T     MM[4][4];
+        for (int k = 0; k < 4; k++)
+        {
+            qq[k] = V3<T>(T(q[3 * k]), T(q[3 * k + 1]), T(q[3 * k + 2]));
+            tt[k] = V3<T>(T(qt[3 * k]), T(qt[3 * k + 1]), T(qt[3 * k + 2]));
+            for (int j = 0; j < 4; j++) MM[k][j] = T(M[4 * k + j]);
+        }
+        BodyGH<T> o;
+        abd_inertia(qq, tt, MM, T(h2inv), fixed != 0, T(kD), o);
+        abd_orthogonality(qq, T(stiff), o);
+        for (int k = 0; k < 4; k++)
+            for (int d = 0; d < 3; d++) out_g[3 * k + d] = (double)o.g[k][d];
+        for (int b = 0; b < 16; b++)
+            for (int r = 0; r < 3; r++)
+                for (int c = 0; c < 3; c++) out_H[9 * b + 3 * r + c] = (double)o.H[b].m[r][c];
+    };
+    if (precision == 64) run(0.0);
+    else run(0.0f);
+}
+
+// compute_trimesh_dyadic_mass of a closed triangle surface: out = m, m x_bar (3), m x x^T (9).
+void orc_dyadic_mass(const float* x, int nv, const int32_t* tris, int nt, float rho, double* out)
+{
+    std::vector<V3<float>> P(nv);
+    for (int i = 0; i < nv; i++) P[i] = V3<float>(x[3 * i], x[3 * i + 1], x[3 * i + 2]);
+    std::vector<uint32_t> t(3 * (size_t)nt);
+    for (size_t i = 0; i < t.size(); i++) t[i] = (uint32_t)tris[i];
+    float m, mx[3], mxx[3][3];
+    dyadic_mass(P, t.data(), (size_t)nt, rho, m, mx, mxx);
+    out[0] = m;
+    for (int a = 0; a < 3; a++) out[1 + a] = mx[a];
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) out[4 + 3 * a + b] = mxx[a][b];
 }
 }

@@ -18,6 +18,7 @@
 #include <map>
 #include <string>
 #include <vector>
+#include "orc_abd.h"
 #include "orc_collision.h"
 #include "orc_energy.h"
 #include "orc_lbvh.h"
@@ -28,6 +29,8 @@ namespace orc
 struct Material
 {
     double thickness = 2e-3, youngs = 1e5, poisson = 0.25, bending = 5e-3, density = 1e3, mass = 0.0, d_hat = 2e-3;
+    int    body        = 0;    // 0 cloth, 1 rigid shell, 2 rigid solid (RigidMaterial, is_shell)
+    double rigid_kappa = 1e6;  // RigidMaterial::stiffness (orthogonality)
 };
 struct Anim
 {
@@ -93,8 +96,26 @@ struct Scene
         V        n;
         T        k1, k2, area_kappa, bary0, bary1;
         T        w[4];
+        // generalized DOFs (narrow_phase.cpp:2600-2670): soft vertices with their weights, a rigid
+        // side as its body's four rows with the weights sum_side w (1, X)
+        uint32_t gr[8];
+        T        gw[8];
+        int      ng = 0;
     };
     std::vector<Contact> contacts;
+    // ---- affine bodies: rows [nv, nr) are 4 per body (p, a0, a1, a2); rigid vertices keep rows
+    // of their own with identity blocks and follow their body
+    uint32_t nr = 0, nb = 0;
+    struct Body
+    {
+        uint32_t row0, v0, v1;
+        bool     fixed;
+        T        stiff;  // kappa * volume
+        T        M[4][4];
+    };
+    std::vector<Body>     bodies;
+    std::vector<uint32_t> vert_body;  // per vertex: body id or ~0u
+    std::vector<V>        body_X;     // per vertex: body-frame position (the rest position)
     // ---- per-iteration stats
     T   toi = 1, alpha = 1, max_dx = 0, pin_delta = 0;
     struct CcdArg
@@ -279,6 +300,7 @@ struct Scene
             for (size_t k = 0; k < m.pinned.size(); k++)
             {
                 is_fixed[bv + m.pinned[k]] = 1;
+                if (m.mat.body != 0) continue;  // a pinned rigid vertex fixes its body (no target)
                 fixed_verts.push_back(bv + m.pinned[k]);
                 fixed_anim.push_back(m.pin_anim[k]);
             }
@@ -369,7 +391,8 @@ struct Scene
         std::vector<float> moff(meshes.size()), mdh(meshes.size());
         for (size_t mi = 0; mi < meshes.size(); mi++)
         {
-            float so = 0.5f * (float)meshes[mi].mat.thickness, sd = (float)meshes[mi].mat.d_hat;
+            float so = meshes[mi].mat.body == 2 ? 0.0f : 0.5f * (float)meshes[mi].mat.thickness;
+            float sd = (float)meshes[mi].mat.d_hat;
             float dh = sd;
             float safe = 0.9f * min_dist[mi];
             if (!(so + dh < safe))
@@ -385,11 +408,78 @@ struct Scene
             offset[i] = T(moff[vert_mesh[i]]);
             dhat[i]   = T(mdh[vert_mesh[i]]);
         }
-        // ---- culled elements and their constants (init_sim_data.cpp:189-480)
+        // ---- affine bodies (init_sim_data.cpp:520-760): one per rigid mesh, rows after the vertices
+        bodies.clear();
+        vert_body.assign(nv, ~0u);
+        body_X.assign(nv, V());
+        {
+            uint32_t vbase = 0;
+            for (size_t mi = 0; mi < meshes.size(); mi++)
+            {
+                const MeshIn& m  = meshes[mi];
+                uint32_t      mv = (uint32_t)(m.x.size() / 3);
+                if (m.mat.body != 0)
+                {
+                    Body b{};
+                    b.row0 = nv + 4u * (uint32_t)bodies.size();
+                    b.v0   = vbase;
+                    b.v1   = vbase + mv;
+                    b.fixed = false;
+                    for (uint32_t i = b.v0; i < b.v1; i++)
+                    {
+                        vert_body[i] = (uint32_t)bodies.size();
+                        body_X[i]    = rest_x[i];
+                        b.fixed      = b.fixed || is_fixed[i];
+                    }
+                    float mm = 0.0f, mx[3] = {0, 0, 0}, mxx[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, vol = 0.0f;
+                    if (m.mat.body == 1)
+                    {
+                        // shell: vertex masses (area share of area * thickness * density)
+                        for (uint32_t i = b.v0; i < b.v1; i++)
+                        {
+                            float w = vvol[i] / body_vol[mi];
+                            float vm = w * body_mass[mi];
+                            V3<float> X = rxf[i];
+                            mm += vm;
+                            for (int a = 0; a < 3; a++)
+                            {
+                                mx[a] += vm * X[a];
+                                for (int c = 0; c < 3; c++) mxx[a][c] = mxx[a][c] + vm * (X[a] * X[c]);
+                            }
+                        }
+                        vol = body_vol[mi];
+                    }
+                    else
+                    {
+                        std::vector<uint32_t> ft;
+                        for (uint32_t f = 0; f < nf; f++)
+                            if (face_mesh[f] == mi)
+                                for (int k = 0; k < 3; k++) ft.push_back(faces[3 * f + k]);
+                        dyadic_mass(rxf, ft.data(), ft.size() / 3, (float)m.mat.density, mm, mx, mxx);
+                        for (size_t f = 0; f < ft.size() / 3; f++)
+                        {
+                            V3<float> v0 = rxf[ft[3 * f]], v1 = rxf[ft[3 * f + 1]], v2 = rxf[ft[3 * f + 2]];
+                            vol += dot(v0, cross(v1 - v0, v2 - v0)) / 6.0f;
+                        }
+                    }
+                    float Mf[4][4];
+                    body_mass_matrix(mm, mx, mxx, Mf);
+                    for (int a = 0; a < 4; a++)
+                        for (int c = 0; c < 4; c++) b.M[a][c] = T(Mf[a][c]);
+                    b.stiff = T((float)m.mat.rigid_kappa * vol);
+                    bodies.push_back(b);
+                }
+                vbase += mv;
+            }
+        }
+        nb = (uint32_t)bodies.size();
+        nr = nv + 4 * nb;
+        // ---- culled elements and their constants (init_sim_data.cpp:189-480); rigid meshes have none
         sf.clear(); sf_Dm.clear(); sf_area.clear(); sf_mu.clear(); sf_lambda.clear();
         for (uint32_t f = 0; f < nf; f++)
         {
             const uint32_t* t = &faces[3 * f];
+            if (vert_body[t[0]] != ~0u) continue;
             if (is_fixed[t[0]] && is_fixed[t[1]] && is_fixed[t[2]]) continue;
             sf.push_back(t[0]); sf.push_back(t[1]); sf.push_back(t[2]);
             float d[4];
@@ -408,6 +498,7 @@ struct Scene
         for (uint32_t h = 0; h < nh; h++)
         {
             const uint32_t* e = &hinges[4 * h];
+            if (vert_body[e[0]] != ~0u) continue;
             if (is_fixed[e[0]] && is_fixed[e[1]] && is_fixed[e[2]] && is_fixed[e[3]]) continue;
             for (int k = 0; k < 4; k++) bh.push_back(e[k]);
             V3<float> p0 = rxf[e[0]], p1 = rxf[e[1]], p2 = rxf[e[2]], p3 = rxf[e[3]];
@@ -420,7 +511,7 @@ struct Scene
             bh_k.push_back(T((float)meshes[hinge_mesh[h]].mat.bending));
         }
         // ---- CSR adjacency of material force vertices (init_sim_data.cpp:760-896)
-        std::vector<std::vector<uint32_t>> adj(nv);
+        std::vector<std::vector<uint32_t>> adj(nr);
         auto ins = [&](uint32_t a, uint32_t b) {
             auto& l = adj[a];
             if (std::find(l.begin(), l.end(), b) == l.end()) l.push_back(b);
@@ -433,16 +524,20 @@ struct Scene
             for (int i = 0; i < 4; i++)
                 for (int j = 0; j < 4; j++)
                     if (i != j) ins(bh[4 * h + i], bh[4 * h + j]);
+        for (const Body& b : bodies)
+            for (uint32_t i = 0; i < 4; i++)
+                for (uint32_t j = 0; j < 4; j++)
+                    if (i != j) ins(b.row0 + i, b.row0 + j);
         for (auto& l : adj) std::sort(l.begin(), l.end());
-        csr.assign(nv + 1, 0);
-        uint32_t cur = nv + 1;
-        for (uint32_t i = 0; i < nv; i++)
+        csr.assign(nr + 1, 0);
+        uint32_t cur = nr + 1;
+        for (uint32_t i = 0; i < nr; i++)
         {
             csr[i] = cur;
             cur += (uint32_t)adj[i].size();
         }
-        csr[nv] = cur;
-        for (uint32_t i = 0; i < nv; i++)
+        csr[nr] = cur;
+        for (uint32_t i = 0; i < nr; i++)
             for (uint32_t c : adj[i]) csr.push_back(c);
         auto offs = [&](const uint32_t* el, int N, std::vector<uint16_t>& out) {
             for (int i = 0; i < N; i++)
@@ -456,15 +551,22 @@ struct Scene
         sf_off.clear(); bh_off.clear();
         for (size_t f = 0; f < sf.size() / 3; f++) offs(&sf[3 * f], 3, sf_off);
         for (size_t h = 0; h < bh.size() / 4; h++) offs(&bh[4 * h], 4, bh_off);
-        // ---- state (restart_system, solver_interface.cpp:184-205)
+        // ---- state (restart_system, solver_interface.cpp:184-205); a body starts at q = (0, I)
         x.assign(rest_x.begin(), rest_x.end());
-        x_outer = rest_x;
-        v_outer.assign(nv, V());
-        v.assign(nv, V());
+        for (uint32_t b = 0; b < nb; b++)
+        {
+            x.push_back(V());
+            x.push_back(V(T(1), T(0), T(0)));
+            x.push_back(V(T(0), T(1), T(0)));
+            x.push_back(V(T(0), T(0), T(1)));
+        }
+        x_outer = x;
+        v_outer.assign(nr, V());
+        v.assign(nr, V());
         x_step = x_iter = x_tilde = dx = x;
         target = rest_x;
-        cgB.assign(nv, V());
-        diag.assign(nv, M3<T>());
+        cgB.assign(nr, V());
+        diag.assign(nr, M3<T>());
         offdiag.assign(csr.size(), M3<T>());
         tree_face.init(TreeFace, faces.data(), nf);
         tree_edge.init(TreeEdge, edges.data(), ne);
@@ -477,7 +579,7 @@ struct Scene
     {
         update_targets(float(frame) * float(prm.dt));
         for (uint32_t vid : fixed_verts) v_outer[vid] = (target[vid] - x_outer[vid]) / T(float(prm.dt));
-        for (uint32_t i = 0; i < nv; i++)
+        for (uint32_t i = 0; i < nr; i++)
         {
             x_step[i] = x_outer[i];
             x[i]      = x_outer[i];
@@ -492,10 +594,14 @@ struct Scene
     {
         const T   h = T(float(prm.dt));
         const V   g(T(float(prm.gravity[0])), T(float(prm.gravity[1])), T(float(prm.gravity[2])));
-        for (uint32_t i = 0; i < nv; i++)
+        for (uint32_t i = 0; i < nr; i++)
         {
-            V vp = v[i] + h * g;
-            if (is_fixed[i]) vp = v[i];
+            // fn_abd_predict_position (newton_solver.cpp:640-675): gravity on the translation
+            // block only; a fixed body (or vertex) keeps its velocity
+            bool fixed = i < nv ? (is_fixed[i] != 0) : bodies[(i - nv) / 4].fixed;
+            bool grav  = i < nv || (i - nv) % 4 == 0;
+            V    vp    = grav ? v[i] + h * g : v[i];
+            if (fixed) vp = v[i];
             x_iter[i]  = x_step[i];
             x_tilde[i] = x_step[i] + h * vp;
             x[i]       = x_step[i];
@@ -508,6 +614,12 @@ struct Scene
         const T h = T(float(prm.dt)), h2inv = T(1) / (h * h), kD = T(float(prm.k_dirichlet));
         for (uint32_t i = 0; i < nv; i++)
         {
+            if (vert_body[i] != ~0u)
+            {
+                cgB[i]  = V();
+                diag[i] = M3<T>::diag(T(1));
+                continue;
+            }
             T     m    = vert_mass[i];
             V     grad = ((-m) * h2inv) * (x[i] - x_tilde[i]);
             M3<T> hess = M3<T>::diag((T(1) * m) * h2inv);
@@ -524,7 +636,7 @@ struct Scene
             const T kap = T(float(prm.kappa)), fy = T(float(prm.floor_y));
             for (uint32_t i = 0; i < nv; i++)
             {
-                if (is_fixed[i]) continue;
+                if (is_fixed[i] || vert_body[i] != ~0u) continue;
                 T dh = dhat[i], th = offset[i], dist = x[i].y - fy;
                 if (dist - th < dh)
                 {
@@ -614,6 +726,88 @@ struct Scene
                 }
             }
         }
+        for (const Body& b : bodies) body_terms(b, h2inv, kD);
+    }
+
+    // fn_evaluate_abd_inertia / _orthogonality / _ground_collision and
+    // fn_material_energy_assembly_affine_body (newton_solver.cpp:803-813, 1176-1413) for one body.
+    BodyGH<T> body_gh(const Body& b, T h2inv, T kD) const
+    {
+        BodyGH<T> o;
+        V         q[4] = {x[b.row0], x[b.row0 + 1], x[b.row0 + 2], x[b.row0 + 3]};
+        V         qt[4] = {x_tilde[b.row0], x_tilde[b.row0 + 1], x_tilde[b.row0 + 2], x_tilde[b.row0 + 3]};
+        abd_inertia(q, qt, b.M, h2inv, b.fixed, kD, o);
+        abd_orthogonality(q, b.stiff, o);
+        if (prm.use_floor)
+            for (uint32_t i = b.v0; i < b.v1; i++)
+                if (!is_fixed[i])
+                    abd_ground_vertex(x[i], body_X[i], T(float(prm.floor_y)), dhat[i], offset[i], vert_area[i],
+                                      T(float(prm.kappa)), o);
+        return o;
+    }
+    void body_terms(const Body& b, T h2inv, T kD)
+    {
+        BodyGH<T> o = body_gh(b, h2inv, kD);
+        for (int ii = 0; ii < 4; ii++)
+        {
+            uint32_t r = b.row0 + ii;
+            cgB[r]     = -o.g[ii];
+            diag[r]    = o.H[ii];
+            for (int jj = 0; jj < 4; jj++)
+                if (jj != ii) offdiag[csr[r] + (jj < ii ? jj : jj - 1)] = o.H[abd_block(ii, jj)];
+        }
+    }
+
+    // narrow_phase.cpp:2600-2670: generalized DOFs of a contact side: soft vertices as they are, a
+    // rigid side as its body's rows with weights sum_side w (1, X)
+    void lift(Contact& c) const
+    {
+        c.ng = 0;
+        for (int side = 0; side < 2; side++)
+        {
+            int      k0 = c.type == 4 ? (side == 0 ? 0 : 1) : 2 * side;
+            int      k1 = c.type == 4 ? (side == 0 ? 1 : 4) : 2 * side + 2;
+            uint32_t bid = vert_body[c.idx[k0]];
+            if (bid == ~0u)
+            {
+                for (int k = k0; k < k1; k++)
+                {
+                    c.gr[c.ng] = c.idx[k];
+                    c.gw[c.ng] = c.w[k];
+                    c.ng++;
+                }
+                continue;
+            }
+            T aw[4] = {T(0), T(0), T(0), T(0)};
+            for (int k = k0; k < k1; k++)
+            {
+                V X = body_X[c.idx[k]];
+                aw[0] = aw[0] + c.w[k] * T(1);
+                aw[1] = aw[1] + c.w[k] * X.x;
+                aw[2] = aw[2] + c.w[k] * X.y;
+                aw[3] = aw[3] + c.w[k] * X.z;
+            }
+            for (int k = 0; k < 4; k++)
+            {
+                c.gr[c.ng] = bodies[bid].row0 + k;
+                c.gw[c.ng] = aw[k];
+                c.ng++;
+            }
+        }
+    }
+
+    // fn_apply_dx_affine_bodies (newton_solver.cpp:505-526): x = A X + p for every rigid vertex
+    void rigid_positions()
+    {
+        for (const Body& b : bodies)
+        {
+            V p = x[b.row0], a0 = x[b.row0 + 1], a1 = x[b.row0 + 2], a2 = x[b.row0 + 3];
+            for (uint32_t i = b.v0; i < b.v1; i++)
+            {
+                V X  = body_X[i];
+                x[i] = ((a0 * X.x + a1 * X.y) + a2 * X.z) + p;
+            }
+        }
     }
 
     // ---- broad phase (newton_solver.cpp:3174-3235)
@@ -657,6 +851,7 @@ struct Scene
             uint32_t        vid = pairs_vf[2 * p], fid = pairs_vf[2 * p + 1];
             const uint32_t* f   = &faces[3 * fid];
             if (vid == f[0] || vid == f[1] || vid == f[2]) continue;
+            if (vert_body[vid] != ~0u && vert_body[vid] == vert_body[f[0]]) continue;  // narrow_phase.cpp:806-807
             V     P = x[vid], t0 = x[f[0]], t1 = x[f[1]], t2 = x[f[2]];
             V3<T> b = pt_coeff_unclassified(P, t0, t1, t2);
             V     X = (b.x * (P - t0) + b.y * (P - t1)) + b.z * (P - t2);
@@ -680,6 +875,7 @@ struct Scene
                 c.w[1]       = -b.x;
                 c.w[2]       = -b.y;
                 c.w[3]       = (b.x + b.y) - T(1);
+                lift(c);
                 contacts.push_back(c);
             }
         }
@@ -689,6 +885,7 @@ struct Scene
             const uint32_t* A = &edges[2 * l];
             const uint32_t* B = &edges[2 * r];
             if (A[0] == B[0] || A[0] == B[1] || A[1] == B[0] || A[1] == B[1]) continue;
+            if (vert_body[A[0]] != ~0u && vert_body[A[0]] == vert_body[B[0]]) continue;  // narrow_phase.cpp:972-974
             V     a0 = x[A[0]], a1 = x[A[1]], b0 = x[B[0]], b1 = x[B[1]];
             C4<T> c4 = ee_coeff_unclassified(a0, a1, b0, b1);
             V     x0 = c4.v[0] * a0 + c4.v[1] * a1;
@@ -714,6 +911,7 @@ struct Scene
                 c.w[1]       = T(1) - c4.v[0];
                 c.w[2]       = -c4.v[2];
                 c.w[3]       = c4.v[2] - T(1);
+                lift(c);
                 contacts.push_back(c);
             }
         }
@@ -725,11 +923,11 @@ struct Scene
         for (const Contact& c : contacts)
         {
             M3<T> nn = outer(c.n, c.n);
-            for (int k = 0; k < 4; k++)
+            for (int k = 0; k < c.ng; k++)
             {
-                uint32_t i = c.idx[k];
-                cgB[i]     = cgB[i] + (c.k1 * c.w[k]) * c.n;
-                diag[i]    = diag[i] + ((c.k2 * c.w[k]) * c.w[k]) * nn;
+                uint32_t i = c.gr[k];
+                cgB[i]     = cgB[i] + (c.k1 * c.gw[k]) * c.n;
+                diag[i]    = diag[i] + ((c.k2 * c.gw[k]) * c.gw[k]) * nn;
             }
         }
     }
@@ -738,7 +936,7 @@ struct Scene
     // (the per-pair SpMV form of narrow_phase.cpp:2730-2775).
     void spmv(const std::vector<V>& p, std::vector<V>& y) const
     {
-        for (uint32_t i = 0; i < nv; i++)
+        for (uint32_t i = 0; i < nr; i++)
         {
             V s = diag[i] * p[i];
             for (uint32_t k = csr[i]; k < csr[i + 1]; k++) s = s + offdiag[k] * p[csr[k]];
@@ -747,12 +945,12 @@ struct Scene
         for (const Contact& c : contacts)
         {
             M3<T> xx = c.k2 * outer(c.n, c.n);
-            for (int j = 0; j < 4; j++)
+            for (int j = 0; j < c.ng; j++)
             {
                 V o;
-                for (int jj = 0; jj < 4; jj++)
-                    if (j != jj) o = o + ((c.w[j] * c.w[jj]) * xx) * p[c.idx[jj]];
-                y[c.idx[j]] = y[c.idx[j]] + o;
+                for (int jj = 0; jj < c.ng; jj++)
+                    if (j != jj) o = o + ((c.gw[j] * c.gw[jj]) * xx) * p[c.gr[jj]];
+                y[c.gr[j]] = y[c.gr[j]] + o;
             }
         }
     }
@@ -776,18 +974,18 @@ struct Scene
     // rz_k <= tol^2 rz_0 (pcg_rel_tol > 0) is tested at the same iterations.
     void pcg()
     {
-        std::vector<M3<T>> Minv(nv);
-        for (uint32_t i = 0; i < nv; i++) Minv[i] = inverse(diag[i]);
-        std::vector<V> r(cgB), z(nv), p(nv, V()), q(nv, V());
-        dx.assign(nv, V());
-        std::vector<T> tmp(nv), tmp2(nv);
+        std::vector<M3<T>> Minv(nr);
+        for (uint32_t i = 0; i < nr; i++) Minv[i] = inverse(diag[i]);
+        std::vector<V> r(cgB), z(nr), p(nr, V()), q(nr, V());
+        dx.assign(nr, V());
+        std::vector<T> tmp(nr), tmp2(nr);
         T rz_old = 0, rz0 = 0;
         int it = 0;
         const T rel2 = T(prm.pcg_rel_tol * prm.pcg_rel_tol), absl = T(prm.pcg_abs_tol);
         pcg_rel = 0;
         for (it = 0; it < prm.pcg_max; it++)
         {
-            for (uint32_t i = 0; i < nv; i++)
+            for (uint32_t i = 0; i < nr; i++)
             {
                 z[i]    = Minv[i] * r[i];
                 tmp[i]  = dot(r[i], z[i]);
@@ -804,12 +1002,12 @@ struct Scene
             }
             T beta = rz_old < T(1e-10f) ? T(0) : rz / rz_old;
             rz_old = rz;
-            for (uint32_t i = 0; i < nv; i++) p[i] = z[i] + beta * p[i];
+            for (uint32_t i = 0; i < nr; i++) p[i] = z[i] + beta * p[i];
             spmv(p, q);
-            for (uint32_t i = 0; i < nv; i++) tmp[i] = dot(p[i], q[i]);
+            for (uint32_t i = 0; i < nr; i++) tmp[i] = dot(p[i], q[i]);
             T pq    = tree_sum(tmp);
             T alpha = pq < T(1e-10f) ? T(0) : rz / pq;
-            for (uint32_t i = 0; i < nv; i++)
+            for (uint32_t i = 0; i < nr; i++)
             {
                 dx[i] = dx[i] + alpha * p[i];
                 r[i]  = r[i] - alpha * q[i];
@@ -844,6 +1042,7 @@ struct Scene
             uint32_t        vid = pairs_vf[2 * p], fid = pairs_vf[2 * p + 1];
             const uint32_t* f   = &faces[3 * fid];
             if (vid == f[0] || vid == f[1] || vid == f[2]) continue;
+            if (vert_body[vid] != ~0u && vert_body[vid] == vert_body[f[0]]) continue;  // narrow_phase.cpp:324-325
             V a0[4] = {xb[vid], xb[f[0]], xb[f[1]], xb[f[2]]};
             V a1[4] = {xe[vid], xe[f[0]], xe[f[1]], xe[f[2]]};
             T toi   = accd(a0, a1, offset[vid] + offset[f[0]], 0);
@@ -859,6 +1058,7 @@ struct Scene
             const uint32_t* A = &edges[2 * l];
             const uint32_t* B = &edges[2 * r];
             if (A[0] == B[0] || A[0] == B[1] || A[1] == B[0] || A[1] == B[1]) continue;
+            if (vert_body[A[0]] != ~0u && vert_body[A[0]] == vert_body[B[0]]) continue;  // narrow_phase.cpp:471-472
             V a0[4] = {xb[A[0]], xb[A[1]], xb[B[0]], xb[B[1]]};
             V a1[4] = {xe[A[0]], xe[A[1]], xe[B[0]], xe[B[1]]};
             T toi   = accd(a0, a1, offset[A[0]] + offset[B[0]], 1);
@@ -871,9 +1071,12 @@ struct Scene
         return t;
     }
 
+    // device_apply_dx (newton_solver.cpp:3665-3681): soft vertices and body DOFs, then the rigid
+    // vertices from their bodies
     void apply_dx(T a)
     {
-        for (uint32_t i = 0; i < nv; i++) x[i] = x_iter[i] + a * dx[i];
+        for (uint32_t i = 0; i < nr; i++) x[i] = x_iter[i] + a * dx[i];
+        rigid_positions();
     }
 
     // newton_solver.cpp:3685-3884 (CCD line search, no energy backtracking)
@@ -889,7 +1092,7 @@ struct Scene
         }
         alpha  = toi;
         max_dx = 0;
-        for (uint32_t i = 0; i < nv; i++) max_dx = std::max(max_dx, length(dx[i]));
+        for (uint32_t i = 0; i < nr; i++) max_dx = std::max(max_dx, length(dx[i]));
         if (max_dx < T(float(prm.newton_tol)) * T(float(prm.dt)))
         {
             apply_dx(alpha);
@@ -951,7 +1154,7 @@ struct Scene
     {
         const T h = T(float(prm.dt));
         const T damp = std::exp(-T(float(prm.damping)) * h);
-        for (uint32_t i = 0; i < nv; i++)
+        for (uint32_t i = 0; i < nr; i++)
         {
             V vel     = (x[i] - x_step[i]) / h;
             vel       = vel * damp;

@@ -53,6 +53,8 @@ def load() -> C.CDLL:
     lib.orc_lbvh_get.argtypes = [vp, C.c_char_p, vp, C.c_int64]
     lib.orc_lbvh_query.restype = C.c_int64
     lib.orc_lbvh_query.argtypes = [vp, fp, C.c_uint32, C.c_int, C.POINTER(C.c_uint32), C.c_int64]
+    lib.orc_abd_terms.argtypes = [C.c_int, dp, dp, dp, C.c_double, C.c_int, C.c_double, C.c_double, dp, dp]
+    lib.orc_dyadic_mass.argtypes = [fp, C.c_int, C.POINTER(C.c_int32), C.c_int, C.c_float, dp]
     _LIB = lib
     return lib
 
@@ -226,3 +228,22 @@ def dihedral(x, precision: int = 64):
     g = np.zeros(12)
     a = load().orc_dihedral(_d(x), precision, g.ctypes.data_as(C.POINTER(C.c_double)))
     return a, g.reshape(4, 3)
+
+
+def abd_terms(q, q_tilde, M, h2inv: float, fixed: bool, kD: float, stiff: float, precision: int = 64):
+    """Affine-body inertia + orthogonality of one body: (gradient (4, 3), Hessian blocks (16, 3, 3))."""
+    g = np.zeros(12)
+    H = np.zeros(144)
+    load().orc_abd_terms(precision, _d(np.ravel(q)), _d(np.ravel(q_tilde)), _d(np.ravel(M)), h2inv, int(fixed), kD,
+                         stiff, _d(g), _d(H))
+    return g.reshape(4, 3), H.reshape(16, 3, 3)
+
+
+def dyadic_mass(x, tris, rho: float):
+    """Mass, first moment (3) and second moment (3, 3) of the solid bounded by a closed triangle surface."""
+    x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 3)
+    t = np.ascontiguousarray(tris, dtype=np.int32).reshape(-1, 3)
+    out = np.zeros(13)
+    load().orc_dyadic_mass(x.ctypes.data_as(C.POINTER(C.c_float)), len(x), t.ctypes.data_as(C.POINTER(C.c_int32)),
+                           len(t), rho, _d(out))
+    return out[0], out[1:4], out[4:].reshape(3, 3)
