@@ -125,6 +125,7 @@ struct QueryArgs
     uint32_t        n_queries;
     const uint32_t* vert_scene;  // per vertex: segment of the query (null: segment 0)
     const uint8_t*  active;      // per segment: 0 skips its queries (null: all)
+    const uint8_t*  row_kind;    // QueryVerts: rows of kind >= 2 (affine-body DOFs) are not points (null: all)
     const float*    x0;
     const float*    x1;
     const uint32_t* edges;

@@ -69,17 +69,22 @@ struct ContactView
     const float4*   aux;
     const uint32_t* count;
     uint32_t        capacity;
+    // generalized DOFs, two entries per contact: up to 8 rows (~0u: unused) and their weights (a
+    // soft vertex with its weight, a rigid side as its body's 4 rows with weights sum w (1, X))
+    const uint4*    gr;
+    const float4*   gw;
 };
-// Per-vertex contact slots: entries list[ptr[i] .. ptr[i+1]) are (contact << 2 | slot); ptr null
-// means no contacts.
+// Per-row contact lists: entries list[ptr[i] .. ptr[i+1]) are (contact << 3 | entry), entry one of
+// the contact's generalized DOFs; ptr null means no contacts.
 struct VertContacts
 {
     const uint32_t* ptr;
     const uint32_t* list;
-    const uint32_t* row;     // per slot: its vertex
+    const uint32_t* row;     // per slot: its row
     const float4*   coef;    // per slot: (k2 w_j n, contact id bits)
     float*          tc;      // per contact: sum_j w_j n.p_j of the vector being multiplied
-    uint32_t        nslots;  // 4 x contacts (set by the host before an SpMV)
+    uint32_t        nslots;  // an upper bound of the slots (sizes grids)
+    const uint32_t* total;   // the number of slots (ptr[rows])
 };
 struct ContactBuf
 {
@@ -94,6 +99,7 @@ struct ContactBuf
 struct VertArgs
 {
     uint32_t        nv;
+    const uint8_t*  row_kind;  // 0 soft vertex, 1 rigid vertex (identity row), 2+ body row
     const float*    x;
     const float*    x_tilde;
     const float*    mass;
@@ -129,6 +135,7 @@ struct DcdArgs
     float           kappa;
     int             fp64;  // mixed-precision mode: distances and barrier in double
     ContactBuf      out;
+    const uint32_t* vert_body;  // per row: body id or ~0u; pairs within one body are skipped
 };
 
 struct CcdArgs
@@ -152,6 +159,7 @@ struct CcdArgs
     uint32_t*       rest_count;
     SceneStatus*    sst;         // step bound per scene (scene of the pair's first vertex)
     const uint32_t* vert_scene;
+    const uint32_t* vert_body;   // per row: body id or ~0u; pairs within one body are skipped
 };
 
 struct LsArgs
@@ -167,6 +175,35 @@ struct LsArgs
     Status*        st;
     SceneStatus*   sst;
     SegView        seg;  // launched over the scene-aligned row blocks
+    const uint8_t* row_kind;  // body rows (kind >= 2) have no ground test
+};
+
+// Affine bodies (one per rigid mesh): rows row0 .. row0 + 3 hold q = (p, a0, a1, a2), vertices
+// [v0, v1) follow x = p + A X (X: body_X). Same layout as the host setup's body record.
+struct BodyDev
+{
+    uint32_t row0, v0, v1, fixed;
+    float    stiff;    // kappa * volume
+    float    M[16];    // mass matrix (times I3)
+    uint32_t blk[12];  // SELL blocks (ii, jj != ii), row-major
+};
+struct AbdArgs
+{
+    uint32_t       nb, n_rows, nblocks;
+    const BodyDev* bodies;
+    const float*   x;
+    const float*   x_tilde;
+    const float*   body_X;
+    const uint8_t* fixed;
+    const float*   offset;
+    const float*   dhat;
+    const float*   area;
+    float          h2inv, kD, kappa, floor_y;
+    int            use_floor;
+    float*         cgB;
+    float*         diag;
+    float*         diag_mat;
+    float*         sell_val;
 };
 
 struct PcgArgs
@@ -194,8 +231,17 @@ struct PcgArgs
 void launch_fill_u32(uint32_t* a, uint32_t n, uint32_t v, hipStream_t s);
 void launch_pin_velocity(uint32_t nfix, const uint32_t* ids, const float* targets, const float* x, float* v, float dt,
                          hipStream_t s);
-void launch_predict(uint32_t nv, const float* x_step, const float* v, const uint8_t* fixed, float h, const float g[3],
-                    float* x_iter, float* x_tilde, float* x, hipStream_t s);
+void launch_predict(uint32_t nv, const float* x_step, const float* v, const uint8_t* fixed, const uint8_t* row_kind,
+                    float h, const float g[3], float* x_iter, float* x_tilde, float* x, hipStream_t s);
+// Affine-body inertia, orthogonality and lifted ground terms into cgB, diag and the body rows'
+// SELL blocks (after launch_offdiag).
+void launch_abd_material(const AbdArgs& a, hipStream_t s);
+// x of every rigid vertex from its body's q.
+void launch_rigid_positions(uint32_t n_rows, const uint8_t* row_kind, const uint32_t* vert_body, const BodyDev* bodies,
+                            const float* body_X, float* x, hipStream_t s);
+// Generalized DOF lists of the contacts (after the DCD).
+void launch_contact_lift(const ContactView& c, unsigned grid, const uint32_t* vert_body, const BodyDev* bodies,
+                         const float* body_X, uint4* gr, float4* gw, hipStream_t s);
 void launch_frame_end(uint32_t nv, float* x_step, const float* x, float* v, float h, float damp, hipStream_t s);
 void launch_face_stage(uint32_t nsf, const uint32_t* sf, const float4* Dm, const float4* c, const float* x, float* stage,
                        hipStream_t s);

@@ -114,14 +114,16 @@ __global__ void k_pin_velocity(uint32_t nfix, const uint32_t* __restrict__ ids, 
 }
 
 // fn_predict_position, newton_solver.cpp:577-608
+// (+ fn_abd_predict_position, newton_solver.cpp:640-675: gravity on a body's translation row only)
 __global__ void k_predict(uint32_t nv, const float* __restrict__ x_step, const float* __restrict__ v,
-                          const uint8_t* __restrict__ fixed, float h, f3 g, float* __restrict__ x_iter,
-                          float* __restrict__ x_tilde, float* __restrict__ x)
+                          const uint8_t* __restrict__ fixed, const uint8_t* __restrict__ kind, float h, f3 g,
+                          float* __restrict__ x_iter, float* __restrict__ x_tilde, float* __restrict__ x)
 {
     uint32_t i = blockIdx.x * kB + threadIdx.x;
     if (i >= nv) return;
-    f3 xs = ld3(x_step, i), vi = ld3(v, i);
-    f3 vp = fixed[i] ? vi : vi + h * g;
+    f3   xs   = ld3(x_step, i), vi = ld3(v, i);
+    bool grav = kind[i] <= 2u;
+    f3   vp   = fixed[i] ? vi : (grav ? vi + h * g : vi);
     st3(x_iter, i, xs);
     st3(x_tilde, i, xs + h * vp);
     st3(x, i, xs);
@@ -202,6 +204,18 @@ __global__ __launch_bounds__(kB) void k_vert_assemble(VertArgs a)
 {
     uint32_t i = blockIdx.x * kB + threadIdx.x;
     if (i >= a.nv) return;
+    const uint8_t kind = a.row_kind[i];
+    if (kind >= 2u) return;  // body rows: k_abd_material
+    if (kind == 1u)
+    {
+        // a rigid vertex follows its body: an identity row that solves to dx = 0
+        m3 I = m3_zero();
+        I.m[0] = I.m[4] = I.m[8] = 1.0f;
+        st3(a.cgB, i, mk(0.0f, 0.0f, 0.0f));
+        st_planes(a.diag, a.nv, i, I);
+        if (a.diag_mat) st_planes(a.diag_mat, a.nv, i, I);
+        return;
+    }
     float m    = a.mass[i];
     f3    grad = ((-m) * a.h2inv) * (ld3(a.x, i) - ld3(a.x_tilde, i));
     m3    H    = m3_zero();
@@ -282,6 +296,159 @@ __global__ __launch_bounds__(kB) void k_offdiag(uint32_t nblocks, const uint32_t
     st_planes(val, nblocks, b, H);
 }
 
+// ---------------------------------------------------------------- affine bodies
+LCSX_D m3 m3_diag(float d)
+{
+    m3 a   = m3_zero();
+    a.m[0] = a.m[4] = a.m[8] = d;
+    return a;
+}
+LCSX_D m3 m3_add(const m3& a, const m3& b)
+{
+    m3 o;
+    for (int k = 0; k < 9; k++) o.m[k] = a.m[k] + b.m[k];
+    return o;
+}
+LCSX_D m3 m3_scale(float s, const m3& a)
+{
+    m3 o;
+    for (int k = 0; k < 9; k++) o.m[k] = s * a.m[k];
+    return o;
+}
+LCSX_D m3 m3_outer(f3 u, f3 v)
+{
+    float a[3] = {u.x, u.y, u.z}, b[3] = {v.x, v.y, v.z};
+    m3    o;
+    for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) o.m[3 * r + c] = a[r] * b[c];
+    return o;
+}
+LCSX_D m3 m3_transpose(const m3& a)
+{
+    m3 o;
+    for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) o.m[3 * r + c] = a.m[3 * c + r];
+    return o;
+}
+
+// One thread per body: fn_evaluate_abd_inertia (newton_solver.cpp:1176-1245), _orthogonality
+// (1246-1335), _ground_collision over the body's vertices (1336-1413) and the assembly of its 16
+// blocks (fn_material_energy_assembly_affine_body, 803-813): cgB = -gradient, the four diagonal
+// blocks, the twelve off-diagonal blocks into the SELL planes. Block order: diagonal blocks,
+// then (ii, jj != ii) row-major.
+__global__ __launch_bounds__(kB) void k_abd_material(AbdArgs a)
+{
+    uint32_t bi = blockIdx.x * kB + threadIdx.x;
+    if (bi >= a.nb) return;
+    const BodyDev& B = a.bodies[bi];
+    f3             q[4], dq[4], g[4];
+    for (int k = 0; k < 4; k++)
+    {
+        q[k]  = ld3(a.x, B.row0 + k);
+        dq[k] = q[k] - ld3(a.x_tilde, B.row0 + k);
+        g[k]  = mk(0.0f, 0.0f, 0.0f);
+    }
+    for (int ii = 0; ii < 4; ii++)
+        for (int jj = 0; jj < 4; jj++) g[ii] = g[ii] + B.M[4 * ii + jj] * dq[jj];
+    const float ah = (B.fixed ? (1.0f + a.kD) : 1.0f) * a.h2inv;
+    f3          G[4];
+    m3          H[16];
+    for (int k = 0; k < 4; k++)
+    {
+        G[k] = ah * g[k];
+        H[k] = m3_diag(ah * B.M[5 * k]);
+    }
+    int idx = 4;
+    for (int ii = 0; ii < 4; ii++)
+        for (int jj = 0; jj < 4; jj++)
+            if (ii != jj) H[idx++] = m3_diag(ah * B.M[4 * ii + jj]);
+    // orthogonality of A = (a0 a1 a2): 4 kappa V (sum_j (a_i . a_j) a_j - a_i) and exact Hessian
+    const f3 A[3] = {q[1], q[2], q[3]};
+    const float s4 = 4.0f * B.stiff;
+    for (int ii = 0; ii < 3; ii++)
+    {
+        f3 grad = -1.0f * A[ii];
+        for (int jj = 0; jj < 3; jj++) grad = grad + dot(A[ii], A[jj]) * A[jj];
+        G[1 + ii] = G[1 + ii] + s4 * grad;
+    }
+    m3 oh[6];
+    int o = 0;
+    for (int ii = 0; ii < 3; ii++)
+        for (int jj = ii; jj < 3; jj++)
+        {
+            m3 h;
+            if (ii == jj)
+            {
+                h = m3_add(m3_outer(A[ii], A[ii]), m3_diag(dot(A[ii], A[ii]) - 1.0f));
+                for (int kk = 0; kk < 3; kk++) h = m3_add(h, m3_outer(A[kk], A[kk]));
+            }
+            else
+                h = m3_add(m3_outer(A[jj], A[ii]), m3_diag(dot(A[ii], A[jj])));
+            oh[o++] = m3_add(m3_zero(), m3_scale(s4, h));
+        }
+    H[1]  = m3_add(H[1], oh[0]);
+    H[8]  = m3_add(H[8], oh[1]);
+    H[9]  = m3_add(H[9], oh[2]);
+    H[11] = m3_add(H[11], m3_transpose(oh[1]));
+    H[2]  = m3_add(H[2], oh[3]);
+    H[12] = m3_add(H[12], oh[4]);
+    H[14] = m3_add(H[14], m3_transpose(oh[2]));
+    H[15] = m3_add(H[15], m3_transpose(oh[4]));
+    H[3]  = m3_add(H[3], oh[5]);
+    // ground barrier of the body's free vertices, lifted with weights (1, X)
+    if (a.use_floor)
+        for (uint32_t v = B.v0; v < B.v1; v++)
+        {
+            if (a.fixed[v]) continue;
+            float dist = a.x[3 * (size_t)v + 1] - a.floor_y, th = a.offset[v], dh = a.dhat[v];
+            if (!(dist - th < dh)) continue;
+            const f3    n     = mk(0.0f, 1.0f, 0.0f);
+            float       stiff = a.kappa * a.area[v];
+            float       k1    = stiff * barrier_d1(dist - th, dh);
+            float       k2    = stiff * barrier_d2(dist - th, dh);
+            f3          gr    = k1 * n;
+            m3          hs    = m3_scale(k2, m3_outer(n, n));
+            const float w[4]  = {1.0f, a.body_X[3 * (size_t)v], a.body_X[3 * (size_t)v + 1], a.body_X[3 * (size_t)v + 2]};
+            int         id    = 4;
+            for (int ii = 0; ii < 4; ii++)
+            {
+                G[ii] = G[ii] + w[ii] * gr;
+                for (int jj = 0; jj < 4; jj++)
+                {
+                    m3 hh = m3_scale(w[ii] * w[jj], hs);
+                    if (ii == jj) H[ii] = m3_add(H[ii], hh);
+                    else
+                    {
+                        H[id] = m3_add(H[id], hh);
+                        id++;
+                    }
+                }
+            }
+        }
+    for (int k = 0; k < 4; k++)
+    {
+        uint32_t r = B.row0 + k;
+        st3(a.cgB, r, -G[k]);
+        st_planes(a.diag, a.n_rows, r, H[k]);
+        if (a.diag_mat) st_planes(a.diag_mat, a.n_rows, r, H[k]);
+    }
+    for (int k = 0; k < 12; k++) st_planes(a.sell_val, a.nblocks, B.blk[k], H[4 + k]);
+}
+
+// fn_apply_dx_affine_bodies (newton_solver.cpp:505-526): x = A X + p for every rigid vertex.
+__global__ __launch_bounds__(kB) void k_rigid_positions(uint32_t n, const uint8_t* __restrict__ kind,
+                                                         const uint32_t* __restrict__ vert_body,
+                                                         const BodyDev* __restrict__ bodies, const float* __restrict__ body_X,
+                                                         float* __restrict__ x)
+{
+    uint32_t i = blockIdx.x * kB + threadIdx.x;
+    if (i >= n || kind[i] != 1u) return;
+    uint32_t r  = bodies[vert_body[i]].row0;
+    f3       p  = ld3(x, r), a0 = ld3(x, r + 1), a1 = ld3(x, r + 2), a2 = ld3(x, r + 3);
+    f3       X  = ld3(body_X, i);
+    st3(x, i, ((X.x * a0 + X.y * a1) + X.z * a2) + p);
+}
+
 // ---------------------------------------------------------------- narrow phase
 // Contact record (the reference's CollisionPairTemplate, collision_data.h:48-127):
 // idx (type bit 31 in idx.z for VF, idx.w for EE), (n, k2), weights, (k1, area*kappa, bary0, bary1).
@@ -306,12 +473,15 @@ __global__ __launch_bounds__(kB) void k_dcd_vf(DcdArgs a)
     {
         uint32_t vid = a.pairs[2 * (size_t)p], fid = a.pairs[2 * (size_t)p + 1];
         uint32_t f0 = a.faces[3 * fid], f1 = a.faces[3 * fid + 1], f2 = a.faces[3 * fid + 2];
+        // shared vertex, or both sides on one rigid body (narrow_phase.cpp:806-807)
+        bool     skip = (vid == f0 || vid == f1 || vid == f2) ||
+                    (a.vert_body[vid] != kInvalid && a.vert_body[vid] == a.vert_body[f0]);
         bool     emit = false;
         uint4    idx  = make_uint4(vid, f0, f1 | 0x80000000u, f2);
         f3       nrm  = mk(0.0f, 0.0f, 0.0f);
         float    k1 = 0.0f, k2 = 0.0f, ak = 0.0f;
         f3       b  = mk(0.0f, 0.0f, 0.0f);
-        if (!(vid == f0 || vid == f1 || vid == f2) && a.fp64)
+        if (!skip && a.fp64)
         {
             // mixed-precision mode: barycentrics, distance and barrier in double
             using namespace fp64;
@@ -334,7 +504,7 @@ __global__ __launch_bounds__(kB) void k_dcd_vf(DcdArgs a)
                 emit      = true;
             }
         }
-        else if (!(vid == f0 || vid == f1 || vid == f2))
+        else if (!skip)
         {
             f3 P = ld3(a.x, vid), t0 = ld3(a.x, f0), t1 = ld3(a.x, f1), t2 = ld3(a.x, f2);
             b         = pt_coeff_unclassified(P, t0, t1, t2);
@@ -365,12 +535,15 @@ __global__ __launch_bounds__(kB) void k_dcd_ee(DcdArgs a)
     {
         uint32_t l = a.pairs[2 * (size_t)p], r = a.pairs[2 * (size_t)p + 1];
         uint32_t a0 = a.edges[2 * l], a1 = a.edges[2 * l + 1], b0 = a.edges[2 * r], b1 = a.edges[2 * r + 1];
+        // shared vertex, or both edges on one rigid body (narrow_phase.cpp:972-974)
+        bool     skip = (a0 == b0 || a0 == b1 || a1 == b0 || a1 == b1) ||
+                    (a.vert_body[a0] != kInvalid && a.vert_body[a0] == a.vert_body[b0]);
         bool     emit = false;
         uint4    idx  = make_uint4(a0, a1, b0, b1 | 0x80000000u);
         f3       nrm  = mk(0.0f, 0.0f, 0.0f);
         float    k1 = 0.0f, k2 = 0.0f, ak = 0.0f;
         c4       c  = {{0.0f, 0.0f, 0.0f, 0.0f}};
-        if (!(a0 == b0 || a0 == b1 || a1 == b0 || a1 == b1) && a.fp64)
+        if (!skip && a.fp64)
         {
             using namespace fp64;
             d3     pa0 = wide(ld3(a.x, a0)), pa1 = wide(ld3(a.x, a1)), pb0 = wide(ld3(a.x, b0)), pb1 = wide(ld3(a.x, b1));
@@ -392,7 +565,7 @@ __global__ __launch_bounds__(kB) void k_dcd_ee(DcdArgs a)
                 emit      = true;
             }
         }
-        else if (!(a0 == b0 || a0 == b1 || a1 == b0 || a1 == b1))
+        else if (!skip)
         {
             f3 pa0 = ld3(a.x, a0), pa1 = ld3(a.x, a1), pb0 = ld3(a.x, b0), pb1 = ld3(a.x, b1);
             c          = ee_coeff_unclassified(pa0, pa1, pb0, pb1);
@@ -423,18 +596,92 @@ __global__ void k_fill_u32(uint32_t* a, uint32_t n, uint32_t v)
 }
 
 // ---------------------------------------------------------------- contact adjacency
-// Per-vertex lists of (contact << 2 | slot), built once per Newton iteration after the DCD, so
-// that contact terms are gathered by the rows that own them instead of scattered with atomics.
+// Generalized DOFs of a contact (narrow_phase.cpp:2600-2670): each side is either soft (its
+// vertices with their weights) or one rigid body (its four rows with the weights sum_side w (1, X)).
+__global__ __launch_bounds__(kB) void k_contact_lift(ContactView c, const uint32_t* __restrict__ vert_body,
+                                                      const BodyDev* __restrict__ bodies, const float* __restrict__ body_X,
+                                                      uint4* __restrict__ gr, float4* __restrict__ gw)
+{
+    uint32_t n = min(*c.count, c.capacity);
+    for (uint32_t k = blockIdx.x * kB + threadIdx.x; k < n; k += gridDim.x * kB)
+    {
+        uint4    id     = c.idx[k];
+        float4   w4     = c.w[k];
+        bool     vf     = (id.z & 0x80000000u) != 0u;
+        uint32_t ids[4] = {id.x & 0x7fffffffu, id.y & 0x7fffffffu, id.z & 0x7fffffffu, id.w & 0x7fffffffu};
+        float    w[4]   = {w4.x, w4.y, w4.z, w4.w};
+        uint32_t r[8];
+        float    g[8];
+        int      ng = 0;
+        for (int side = 0; side < 2; side++)
+        {
+            int      k0 = vf ? (side == 0 ? 0 : 1) : 2 * side, k1 = vf ? (side == 0 ? 1 : 4) : 2 * side + 2;
+            uint32_t b  = vert_body ? vert_body[ids[k0]] : kInvalid;
+            if (b == kInvalid)
+            {
+                for (int j = k0; j < k1; j++)
+                {
+                    r[ng] = ids[j];
+                    g[ng] = w[j];
+                    ng++;
+                }
+                continue;
+            }
+            float aw[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+            for (int j = k0; j < k1; j++)
+            {
+                f3 X  = ld3(body_X, ids[j]);
+                aw[0] = aw[0] + w[j] * 1.0f;
+                aw[1] = aw[1] + w[j] * X.x;
+                aw[2] = aw[2] + w[j] * X.y;
+                aw[3] = aw[3] + w[j] * X.z;
+            }
+            uint32_t r0 = bodies[b].row0;
+            for (int j = 0; j < 4; j++)
+            {
+                r[ng] = r0 + j;
+                g[ng] = aw[j];
+                ng++;
+            }
+        }
+        for (int j = ng; j < 8; j++)
+        {
+            r[j] = kInvalid;
+            g[j] = 0.0f;
+        }
+        gr[2 * (size_t)k]     = make_uint4(r[0], r[1], r[2], r[3]);
+        gr[2 * (size_t)k + 1] = make_uint4(r[4], r[5], r[6], r[7]);
+        gw[2 * (size_t)k]     = make_float4(g[0], g[1], g[2], g[3]);
+        gw[2 * (size_t)k + 1] = make_float4(g[4], g[5], g[6], g[7]);
+    }
+}
+
+LCSX_D void gdofs(const ContactView& c, uint32_t k, uint32_t r[8], float w[8])
+{
+    uint4  a = c.gr[2 * (size_t)k], b = c.gr[2 * (size_t)k + 1];
+    float4 u = c.gw[2 * (size_t)k], v = c.gw[2 * (size_t)k + 1];
+    r[0] = a.x; r[1] = a.y; r[2] = a.z; r[3] = a.w; r[4] = b.x; r[5] = b.y; r[6] = b.z; r[7] = b.w;
+    w[0] = u.x; w[1] = u.y; w[2] = u.z; w[3] = u.w; w[4] = v.x; w[5] = v.y; w[6] = v.z; w[7] = v.w;
+}
+LCSX_D float gweight(const ContactView& c, uint32_t k, uint32_t j)
+{
+    float4 u = c.gw[2 * (size_t)k + (j >> 2)];
+    uint32_t l = j & 3u;
+    return l == 0 ? u.x : (l == 1 ? u.y : (l == 2 ? u.z : u.w));
+}
+
+// Per-row lists of (contact << 3 | entry), built once per Newton iteration after the DCD, so that
+// contact terms are gathered by the rows that own them instead of scattered with atomics.
 __global__ __launch_bounds__(kB) void k_vc_count(ContactView c, uint32_t* __restrict__ cnt)
 {
     uint32_t n = min(*c.count, c.capacity);
     for (uint32_t k = blockIdx.x * kB + threadIdx.x; k < n; k += gridDim.x * kB)
     {
-        uint4 id = c.idx[k];
-        atomicAdd(&cnt[id.x & 0x7fffffffu], 1u);
-        atomicAdd(&cnt[id.y & 0x7fffffffu], 1u);
-        atomicAdd(&cnt[id.z & 0x7fffffffu], 1u);
-        atomicAdd(&cnt[id.w & 0x7fffffffu], 1u);
+        uint32_t r[8];
+        float    w[8];
+        gdofs(c, k, r, w);
+        for (int j = 0; j < 8; j++)
+            if (r[j] != kInvalid) atomicAdd(&cnt[r[j]], 1u);
     }
 }
 
@@ -524,13 +771,15 @@ __global__ __launch_bounds__(kB) void k_vc_fill(ContactView c, const uint32_t* _
     uint32_t n = min(*c.count, c.capacity);
     for (uint32_t k = blockIdx.x * kB + threadIdx.x; k < n; k += gridDim.x * kB)
     {
-        uint4    id     = c.idx[k];
-        uint32_t ids[4] = {id.x & 0x7fffffffu, id.y & 0x7fffffffu, id.z & 0x7fffffffu, id.w & 0x7fffffffu};
-        for (uint32_t j = 0; j < 4; j++)
+        uint32_t r[8];
+        float    w[8];
+        gdofs(c, k, r, w);
+        for (uint32_t j = 0; j < 8; j++)
         {
-            uint32_t v   = ids[j];
+            uint32_t v = r[j];
+            if (v == kInvalid) continue;
             uint32_t pos = ptr[v] + atomicAdd(&cursor[v], 1u);
-            list[pos]    = (k << 2) | j;
+            list[pos]    = (k << 3) | j;
             row[pos]     = v;
         }
     }
@@ -551,12 +800,12 @@ __global__ __launch_bounds__(kB) void k_contact_assemble(ContactView c, VertCont
     for (int k = 0; k < 9; k++) D[k] = 0.0f;
     for (uint32_t e = e0; e < e1; e++)
     {
-        uint32_t ent = vc.list[e], k = ent >> 2, j = ent & 3u;
-        float4   nk = c.nk[k], w4 = c.w[k], ax = c.aux[k];
-        float    w[4]   = {w4.x, w4.y, w4.z, w4.w};
+        uint32_t ent = vc.list[e], k = ent >> 3, j = ent & 7u;
+        float4   nk = c.nk[k], ax = c.aux[k];
+        float    wj     = gweight(c, k, j);
         float    nv3[3] = {nk.x, nk.y, nk.z};
-        g               = g + (ax.x * w[j]) * mk(nk.x, nk.y, nk.z);
-        float s         = (nk.w * w[j]) * w[j];
+        g               = g + (ax.x * wj) * mk(nk.x, nk.y, nk.z);
+        float s         = (nk.w * wj) * wj;
 #pragma unroll
         for (int r = 0; r < 3; r++)
 #pragma unroll
@@ -576,10 +825,9 @@ __global__ __launch_bounds__(kB) void k_vc_coeffs(ContactView c, const uint32_t*
     uint32_t total = ptr[nv];
     for (uint32_t e = blockIdx.x * kB + threadIdx.x; e < total; e += gridDim.x * kB)
     {
-        uint32_t ent = list[e], k = ent >> 2, j = ent & 3u;
-        float4   nk = c.nk[k], w4 = c.w[k];
-        float    w[4] = {w4.x, w4.y, w4.z, w4.w};
-        f3       a    = (nk.w * w[j]) * mk(nk.x, nk.y, nk.z);
+        uint32_t ent = list[e], k = ent >> 3, j = ent & 7u;
+        float4   nk = c.nk[k];
+        f3       a  = (nk.w * gweight(c, k, j)) * mk(nk.x, nk.y, nk.z);
         coef[e]       = make_float4(a.x, a.y, a.z, __uint_as_float(k));
     }
 }
@@ -621,21 +869,27 @@ __global__ __launch_bounds__(1024) void k_chunk_table(SegView g, const uint32_t*
     if (t == 1023u) sc_cb0[S] = part[1023];
 }
 
-// t_c = sum_j w_j n.p_j for every contact, with p given by pfun.
+// t_c = sum_j w_j n.p_j over the contact's generalized DOFs, with p given by pfun.
+template <class PF>
+LCSX_D float contact_t1(const ContactView& c, uint32_t k, PF pfun)
+{
+    uint32_t r[8];
+    float    w[8];
+    gdofs(c, k, r, w);
+    float4 nk = c.nk[k];
+    f3     nn = mk(nk.x, nk.y, nk.z);
+    float  t  = w[0] * dot(nn, pfun(r[0]));
+    for (int j = 1; j < 8; j++)
+        if (r[j] != kInvalid) t = t + w[j] * dot(nn, pfun(r[j]));
+    return t;
+}
+
+// t_c for every contact, with p given by pfun.
 template <class PF>
 LCSX_D void contact_t(const ContactView& c, float* __restrict__ tc, PF pfun)
 {
     uint32_t n = min(*c.count, c.capacity);
-    for (uint32_t k = blockIdx.x * kB + threadIdx.x; k < n; k += gridDim.x * kB)
-    {
-        uint4    id     = c.idx[k];
-        float4   nk     = c.nk[k], w4 = c.w[k];
-        f3       nn     = mk(nk.x, nk.y, nk.z);
-        float    t      = w4.x * dot(nn, pfun(id.x & 0x7fffffffu));
-        t               = t + w4.y * dot(nn, pfun(id.y & 0x7fffffffu));
-        t               = t + w4.z * dot(nn, pfun(id.z & 0x7fffffffu));
-        tc[k]           = t + w4.w * dot(nn, pfun(id.w & 0x7fffffffu));
-    }
+    for (uint32_t k = blockIdx.x * kB + threadIdx.x; k < n; k += gridDim.x * kB) tc[k] = contact_t1(c, k, pfun);
 }
 
 // One 64-slot chunk per wavefront, starting at `base`: slot contributions a t_c are summed over
@@ -709,16 +963,10 @@ __global__ __launch_bounds__(kB) void k_contact_t_pcg(PcgArgs a, ContactView c, 
     const uint32_t e    = a.seg.cb_slot0[ch] + threadIdx.x;
     if (e >= vc.ptr[a.seg.sc_v0[scene + 1]]) return;
     const uint32_t ent = vc.list[e];
-    if ((ent & 3u) != 0u) return;
-    const uint32_t k  = ent >> 2;
+    if ((ent & 7u) != 0u) return;
+    const uint32_t k  = ent >> 3;
     auto           pf = [&](uint32_t v) { return ld3(a.z, v) + beta * ld3(a.p_old, v); };
-    uint4          id = c.idx[k];
-    float4         nk = c.nk[k], w4 = c.w[k];
-    f3             nn = mk(nk.x, nk.y, nk.z);
-    float          t  = w4.x * dot(nn, pf(id.x & 0x7fffffffu));
-    t                 = t + w4.y * dot(nn, pf(id.y & 0x7fffffffu));
-    t                 = t + w4.z * dot(nn, pf(id.z & 0x7fffffffu));
-    vc.tc[k]          = t + w4.w * dot(nn, pf(id.w & 0x7fffffffu));
+    vc.tc[k]          = contact_t1(c, k, pf);
 }
 
 __global__ __launch_bounds__(kB) void k_contact_t_plain(ContactView c, const float* __restrict__ p, float* __restrict__ tc)
@@ -968,7 +1216,7 @@ __global__ __launch_bounds__(kB) void k_slots_plain(VertContacts vc, float* __re
 {
     uint32_t row;
     f3       sum;
-    if (slot_chunk(vc, blockIdx.x * kB + (threadIdx.x & ~63u), vc.nslots, row, sum))
+    if (slot_chunk(vc, blockIdx.x * kB + (threadIdx.x & ~63u), *vc.total, row, sum))
     {
         atomicAdd(&y[3 * (size_t)row], sum.x);
         atomicAdd(&y[3 * (size_t)row + 1], sum.y);
@@ -1033,7 +1281,7 @@ __global__ __launch_bounds__(kB) void k_ground_ccd_maxdx(LsArgs a)
         f3 d = ld3(a.dx, i);
         mdx  = length(d);
         if (!(mdx == mdx)) atomicOr(&a.st->nan, 1u);
-        if (a.use_floor && !a.fixed[i])
+        if (a.use_floor && !a.fixed[i] && a.row_kind[i] < 2u)
         {
             float off = a.offset[i], cy = a.x[3 * (size_t)i + 1];
             if (cy - off < a.floor_y)
@@ -1084,6 +1332,7 @@ __global__ __launch_bounds__(kB) void k_ccd(CcdArgs a)
             v[2] = a.faces[3 * o + 1];
             v[3] = a.faces[3 * o + 2];
             if (v[0] == v[1] || v[0] == v[2] || v[0] == v[3]) continue;
+            if (a.vert_body[v[0]] != kInvalid && a.vert_body[v[0]] == a.vert_body[v[1]]) continue;  // narrow_phase.cpp:324-325
         }
         else
         {
@@ -1092,6 +1341,7 @@ __global__ __launch_bounds__(kB) void k_ccd(CcdArgs a)
             v[2] = a.edges[2 * o];
             v[3] = a.edges[2 * o + 1];
             if (v[0] == v[2] || v[0] == v[3] || v[1] == v[2] || v[1] == v[3]) continue;
+            if (a.vert_body[v[0]] != kInvalid && a.vert_body[v[0]] == a.vert_body[v[2]]) continue;  // narrow_phase.cpp:471-472
         }
         f3 x0[4], x1[4];
         for (int k = 0; k < 4; k++)
@@ -1221,10 +1471,10 @@ void launch_pin_velocity(uint32_t nfix, const uint32_t* ids, const float* target
 {
     if (nfix) k_pin_velocity<<<grid_for(nfix, kB), kB, 0, s>>>(nfix, ids, targets, x, v, dt);
 }
-void launch_predict(uint32_t nv, const float* x_step, const float* v, const uint8_t* fixed, float h, const float g[3],
-                    float* x_iter, float* x_tilde, float* x, hipStream_t s)
+void launch_predict(uint32_t nv, const float* x_step, const float* v, const uint8_t* fixed, const uint8_t* row_kind,
+                    float h, const float g[3], float* x_iter, float* x_tilde, float* x, hipStream_t s)
 {
-    k_predict<<<grid_for(nv, kB), kB, 0, s>>>(nv, x_step, v, fixed, h, mk(g[0], g[1], g[2]), x_iter, x_tilde, x);
+    k_predict<<<grid_for(nv, kB), kB, 0, s>>>(nv, x_step, v, fixed, row_kind, h, mk(g[0], g[1], g[2]), x_iter, x_tilde, x);
 }
 void launch_frame_end(uint32_t nv, float* x_step, const float* x, float* v, float h, float damp, hipStream_t s)
 {
@@ -1245,6 +1495,20 @@ void launch_offdiag(uint32_t nblocks, const uint32_t* ptr, const uint32_t* con, 
                     hipStream_t s)
 {
     if (nblocks) k_offdiag<<<grid_for(nblocks, kB), kB, 0, s>>>(nblocks, ptr, con, fs, hs, val);
+}
+void launch_abd_material(const AbdArgs& a, hipStream_t s)
+{
+    if (a.nb) k_abd_material<<<grid_for(a.nb, kB), kB, 0, s>>>(a);
+}
+void launch_rigid_positions(uint32_t n_rows, const uint8_t* row_kind, const uint32_t* vert_body, const BodyDev* bodies,
+                            const float* body_X, float* x, hipStream_t s)
+{
+    k_rigid_positions<<<grid_for(n_rows, kB), kB, 0, s>>>(n_rows, row_kind, vert_body, bodies, body_X, x);
+}
+void launch_contact_lift(const ContactView& c, unsigned grid, const uint32_t* vert_body, const BodyDev* bodies,
+                         const float* body_X, uint4* gr, float4* gw, hipStream_t s)
+{
+    k_contact_lift<<<grid, kB, 0, s>>>(c, vert_body, bodies, body_X, gr, gw);
 }
 void launch_dcd(const DcdArgs& a, int kind, unsigned grid, hipStream_t s)
 {

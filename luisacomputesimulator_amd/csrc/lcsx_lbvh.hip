@@ -632,6 +632,7 @@ __global__ __launch_bounds__(kQueryBlock) void k_query(QueryArgs a, const float4
         if (a.vert_scene && a.mode != QueryBoxes) g = a.vert_scene[a.mode == QueryVerts ? q : a.edges[2 * q]];
         root = roots[g];
         if (root == kInvalid || (a.active && a.active[g] == 0)) active = false;
+        if (a.mode == QueryVerts && a.row_kind && a.row_kind[q] >= 2u) active = false;
     }
     int sp = 0;
     if (active) stack[sp++][tid] = root;

@@ -147,10 +147,13 @@ void animate_target(const lcsx_fixed_anim& a, float t, const float in[3], float 
 void Setup::build(const std::vector<MeshDesc>& meshes)
 {
     *this = Setup();
+    std::vector<uint32_t> mesh_v0(meshes.size(), 0);
     for (size_t mi = 0; mi < meshes.size(); mi++)
     {
         const MeshDesc&       m  = meshes[mi];
         uint32_t              bv = nv;
+        mesh_v0[mi]              = bv;
+        const bool            rigid = m.mat.body != 0;
         std::vector<uint32_t> E, H;
         topology(m.tris, E, H);
         uint32_t mv = (uint32_t)(m.x.size() / 3);
@@ -165,12 +168,42 @@ void Setup::build(const std::vector<MeshDesc>& meshes)
         for (size_t k = 0; k < m.pinned.size(); k++)
         {
             is_fixed[bv + m.pinned[k]] = 1;
+            if (rigid) continue;  // a pinned rigid vertex fixes its body (no target)
             fixed_verts.push_back(bv + m.pinned[k]);
             fixed_mesh.push_back((uint32_t)mi);
             fixed_local.push_back((uint32_t)k);
             fixed_anim.push_back(m.anim[k]);
         }
+        row_kind.insert(row_kind.end(), mv, rigid ? 1 : 0);
+        vert_body.insert(vert_body.end(), mv, rigid ? (uint32_t)bodies.size() : ~0u);
+        for (uint32_t i = 0; i < mv; i++)
+            for (int d = 0; d < 3; d++) body_X.push_back(rigid ? m.x[3 * i + d] : 0.0f);
+        if (rigid)
+        {
+            Body b{};
+            b.v0 = bv;
+            b.v1 = bv + mv;
+            b.fixed = m.pinned.empty() ? 0u : 1u;
+            bodies.push_back(b);
+        }
         nv += mv;
+        // the scene's body rows follow its last mesh
+        const bool scene_end = mi + 1 == meshes.size() || meshes[mi + 1].scene != m.scene;
+        if (scene_end)
+            for (size_t b = 0; b < bodies.size(); b++)
+            {
+                if (bodies[b].row0 != 0 || vert_mesh[bodies[b].v0] > mi) continue;
+                if (meshes[vert_mesh[bodies[b].v0]].scene != m.scene) continue;
+                bodies[b].row0 = nv;
+                static const float q0[12] = {0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0, 1};
+                rest_x.insert(rest_x.end(), q0, q0 + 12);
+                vert_mesh.insert(vert_mesh.end(), 4, vert_mesh[bodies[b].v0]);
+                is_fixed.insert(is_fixed.end(), 4, (uint8_t)bodies[b].fixed);
+                for (int k = 0; k < 4; k++) row_kind.push_back((uint8_t)(2 + k));
+                vert_body.insert(vert_body.end(), 4, (uint32_t)b);
+                body_X.insert(body_X.end(), 12, 0.0f);
+                nv += 4;
+            }
     }
     nf = (uint32_t)(faces.size() / 3);
     ne = (uint32_t)(edges.size() / 2);
@@ -181,10 +214,12 @@ void Setup::build(const std::vector<MeshDesc>& meshes)
     sc_v0.assign(nscenes + 1, 0);
     sc_f0.assign(nscenes + 1, 0);
     sc_e0.assign(nscenes + 1, 0);
+    sc_verts.assign(nscenes, 0);
     for (size_t mi = 0; mi < meshes.size(); mi++)
     {
         uint32_t sc = meshes[mi].scene;
-        sc_v0[sc + 1] += (uint32_t)(meshes[mi].x.size() / 3);
+        sc_v0[sc + 1] += (uint32_t)(meshes[mi].x.size() / 3) + (meshes[mi].mat.body != 0 ? 4u : 0u);
+        sc_verts[sc] += (uint32_t)(meshes[mi].x.size() / 3);
     }
     for (uint32_t f = 0; f < nf; f++) sc_f0[meshes[face_mesh[f]].scene + 1]++;
     for (uint32_t e = 0; e < ne; e++) sc_e0[meshes[edge_mesh[e]].scene + 1]++;
@@ -273,7 +308,8 @@ void Setup::build(const std::vector<MeshDesc>& meshes)
     for (uint32_t i = 0; i < nv; i++)
     {
         const lcsx_material& mt = meshes[vert_mesh[i]].mat;
-        float                so = 0.5f * mt.thickness, sd = mt.d_hat, safe = 0.9f * min_dist[vert_mesh[i]];
+        // a rigid solid has no contact thickness (init_mesh_data.cpp:405-414)
+        float so = mt.body == 2 ? 0.0f : 0.5f * mt.thickness, sd = mt.d_hat, safe = 0.9f * min_dist[vert_mesh[i]];
         if (!(so + mt.d_hat < safe))
         {
             if (so > 0.5f * safe) so = 0.5f * safe;
@@ -287,6 +323,7 @@ void Setup::build(const std::vector<MeshDesc>& meshes)
     for (uint32_t f = 0; f < nf; f++)
     {
         const uint32_t* t = &faces[3 * f];
+        if (vert_body[t[0]] != ~0u) continue;  // rigid meshes have no stretch terms
         if (is_fixed[t[0]] && is_fixed[t[1]] && is_fixed[t[2]]) continue;
         sf.insert(sf.end(), t, t + 3);
         float d[4];
@@ -302,6 +339,7 @@ void Setup::build(const std::vector<MeshDesc>& meshes)
     for (uint32_t h = 0; h < nh; h++)
     {
         const uint32_t* q = &hinges[4 * h];
+        if (vert_body[q[0]] != ~0u) continue;
         if (is_fixed[q[0]] && is_fixed[q[1]] && is_fixed[q[2]] && is_fixed[q[3]]) continue;
         bh.insert(bh.end(), q, q + 4);
         f3 p0 = at(rest_x, q[0]), p1 = at(rest_x, q[1]), p2 = at(rest_x, q[2]), p3 = at(rest_x, q[3]);
@@ -326,6 +364,10 @@ void Setup::build(const std::vector<MeshDesc>& meshes)
         for (int i = 0; i < 4; i++)
             for (int j = 0; j < 4; j++)
                 if (i != j) add(bh[4 * e + i], bh[4 * e + j]);
+    for (const Body& b : bodies)
+        for (uint32_t i = 0; i < 4; i++)
+            for (uint32_t j = 0; j < 4; j++)
+                if (i != j) add(b.row0 + i, b.row0 + j);
     for (auto& l : adj)
     {
         std::sort(l.begin(), l.end());
@@ -418,6 +460,126 @@ void Setup::build(const std::vector<MeshDesc>& meshes)
     blk_contrib.assign(blk_ptr[nblocks], 0);
     std::vector<uint32_t> fill(blk_ptr.begin(), blk_ptr.end() - 1);
     visit([&](uint32_t blk, uint32_t code) { blk_contrib[fill[blk]++] = code; });
+
+    // affine bodies: mass matrix, orthogonality stiffness, SELL blocks of the body rows
+    for (Body& b : bodies)
+    {
+        const uint32_t       mi = vert_mesh[b.v0];
+        const lcsx_material& mt = meshes[mi].mat;
+        float                m = 0.0f, mx[3] = {0, 0, 0}, mxx[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, vol = 0.0f;
+        if (mt.body == 1)
+            rigid_shell_mass(b, mi, vvol, body_vol, body_mass, m, mx, mxx, vol);
+        else
+            rigid_solid_mass(b, mi, face_mesh, m, mx, mxx, vol, mt.density);
+        b.M[0] = m;
+        for (int i = 0; i < 3; i++)
+        {
+            b.M[4 * (1 + i)] = mx[i];
+            b.M[1 + i]       = mx[i];
+            for (int j = 0; j < 3; j++) b.M[4 * (1 + i) + 1 + j] = mxx[i][j];
+        }
+        for (int i = 0; i < 4; i++) b.M[5 * i] = std::max(b.M[5 * i], 1.19209290E-7f);  // init_sim_data.cpp:694
+        b.stiff = mt.rigid_stiffness * vol;
+        int k   = 0;
+        for (uint32_t ii = 0; ii < 4; ii++)
+            for (uint32_t jj = 0; jj < 4; jj++)
+                if (ii != jj) b.blk[k++] = slot_of(b.row0 + ii, b.row0 + jj);
+    }
+}
+
+// Shell body (RigidMaterial is_shell, init_sim_data.cpp:600-640): mass carried by the vertices
+// (the cloth vertex-mass rule), volume = area x thickness.
+void Setup::rigid_shell_mass(const Body& b, uint32_t mi, const std::vector<float>& vvol, const std::vector<float>& body_vol,
+                             const std::vector<float>& body_mass, float& m, float mx[3], float mxx[3][3], float& vol) const
+{
+    for (uint32_t i = b.v0; i < b.v1; i++)
+    {
+        float vm   = (vvol[i] / body_vol[mi]) * body_mass[mi];
+        float X[3] = {rest_x[3 * i], rest_x[3 * i + 1], rest_x[3 * i + 2]};
+        m += vm;
+        for (int a = 0; a < 3; a++)
+        {
+            mx[a] += vm * X[a];
+            for (int c = 0; c < 3; c++) mxx[a][c] = mxx[a][c] + vm * (X[a] * X[c]);
+        }
+    }
+    vol = body_vol[mi];
+}
+
+// Solid body (compute_trimesh_dyadic_mass, init_sim_data.cpp:51-188): mass and moments of the
+// enclosed solid by surface integrals over its triangles, volume likewise.
+void Setup::rigid_solid_mass(const Body& b, uint32_t mi, const std::vector<uint32_t>& face_mesh_, float& m, float mx[3],
+                             float mxx[3][3], float& vol, float rho) const
+{
+    (void)b;
+    auto P = [&](uint32_t v) { return mk(rest_x[3 * v], rest_x[3 * v + 1], rest_x[3 * v + 2]); };
+    for (uint32_t f = 0; f < nf; f++)
+    {
+        if (face_mesh_[f] != mi) continue;
+        const f3 p0 = P(faces[3 * f]), p1 = P(faces[3 * f + 1]), p2 = P(faces[3 * f + 2]);
+        const f3 N  = cross(p1 - p0, p2 - p0);
+        m += rho * dot(p0, N) / 6.0;
+        vol += dot(p0, N) / 6.0f;
+        const float a0[3] = {p0.x, p0.y, p0.z}, a1[3] = {p1.x, p1.y, p1.z}, a2[3] = {p2.x, p2.y, p2.z};
+        const float n[3]  = {N.x, N.y, N.z};
+        for (int a = 0; a < 3; a++)
+        {
+            float V = 0.0;
+            V += a0[a] * a0[a] / 12;
+            V += a0[a] * a1[a] / 12;
+            V += a0[a] * a2[a] / 12;
+            V += a1[a] * a1[a] / 12;
+            V += a1[a] * a2[a] / 12;
+            V += a2[a] * a2[a] / 12;
+            mx[a] += rho / 2 * n[a] * V;
+        }
+        for (int a = 0; a < 3; a++)
+        {
+            float V = 0.0;
+            float s0 = a0[a] * a0[a], s1 = a1[a] * a1[a], s2 = a2[a] * a2[a];
+            float c0 = s0 * a0[a], c1 = s1 * a1[a], c2 = s2 * a2[a];
+            V += c0 / 20;
+            V += s0 * a1[a] / 20;
+            V += s0 * a2[a] / 20;
+            V += a0[a] * s1 / 20;
+            V += a0[a] * a1[a] * a2[a] / 20;
+            V += a0[a] * s2 / 20;
+            V += c1 / 20;
+            V += s1 * a2[a] / 20;
+            V += a1[a] * s2 / 20;
+            V += c2 / 20;
+            mxx[a][a] += rho / 3 * n[a] * V;
+        }
+        auto Q = [&](int a, int c) {
+            float V = 0.0;
+            float s0 = a0[a] * a0[a], s1 = a1[a] * a1[a], s2 = a2[a] * a2[a];
+            V += s0 * a0[c] / 20;
+            V += s0 * a1[c] / 60;
+            V += s0 * a2[c] / 60;
+            V += a0[a] * a0[c] * a1[a] / 30;
+            V += a0[a] * a0[c] * a2[a] / 30;
+            V += a0[a] * a1[a] * a1[c] / 30;
+            V += a0[a] * a1[a] * a2[c] / 60;
+            V += a0[a] * a1[c] * a2[a] / 60;
+            V += a0[a] * a2[a] * a2[c] / 30;
+            V += a0[c] * s1 / 60;
+            V += a0[c] * a1[a] * a2[a] / 60;
+            V += a0[c] * s2 / 60;
+            V += s1 * a1[c] / 20;
+            V += s1 * a2[c] / 60;
+            V += a1[a] * a1[c] * a2[a] / 30;
+            V += a1[a] * a2[a] * a2[c] / 30;
+            V += a1[c] * s2 / 60;
+            V += s2 * a2[c] / 20;
+            return rho / 2 * n[a] * V;
+        };
+        mxx[0][1] += Q(0, 1);
+        mxx[0][2] += Q(0, 2);
+        mxx[1][2] += Q(1, 2);
+        mxx[1][0] = mxx[0][1];
+        mxx[2][0] = mxx[0][2];
+        mxx[2][1] = mxx[1][2];
+    }
 }
 
 }  // namespace lcsx

@@ -57,9 +57,30 @@ struct Setup
     // rb_scene[b], and scene s owns blocks [sc_rb0[s], sc_rb0[s+1]).
     uint32_t              nscenes = 1;
     std::vector<uint32_t> sc_v0, sc_f0, sc_e0, sc_rb0, vert_scene, rb_scene, rb_row0;
+    std::vector<uint32_t> sc_verts;  // vertex rows per scene (body rows excluded)
     uint32_t              n_rb = 0;
+    // affine bodies: each rigid mesh is a body with four rows (p, a0, a1, a2) placed after the
+    // vertex rows of its scene; its vertices keep rows of their own and follow x = p + A X.
+    // row_kind: 0 soft vertex, 1 rigid vertex, 2 + k body row k.
+    struct Body
+    {
+        uint32_t row0, v0, v1, fixed;
+        float    stiff;    // kappa * volume
+        float    M[16];    // 4x4 mass matrix (times I3)
+        uint32_t blk[12];  // SELL blocks (ii, jj != ii), row-major
+    };
+    std::vector<Body>     bodies;
+    std::vector<uint8_t>  row_kind;
+    std::vector<uint32_t> vert_body;  // per row: body id (rigid vertex or body row) or ~0u
+    std::vector<float>    body_X;     // per row: body-frame position of a rigid vertex (3)
 
     void build(const std::vector<MeshDesc>& meshes);
+
+private:
+    void rigid_shell_mass(const Body& b, uint32_t mi, const std::vector<float>& vvol, const std::vector<float>& body_vol,
+                          const std::vector<float>& body_mass, float& m, float mx[3], float mxx[3][3], float& vol) const;
+    void rigid_solid_mass(const Body& b, uint32_t mi, const std::vector<uint32_t>& face_mesh_, float& m, float mx[3],
+                          float mxx[3][3], float& vol, float rho) const;
 };
 
 // Pinned-target animation (reference init_mesh_data.h:56-93, luisa::rotation axis-angle).

@@ -127,14 +127,16 @@ void Solver::alloc_collision(uint32_t vf, uint32_t ee, uint32_t nc)
     ccd_rest_vf.alloc(cap_vf);
     ccd_pairs_fresh = false;
     ccd_rest_ee.alloc(cap_ee);
-    vc_list.alloc(4 * (size_t)cap_c);
+    vc_list.alloc(slots_per_contact * (size_t)cap_c);
+    c_gr.alloc(2 * (size_t)cap_c);
+    c_gw.alloc(2 * (size_t)cap_c);
     // per-block partial sums of the SpMV: row blocks + scene-aligned slot chunks (4 slots per contact)
-    const size_t chunks = max_chunks(4 * cap_c) + 1;
+    const size_t chunks = max_chunks(slots_per_contact * cap_c) + 1;
     part_pq.alloc(S.n_rb + chunks);
     cb_scene_d.alloc(chunks);
     cb_slot0_d.alloc(chunks);
-    vc_row.alloc(4 * (size_t)cap_c);
-    vc_coef.alloc(4 * (size_t)cap_c);
+    vc_row.alloc(slots_per_contact * (size_t)cap_c);
+    vc_coef.alloc(slots_per_contact * (size_t)cap_c);
     vc_tc.alloc(cap_c);
     vc_ready = false;
 }
@@ -177,13 +179,14 @@ void Solver::init()
     host_pcg.assign(ns, PcgState{});
     scene_stats.assign(ns, lcsx_stats{});
     verts_per_scene.assign(ns, 0);
-    for (uint32_t sc = 0; sc < ns; sc++) verts_per_scene[sc] = S.sc_v0[sc + 1] - S.sc_v0[sc];
+    for (uint32_t sc = 0; sc < ns; sc++) verts_per_scene[sc] = S.sc_verts[sc];
     if (prm.precision == 1)
     {
         sell_h.alloc(9 * (size_t)std::max(S.nblocks, 1u));
         sell_e.alloc(std::max(S.nblocks, 1u));
         part_res.alloc(2 * (size_t)S.n_rb);
     }
+    slots_per_contact = S.bodies.empty() ? 4u : 8u;
     upload_state_constants();
     tree_face.init(3, faces.p, S.nf, S.sc_f0.data(), ns);
     tree_edge.init(2, edges.p, S.ne, S.sc_e0.data(), ns);
@@ -233,6 +236,15 @@ void Solver::upload_state_constants()
     blk_ptr.upload(S.blk_ptr, s);
     blk_contrib.upload(S.blk_contrib, s);
     fixed_ids.upload(S.fixed_verts, s);
+    row_kind_d.upload(S.row_kind, s);
+    vert_body_d.upload(S.vert_body, s);
+    body_X_d.upload(S.body_X, s);
+    static_assert(sizeof(BodyDev) == sizeof(Setup::Body), "body record layout");
+    if (!S.bodies.empty())
+    {
+        bodies_d.alloc(S.bodies.size());
+        LCSX_HIP(hipMemcpyAsync(bodies_d.p, S.bodies.data(), S.bodies.size() * sizeof(BodyDev), hipMemcpyHostToDevice, s));
+    }
     targets.alloc(3 * std::max<size_t>(S.fixed_verts.size(), 1));
     const size_t       nsf = S.sf.size() / 3, nbh = S.bh.size() / 4;
     std::vector<float4> dm(nsf), cc(nsf), hc(nbh);
@@ -252,19 +264,19 @@ void Solver::upload_state_constants()
 
 ContactView Solver::contact_view() const
 {
-    return ContactView{c_idx.p, c_nk.p, c_w.p, c_aux.p, &status.p->n_contacts, cap_c};
+    return ContactView{c_idx.p, c_nk.p, c_w.p, c_aux.p, &status.p->n_contacts, cap_c, c_gr.p, c_gw.p};
 }
 VertContacts Solver::vert_contacts()
 {
     // no lists before the first contact assembly, or without self collision; the slot count
     // sizes the SpMV's grid, so it is read back once (one small synchronisation)
-    if (!(prm.use_self_collision && vc_ready)) return VertContacts{nullptr, nullptr, nullptr, nullptr, nullptr, 0u};
+    if (!(prm.use_self_collision && vc_ready)) return VertContacts{nullptr, nullptr, nullptr, nullptr, nullptr, 0u, nullptr};
     uint32_t nc = 0;
     LCSX_HIP(hipMemcpyAsync(&nc, &status.p->n_contacts, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
     LCSX_HIP(hipStreamSynchronize(stream));
     nc = std::min(nc, cap_c);
-    if (nc == 0) return VertContacts{nullptr, nullptr, nullptr, nullptr, nullptr, 0u};
-    return VertContacts{vc_ptr.p, vc_list.p, vc_row.p, vc_coef.p, vc_tc.p, 4u * nc};
+    if (nc == 0) return VertContacts{nullptr, nullptr, nullptr, nullptr, nullptr, 0u, nullptr};
+    return VertContacts{vc_ptr.p, vc_list.p, vc_row.p, vc_coef.p, vc_tc.p, slots_per_contact * nc, vc_ptr.p + S.nv};
 }
 // The SpMV's diagonal is the material one: contact diagonal blocks enter through the contact slots.
 SegView Solver::seg_view(bool with_chunks) const
@@ -320,7 +332,8 @@ void Solver::frame_begin()
 
 void Solver::predict()
 {
-    launch_predict(S.nv, x_step.p, v.p, is_fixed.p, prm.implicit_dt, prm.gravity, x_iter.p, x_tilde.p, x.p, stream);
+    launch_predict(S.nv, x_step.p, v.p, is_fixed.p, row_kind_d.p, prm.implicit_dt, prm.gravity, x_iter.p, x_tilde.p, x.p,
+                   stream);
 }
 
 void Solver::iter_begin()
@@ -337,6 +350,7 @@ void Solver::material()
     launch_hinge_stage(nbh, bh.p, bh_c.p, prm.bending_scaling, x.p, hinge_stage.p, stream);
     VertArgs a{};
     a.nv       = S.nv;
+    a.row_kind = row_kind_d.p;
     a.x        = x.p;
     a.x_tilde  = x_tilde.p;
     a.mass     = vert_mass.p;
@@ -361,6 +375,31 @@ void Solver::material()
     a.diag_mat = diag_mat.p;
     launch_vert_assemble(a, stream);
     launch_offdiag(S.nblocks, blk_ptr.p, blk_contrib.p, face_stage.p, hinge_stage.p, sell_val.p, stream);
+    if (!S.bodies.empty())
+    {
+        AbdArgs b{};
+        b.nb        = (uint32_t)S.bodies.size();
+        b.n_rows    = S.nv;
+        b.nblocks   = S.nblocks;
+        b.bodies    = bodies_d.p;
+        b.x         = x.p;
+        b.x_tilde   = x_tilde.p;
+        b.body_X    = body_X_d.p;
+        b.fixed     = is_fixed.p;
+        b.offset    = offset.p;
+        b.dhat      = dhat.p;
+        b.area      = vert_area.p;
+        b.h2inv     = a.h2inv;
+        b.kD        = a.kD;
+        b.kappa     = a.kappa;
+        b.floor_y   = a.floor_y;
+        b.use_floor = a.use_floor;
+        b.cgB       = cgB.p;
+        b.diag      = diag.p;
+        b.diag_mat  = diag_mat.p;
+        b.sell_val  = sell_val.p;
+        launch_abd_material(b, stream);
+    }
     if (prm.precision == 1) launch_pack_fp16(S.nblocks, sell_val.p, sell_h.p, sell_e.p, stream);
     toc("material");
 }
@@ -376,6 +415,7 @@ void Solver::broad_phase(bool ccd)
         QueryArgs q{};
         q.vert_scene = vert_scene_d.p;
         q.active     = active_d.p;
+        q.row_kind   = row_kind_d.p;
         q.mode      = QueryVerts;
         q.n_queries = S.nv;
         q.x0        = x0;
@@ -394,6 +434,7 @@ void Solver::broad_phase(bool ccd)
         QueryArgs q{};
         q.vert_scene = vert_scene_d.p;
         q.active     = active_d.p;
+        q.row_kind   = row_kind_d.p;
         q.mode      = QueryEdges;
         q.n_queries = S.ne;
         q.x0        = x0;
@@ -425,6 +466,7 @@ void Solver::dcd(bool from_ccd_pairs)
     a.kappa  = prm.stiffness_collision;
     a.fp64   = prm.precision == 1;
     a.out    = out;
+    a.vert_body = vert_body_d.p;
     a.pairs = pairs_vf.p;
     a.pair_count = from_ccd_pairs ? &status.p->n_vf_ccd : &status.p->n_vf;
     a.pair_capacity = cap_vf;
@@ -437,6 +479,8 @@ void Solver::dcd(bool from_ccd_pairs)
     a.area_a = edge_area.p;
     a.area_b = edge_area.p;
     launch_dcd(a, 1, pair_grid(cap_ee), stream);
+    launch_contact_lift(contact_view(), pair_grid(cap_c), S.bodies.empty() ? nullptr : vert_body_d.p, bodies_d.p,
+                        body_X_d.p, c_gr.p, c_gw.p, stream);
     toc("dcd");
 }
 
@@ -460,7 +504,7 @@ void Solver::contact_assemble()
     launch_contact_adjacency(contact_view(), S.nv, pair_grid(cap_c), vc_cnt.p, vc_ptr.p, vc_list.p, vc_row.p, vc_coef.p,
                              seg_view(true), sc_cb0_d.p, cb_scene_d.p, cb_slot0_d.p, stream);
     vc_ready = true;
-    VertContacts vc{vc_ptr.p, vc_list.p, vc_row.p, vc_coef.p, vc_tc.p, 0u};  // assembly walks the per-vertex lists
+    VertContacts vc{vc_ptr.p, vc_list.p, vc_row.p, vc_coef.p, vc_tc.p, 0u, vc_ptr.p + S.nv};  // assembly walks the per-row lists
     launch_contact_assemble(contact_view(), vc, cgB.p, diag.p, S.nv, stream);
     toc("contact_assemble");
 }
@@ -641,6 +685,7 @@ void Solver::ccd()
     l.st        = status.p;
     l.sst       = sst.p;
     l.seg       = seg_view(false);
+    l.row_kind  = row_kind_d.p;
     launch_ground_ccd_maxdx(l, stream);
     if (prm.use_ccd_linesearch && prm.use_self_collision)
     {
@@ -652,6 +697,7 @@ void Solver::ccd()
         c.offset   = offset.p;
         c.sst      = sst.p;
         c.vert_scene = vert_scene_d.p;
+        c.vert_body  = vert_body_d.p;
         c.fp64     = prm.precision == 1;
         c.budget   = kCcdFirstPassIters;
         launch_fill_u32(ccd_rest_count.p, 2, 0u, stream);
@@ -693,10 +739,12 @@ void Solver::line_search()
     const SegView g = seg_view(false);
     launch_reset_line_search(status.p, sst.p, S.nscenes, stream);
     launch_apply_dx(S.nv, x_iter.p, dx.p, x.p, nullptr, 1.0f, g, stream);
+    rigid_positions();
     if (prm.use_ccd_linesearch && prm.use_self_collision) broad_phase(true);
     ccd_pairs_fresh = prm.use_ccd_linesearch && prm.use_self_collision;
     ccd();
     launch_apply_dx(S.nv, x_iter.p, dx.p, x.p, sst.p, 1.0f, g, stream);
+    rigid_positions();
     launch_pin_delta((uint32_t)S.fixed_verts.size(), fixed_ids.p, x.p, x_tilde.p, sst.p, vert_scene_d.p, stream);
 }
 
@@ -843,6 +891,12 @@ void Solver::step()
     frame_end();
     for (auto& st : scene_stats) st.frame = frame;
     LCSX_HIP(hipStreamSynchronize(stream));
+}
+
+void Solver::rigid_positions()
+{
+    if (!S.bodies.empty())
+        launch_rigid_positions(S.nv, row_kind_d.p, vert_body_d.p, bodies_d.p, body_X_d.p, x.p, stream);
 }
 
 void Solver::spmv_dx()

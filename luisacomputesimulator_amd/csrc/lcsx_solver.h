@@ -46,7 +46,15 @@ struct Solver
     uint32_t       cap_vf = 0, cap_ee = 0, cap_c = 0;
     DBuf<uint4>    c_idx;
     DBuf<float4>   c_nk, c_w, c_aux;
-    DBuf<uint32_t> vc_cnt, vc_ptr, vc_list, vc_row;  // per-vertex contact slots (rebuilt with each contact assembly)
+    DBuf<uint32_t> vc_cnt, vc_ptr, vc_list, vc_row;  // per-row contact slots (rebuilt with each contact assembly)
+    DBuf<uint4>    c_gr;                              // generalized DOFs of the contacts (2 per contact)
+    DBuf<float4>   c_gw;
+    // affine bodies
+    DBuf<BodyDev>  bodies_d;
+    DBuf<uint8_t>  row_kind_d;
+    DBuf<uint32_t> vert_body_d;
+    DBuf<float>    body_X_d;
+    uint32_t       slots_per_contact = 4;  // 8 with affine bodies (a rigid side has four rows)
     DBuf<float4>   vc_coef;                          // per slot
     DBuf<float>    vc_tc;                    // per contact
     DBuf<float>    vc_qc;                    // contact part of the SpMV (3 per vertex)
@@ -99,6 +107,7 @@ struct Solver
     void frame_end();
     void step();
     void spmv_dx();
+    void rigid_positions();  // x of the rigid vertices from their bodies' rows
     void read_status();
     bool grow_if_needed(bool ccd_only);
 

@@ -341,6 +341,35 @@ def sphere_batch(count: int = 256, seed: int = 0, n_min: int = 17, n_max: int = 
     return [cloth_on_sphere(int(ns[k]), float(off[k, 0]), float(off[k, 1]), seed=seed + k) for k in range(count)]
 
 
+def cloth_rigid_coupling(n: int = 129, boxes: int = 24, seed: int = 0, cloth_height: float = 0.2,
+                         box_side: float = 0.1, box_region=((-0.4, 0.4), (0.3, 0.8), (-0.4, 0.4)),
+                         box_stiffness: float = 1e8, box_density: float = 500.0) -> Scene:
+    """Cloth-rigid coupling: `boxes` affine-body cubes (solid, side 0.1 m, 8 vertices / 12
+    triangles, orthogonality stiffness 1e8, density 500 kg/m^3) at seeded positions uniform in
+    box_region with random orientations, dropped onto an n x n, 1.0 x 1.0 m cloth (the square
+    cloth's material) at `cloth_height`, pinned at its four corners; ground plane y = 0. Draws
+    whose bounding spheres overlap a previous box are redrawn (the boxes start apart)."""
+    rng = np.random.default_rng(seed)
+    cloth = square_cloth(n, seed=seed, height=cloth_height)
+    cm = cloth.meshes[0]
+    meshes = [Mesh("cloth", cm.verts, cm.tris, cm.material,
+                   [FixedPoints(m, 0.001) for m in ("LeftBack", "RightBack", "LeftFront", "RightFront")])]
+    mat = Material(thickness=cm.material.thickness, density=box_density, d_hat=cm.material.d_hat,
+                   body="rigid_solid", rigid_stiffness=box_stiffness)
+    centers = []
+    r = 0.5 * np.sqrt(3.0) * box_side + 2e-3
+    while len(centers) < boxes:
+        c = np.array([rng.uniform(*box_region[k]) for k in range(3)])
+        ang = rng.uniform(0.0, 2.0 * np.pi, size=3)
+        if c[1] - r <= cloth_height + 2e-3 or any(np.linalg.norm(c - o) < 2 * r for o in centers):
+            continue
+        centers.append(c)
+        v, t = box_mesh(c, box_side, rotation_matrix(ang))
+        meshes.append(Mesh(f"box{len(centers) - 1}", v, t, mat, []))
+    prm = Params(stiffness_collision=1e7, pcg_check_every=1, pcg_rel_tol=1e-4)
+    return Scene(f"cloth_rigid_{n}_{boxes}", meshes, prm, frames=60)
+
+
 def small_two_sheets(n: int = 9, gap: float = 2.5e-3) -> Scene:
     """Two small stacked sheets a few millimetres apart (self-contact exercise): the lower one is
     pinned at its four corners, the upper one falls onto it."""

@@ -1,0 +1,118 @@
+"""GPU parity of the affine-body (12-DOF rigid) terms against the CPU oracle.
+
+Rows of a scene with bodies: its vertices (rigid vertices included, as identity rows that follow
+their body), then four rows per body (p, a0, a1, a2). Assembly of the body blocks uses only + and
+* in the oracle's order, so cgB, diagonal and off-diagonal blocks are compared bit for bit;
+frames and iterations go through logf (barriers) and reductions and are held to tolerances.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from luisacomputesimulator_amd import NewtonSolver, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def small_coupling(gap=None, boxes=2, seed=3):
+    sc = scenes.cloth_rigid_coupling(n=17, boxes=boxes, seed=seed, box_region=((-0.25, 0.25), (0.32, 0.5), (-0.25, 0.25)))
+    if gap is not None:
+        # lower every box so its lowest vertex is `gap` above the cloth (contact from the start)
+        for m in sc.meshes[1:]:
+            m.verts[:, 1] -= np.float32(m.verts[:, 1].min() - (0.2 + gap))
+    return sc
+
+
+def make(sc, prec=32):
+    print(f"[gpu-abd] scene={sc.name} verts={sc.num_verts}", flush=True)
+    return NewtonSolver.from_scene(sc), oracle.Oracle(sc, prec)
+
+
+def test_body_rows_and_assembly_bit_exact(require_gpu):
+    sc = small_coupling()
+    g, o = make(sc)
+    nv = sc.num_verts
+    assert g.num_verts == nv + 4 * (len(sc.meshes) - 1)
+    for s in ("frame_begin", "predict", "iter_begin"):
+        g.stage(s)
+        o.stage(s)
+    # let the bodies move off the identity: one predicted step with a spin
+    x = o.get("x").reshape(-1, 3).copy()
+    rng = np.random.default_rng(0)
+    x[nv:] += (1e-3 * rng.standard_normal(x[nv:].shape)).astype(np.float32)
+    o.set("x", x)
+    g.write("x", x)
+    for s in ("material",):
+        g.stage(s)
+        o.stage(s)
+    for name in ("cgB", "diag", "offdiag"):
+        a, b = g.read(name), o.get(name).astype(np.float32)
+        print(f"[gpu-abd] {name}: max |gpu - oracle| {np.abs(a - b).max():.3e}", flush=True)
+        if name == "cgB":
+            # body rows bit for bit; the cloth's bending gradient goes through acosf (tolerance)
+            assert np.array_equal(a[3 * nv:], b[3 * nv:])
+            assert np.allclose(a, b, rtol=1e-5, atol=1e-6 * np.abs(b).max())
+        else:
+            assert np.array_equal(a, b), name
+
+
+def test_free_fall_frames_track_oracle(require_gpu):
+    """Bodies falling toward the cloth (no contact yet): positions within 1e-6 relative of the FP32
+    oracle and within three times its own error of the FP64 twin."""
+    sc = small_coupling()
+    g, o = make(sc)
+    o64 = oracle.Oracle(sc, 64)
+    for f in range(3):
+        g.physics_step_gpu(1)
+        o.step()
+        o64.step()
+        xg, xo, x64 = g.get_simulation_results(), o.get("x").reshape(-1, 3), o64.get("x").reshape(-1, 3)
+        e32 = np.abs(xg - xo).max() / np.abs(xo).max()
+        e64 = np.abs(xg - x64).max() / np.abs(x64).max()
+        eo = np.abs(xo - x64).max() / np.abs(x64).max()
+        print(f"[gpu-abd] frame {f}: vs FP32 oracle {e32:.3e}, vs FP64 {e64:.3e} (FP32 oracle {eo:.3e}) {g.stats()}",
+              flush=True)
+        assert e32 < 1e-6
+        assert e64 <= 3.0 * eo + 1e-6
+
+
+def test_newton_iteration_with_rigid_contacts(require_gpu):
+    """Boxes 1.5 mm above the cloth: contacts between rigid faces / edges and cloth elements, lifted
+    to the bodies' rows. One Newton iteration from an identical state: the active contact set
+    equals the oracle's, positions within 1e-5 relative."""
+    sc = small_coupling(gap=1.5e-3)
+    g, o = make(sc)
+    for s in ("frame_begin", "predict", "newton_iteration"):
+        g.stage(s)
+        o.stage(s)
+    ig, io = g.read("contact_idx").reshape(-1, 4), o.get("contact_idx").reshape(-1, 4)
+    kg = ig[np.lexsort(ig.T[::-1])]
+    ko = io[np.lexsort(io.T[::-1])]
+    rigid = (ko >= 17 * 17).any(axis=1)
+    print(f"[gpu-abd] contacts {len(ko)} ({int(rigid.sum())} with a rigid side), stats {g.stats()}", flush=True)
+    assert rigid.sum() > 0
+    assert np.array_equal(kg, ko)
+    xg, xo = g.read("x").reshape(-1, 3), o.get("x").reshape(-1, 3)
+    err = np.abs(xg - xo).max() / np.abs(xo).max()
+    print(f"[gpu-abd] contact iteration rel err {err:.3e}", flush=True)
+    assert err < 1e-5
+
+
+def test_box_lands_on_floor_without_penetration(require_gpu):
+    """A single box dropped onto the ground: it stops above the floor (offset 0, d_hat 1 mm) and
+    stays orthogonal (A^T A = I within 1e-3)."""
+    v, t = scenes.box_mesh(np.array([0.0, 0.2, 0.0]), 0.1, scenes.rotation_matrix([0.4, 0.2, 0.1]))
+    mat = scenes.Material(thickness=1e-3, density=500.0, d_hat=1e-3, body="rigid_solid", rigid_stiffness=1e8)
+    sc = scenes.Scene("box_floor", [scenes.Mesh("box", v, t, mat, [])],
+                      scenes.Params(stiffness_collision=1e7, use_self_collision=False), frames=40)
+    g = NewtonSolver.from_scene(sc)
+    for f in range(40):
+        g.physics_step_gpu(1)
+        x = g.get_simulation_results()
+        assert np.isfinite(x).all()
+        assert x[:8, 1].min() > 0.0, f
+    A = x[9:12]
+    print(f"[gpu-abd] box after 40 frames: lowest y {x[:8, 1].min():.3e}, A^T A - I {np.abs(A @ A.T - np.eye(3)).max():.3e}",
+          flush=True)
+    assert np.abs(A @ A.T - np.eye(3)).max() < 1e-3
+    assert x[:8, 1].min() < 2e-3  # resting on the ground band
