@@ -26,6 +26,10 @@ The JSON line carries:
                 scenes in one solver (one launch per stage) against the same scenes stepped one by
                 one, each from the batch's state after the warm-up frames (the cloths reach the
                 spheres at about frame 12). Scene-steps/s = scenes x frames / seconds.
+  coupling      cloth-rigid coupling (129 x 129 cloth, 24 affine-body boxes): Newton steps/s in FP32
+                and mixed precision over 60 frames, and both trajectories' relative position error
+                against the FP64 oracle over the first 24 frames (the first box-cloth contacts
+                come at about frame 20).
   cpu_baseline  the CPU oracle (C++ restatement of the reference path, 1 thread) timed on
                 frame 0 of the same workload on this host.
 """
@@ -138,6 +142,56 @@ def batched_leg(args, precision, sequential):
     return out
 
 
+def coupling_leg(args):
+    """Cloth-rigid coupling (129 x 129 cloth pinned at its corners, 24 affine-body boxes): Newton
+    steps/s of all frames in FP32 and in mixed precision, and the position error of both against
+    the FP64 oracle over the first frames (a bounded CPU sample)."""
+    import numpy as np
+
+    import oracle
+    from luisacomputesimulator_amd import NewtonSolver, hiprt, scenes
+
+    out = {"workload": "129x129 cloth pinned at 4 corners + 24 affine-body boxes (0.1 m, solid, density 500, "
+                       "stiffness 1e8), seed 0", "frames": args.coupling_frames}
+    traj = {}
+    for prec in ("fp32", "mixed"):
+        sc = scenes.cloth_rigid_coupling(129, 24, seed=0)
+        sc.params.precision = prec
+        g = NewtonSolver.from_scene(sc)
+        nv = sc.num_verts
+        hiprt.synchronize()
+        newton, xs = 0, []
+        t0 = time.perf_counter()
+        for f in range(args.coupling_frames):
+            g.physics_step_gpu(1)
+            newton += g.stats()["newton_iters"]
+            if f < args.coupling_oracle_frames:
+                xs.append(g.get_simulation_results()[:nv].copy())
+        hiprt.synchronize()
+        el = time.perf_counter() - t0
+        out[prec] = {"newton_steps_per_s": newton / el, "frames_per_s": args.coupling_frames / el,
+                     "newton_iters": newton, "contacts_last_iter": g.stats()["num_contacts"]}
+        traj[prec] = xs
+        print(f"[bench] coupling {prec}: {out[prec]}", file=sys.stderr, flush=True)
+    out["mixed_speedup_vs_fp32"] = out["mixed"]["newton_steps_per_s"] / out["fp32"]["newton_steps_per_s"]
+    if args.coupling_oracle_frames > 0:
+        sc = scenes.cloth_rigid_coupling(129, 24, seed=0)
+        o64 = oracle.Oracle(sc, 64)
+        err = {"fp32": [], "mixed": []}
+        t0 = time.perf_counter()
+        for f in range(args.coupling_oracle_frames):
+            o64.step()
+            x64 = o64.get("x").reshape(-1, 3)[:sc.num_verts]
+            scale = np.abs(x64).max()
+            for prec in err:
+                err[prec].append(float(np.abs(traj[prec][f] - x64).max() / scale))
+            print(f"[bench] coupling FP64 oracle frame {f}: fp32 {err['fp32'][-1]:.2e} mixed {err['mixed'][-1]:.2e}",
+                  file=sys.stderr, flush=True)
+        out["rel_error_vs_fp64_per_frame"] = err
+        out["oracle_seconds"] = time.perf_counter() - t0
+    return out
+
+
 def cpu_baseline(scene):
     """The CPU oracle (FP32, one thread) on frame 0 of the same scene: Newton iterations per second."""
     import oracle
@@ -170,6 +224,11 @@ def main():
     ap.add_argument("--batch-scenes", type=int, default=256, help="batched leg: number of scenes (0: skip)")
     ap.add_argument("--batch-steps", type=int, default=3, help="batched leg: timed frames")
     ap.add_argument("--batch-warmup", type=int, default=16, help="batched leg: frames before timing")
+    ap.add_argument("--coupling-frames", type=int, default=60, help="cloth-rigid leg: GPU frames (0: skip)")
+    ap.add_argument("--legs", default="tube,batch,coupling",
+                    help="comma list of legs: tube (the headline), batch, coupling")
+    ap.add_argument("--coupling-oracle-frames", type=int, default=24,
+                    help="cloth-rigid leg: frames compared with the FP64 oracle")
     ap.add_argument("--profile-pcg", action="store_true",
                     help="after the warm-up frames run one calibration copy and one PCG solve, then exit "
                          "(workload for rocprofv3 counter runs)")
@@ -237,6 +296,16 @@ def main():
                 "achieved": nbytes / (spmv_ms * 1e-3) / 1e9 if spmv_n else None}
 
     precisions = {"fp32": ["fp32"], "mixed": ["mixed"], "both": ["fp32", "mixed"]}[args.precision]
+    legs = set(args.legs.split(","))
+    if "tube" not in legs:
+        # a profiling run of the other legs: their JSON objects only
+        line = {"metric": METRIC, "value": None, "unit": "Newton steps/s", "n_gpus": 1, "legs": sorted(legs)}
+        if "batch" in legs and args.batch_scenes > 0:
+            line["batched"] = {"fp32": batched_leg(args, precisions[0], sequential=False)}
+        if "coupling" in legs and args.coupling_frames > 0:
+            line["coupling"] = coupling_leg(args)
+        print(json.dumps(line), flush=True)
+        return
     res = {}
     for prec in precisions:
         res[prec] = run(prec)
@@ -282,12 +351,14 @@ def main():
                          "spmv_avg_launch_ms": m["spmv_ms"], "spmv_bytes_per_launch": m["nbytes"],
                          "spmv_achieved_gbs": m["achieved"],
                          "pcg_ms": round(m["kt"].get("pcg", (float("nan"), 0))[0], 4)}
-    if args.batch_scenes > 0:
+    if "batch" in legs and args.batch_scenes > 0:
         line["batched"] = {"workload": f"{args.batch_scenes} cloth-on-sphere scenes, n ~ U{{17..161}}, seed 0",
                            "unit": "scene-steps/s", "frames": args.batch_steps, "warmup_frames": args.batch_warmup,
                            "fp32": batched_leg(args, "fp32", sequential=True)}
         if "mixed" in precisions:
             line["batched"]["mixed"] = batched_leg(args, "mixed", sequential=False)
+    if "coupling" in legs and args.coupling_frames > 0:
+        line["coupling"] = coupling_leg(args)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(sc)
     print(json.dumps(line), flush=True)
