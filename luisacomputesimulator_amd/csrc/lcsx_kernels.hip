@@ -1081,10 +1081,11 @@ __global__ __launch_bounds__(kB) void k_spmv_pcg(PcgArgs a, SellView A, VertCont
                 uint32_t cl = A.col[b];
                 f3       pc = ld3(a.z, cl) + beta * ld3(a.p_old, cl);
                 m3       M;
-                if (LP)
+                const int8_t ex = LP ? A.val_e[b] : kKeepFp32;
+                if (ex != kKeepFp32)
                 {
                     // FP16 entries times the block's power of two (exact)
-                    float sc = __int_as_float((127 + (int)A.val_e[b]) << 23);
+                    float sc = __int_as_float((127 + (int)ex) << 23);
 #pragma unroll
                     for (int cc = 0; cc < 9; cc++) M.m[cc] = __half2float(A.val_h[cc * (size_t)A.nblocks + b]) * sc;
                 }
@@ -1117,25 +1118,41 @@ __global__ __launch_bounds__(kB) void k_spmv_pcg(PcgArgs a, SellView A, VertCont
 
 // Mixed precision: FP16 copy of the off-diagonal blocks, each block scaled by 2^-e with 2^e the
 // smallest power of two above its largest magnitude (entries in (-1, 1), 11 significant bits).
-__global__ __launch_bounds__(kB) void k_pack_fp16(uint32_t nb, const float* __restrict__ val, __half* __restrict__ vh,
+// One thread per SELL row (rows past n are slice padding). An affine body's rows keep their FP32
+// blocks (exponent kKeepFp32): its 12x12 system couples the orthogonality stiffness with the
+// rotational inertia, eigenvalues about six orders of magnitude apart, which 11 bits cannot hold.
+__global__ __launch_bounds__(kB) void k_pack_fp16(uint32_t nslices, uint32_t n, const uint32_t* __restrict__ sl_off,
+                                                   uint32_t nb, const float* __restrict__ val,
+                                                   const uint8_t* __restrict__ row_kind, __half* __restrict__ vh,
                                                    int8_t* __restrict__ ve)
 {
-    uint32_t b = blockIdx.x * kB + threadIdx.x;
-    if (b >= nb) return;
-    float v[9], m = 0.0f;
-#pragma unroll
-    for (int c = 0; c < 9; c++)
+    uint32_t r = blockIdx.x * kB + threadIdx.x;
+    if (r >= nslices * 64u) return;
+    const uint32_t s = r >> 6, l = r & 63u;
+    const bool     keep = r < n && row_kind != nullptr && row_kind[r] >= 2;
+    for (uint32_t k = sl_off[s]; k < sl_off[s + 1]; k++)
     {
-        v[c] = val[c * (size_t)nb + b];
-        m    = fmaxf(m, fabsf(v[c]));
-    }
-    int e = 0;
-    if (m > 0.0f) frexpf(m, &e);  // m = f 2^e, f in [0.5, 1)
-    e        = max(-100, min(100, e));
-    float sc = __int_as_float((127 - e) << 23);
+        const size_t b = (size_t)k * 64 + l;
+        if (keep)
+        {
+            ve[b] = kKeepFp32;
+            continue;
+        }
+        float v[9], m = 0.0f;
 #pragma unroll
-    for (int c = 0; c < 9; c++) vh[c * (size_t)nb + b] = __float2half_rn(v[c] * sc);
-    ve[b] = (int8_t)e;
+        for (int c = 0; c < 9; c++)
+        {
+            v[c] = val[c * (size_t)nb + b];
+            m    = fmaxf(m, fabsf(v[c]));
+        }
+        int e = 0;
+        if (m > 0.0f) frexpf(m, &e);  // m = f 2^e, f in [0.5, 1)
+        e        = max(-100, min(100, e));
+        float sc = __int_as_float((127 - e) << 23);
+#pragma unroll
+        for (int c = 0; c < 9; c++) vh[c * (size_t)nb + b] = __float2half_rn(v[c] * sc);
+        ve[b] = (int8_t)e;
+    }
 }
 
 // Knuth's TwoSum: s + e = a + b exactly.
@@ -1554,9 +1571,10 @@ void launch_pcg_spmv(const PcgArgs& a, const SellView& A, const VertContacts& vc
     else
         k_spmv_pcg<false><<<g, kB, 0, s>>>(a, A, vc);
 }
-void launch_pack_fp16(uint32_t nblocks, const float* val, __half* val_h, int8_t* val_e, hipStream_t s)
+void launch_pack_fp16(uint32_t nslices, uint32_t n, const uint32_t* sl_off, uint32_t nblocks, const float* val,
+                      const uint8_t* row_kind, __half* val_h, int8_t* val_e, hipStream_t s)
 {
-    if (nblocks) k_pack_fp16<<<grid_for(nblocks, kB), kB, 0, s>>>(nblocks, val, val_h, val_e);
+    if (nblocks) k_pack_fp16<<<grid_for(nslices * 64u, kB), kB, 0, s>>>(nslices, n, sl_off, nblocks, val, row_kind, val_h, val_e);
 }
 void launch_residual(const SegView& g, const float* b, const float* y, const float* minv, float* r, float* z,
                      float* part_rz, float* part_res, hipStream_t s)

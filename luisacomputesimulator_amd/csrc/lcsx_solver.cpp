@@ -400,7 +400,9 @@ void Solver::material()
         b.sell_val  = sell_val.p;
         launch_abd_material(b, stream);
     }
-    if (prm.precision == 1) launch_pack_fp16(S.nblocks, sell_val.p, sell_h.p, sell_e.p, stream);
+    if (prm.precision == 1)
+        launch_pack_fp16(S.nslices, S.nv, sl_off.p, S.nblocks, sell_val.p, S.bodies.empty() ? nullptr : row_kind_d.p,
+                         sell_h.p, sell_e.p, stream);
     toc("material");
 }
 
