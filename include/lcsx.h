@@ -210,7 +210,9 @@ lcsx_status lcsx_solver_run_stage(lcsx_solver* s, const char* stage);
  *   trees      face_tree.* / edge_tree.* with morton, morton_sorted, sorted_to_orig, parent,
  *              children, object_idx, node_aabb
  *   contacts   pairs_vf, pairs_ee (last broad phase), contact_idx, contact_type, contact_n,
- *              contact_k (k1, k2, kappa * area), contact_w, contact_bary
+ *              contact_k (k1, k2, kappa * area), contact_w, contact_bary, ccd_rest_vf /
+ *              ccd_rest_ee (the last line search's candidate pairs whose ACCD ran past the first
+ *              pass's iteration budget)
  * lcsx_solver_write accepts x, v, x_step, x_iter, x_tilde, dx and cgB. */
 lcsx_status lcsx_solver_read(lcsx_solver* s, const char* name, void* out, size_t bytes, size_t* count_out);
 lcsx_status lcsx_solver_write(lcsx_solver* s, const char* name, const void* in, size_t bytes);
@@ -221,6 +223,14 @@ lcsx_status lcsx_solver_kernel_times(lcsx_solver* s, char* buf, size_t buf_len, 
  * (events attached to the launch; reported as "spmv", only launches before convergence are
  * counted) and the contact slots those launches processed ("spmv_slots", slots per launch). */
 lcsx_status lcsx_solver_set_timing(lcsx_solver* s, int level);
+/* PCG launch mode (no counterpart in the reference, whose backend records its own command
+ * lists): 1 captures each convergence-test interval of PCG iterations as a hipGraph and replays
+ * it; 0 (default) launches every kernel directly. Both run the same kernels with the same
+ * arguments in the same order. On MI355X the replay does not shorten the device time of a solve
+ * (the gaps between dependent kernels remain) and each new chunk shape costs a capture, so direct
+ * launches are the default. Timing level 2 always launches directly (its events are attached to
+ * the SpMV launches). */
+lcsx_status lcsx_solver_set_graphs(lcsx_solver* s, int on);
 
 /* ----------------------------------------------- standalone LBVH (device pointers)
  * LBVH over faces (arity 3) or edges (arity 2), lbvh.h:166-304. All pointers are device

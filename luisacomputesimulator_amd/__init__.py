@@ -88,6 +88,7 @@ _SIGS = {
     "lcsx_solver_write": (C.c_int, [C.c_void_p, C.c_char_p, C.c_void_p, C.c_size_t]),
     "lcsx_solver_kernel_times": (C.c_int, [C.c_void_p, C.c_char_p, C.c_size_t, C.c_int]),
     "lcsx_solver_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
+    "lcsx_solver_set_graphs": (C.c_int, [C.c_void_p, C.c_int]),
     "lcsx_lbvh_create": (C.c_int, [C.c_int, C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]),
     "lcsx_lbvh_destroy": (C.c_int, [C.c_void_p]),
     "lcsx_lbvh_construct": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
@@ -166,7 +167,8 @@ def anim_to_c(a: Anim) -> lcsx_fixed_anim:
 _NP = {"f": np.float32, "u": np.uint32, "q": np.uint64, "b": np.uint8}
 # dtype of each readable buffer (see lcsx_solver_read)
 _DTYPES: Dict[str, str] = {n: "u" for n in ("faces", "edges", "hinges", "csr", "stretch_faces", "bending_hinges",
-                                             "fixed_verts", "pairs_vf", "pairs_ee", "contact_idx", "contact_type")}
+                                             "fixed_verts", "pairs_vf", "pairs_ee", "contact_idx", "contact_type",
+                                             "ccd_rest_vf", "ccd_rest_ee")}
 _DTYPES["is_fixed"] = "b"
 
 
@@ -335,6 +337,10 @@ class NewtonSolver:
 
     def set_timing(self, level: int) -> None:
         _check(self.lib.lcsx_solver_set_timing(self.h, level))
+
+    def set_graphs(self, on: bool) -> None:
+        """Replay captured hipGraphs of the PCG iterations, or launch each kernel (the default)."""
+        _check(self.lib.lcsx_solver_set_graphs(self.h, int(on)))
 
     def kernel_times(self, reset: bool = False) -> Dict[str, tuple]:
         buf = C.create_string_buffer(8192)

@@ -209,6 +209,23 @@ bool read_named(Solver& S, const std::string& name, Blob& out)
         out.set(dl(vf ? S.pairs_vf : S.pairs_ee, 2 * (size_t)n, s));
         return true;
     }
+    if (name == "ccd_rest_vf" || name == "ccd_rest_ee")
+    {
+        // the candidate pairs of the last line search whose ACCD ran past the first pass's budget
+        const bool vf    = name == "ccd_rest_vf";
+        auto       cnt   = dl(S.ccd_rest_count, 2, s);
+        uint32_t   n     = std::min(cnt[vf ? 0 : 1], vf ? S.cap_vf : S.cap_ee);
+        auto       ids   = dl(vf ? S.ccd_rest_vf : S.ccd_rest_ee, n, s);
+        auto       pairs = dl(vf ? S.pairs_vf : S.pairs_ee, 2 * (size_t)(vf ? S.cap_vf : S.cap_ee), s);
+        std::vector<uint32_t> o(2 * (size_t)n);
+        for (uint32_t k = 0; k < n; k++)
+        {
+            o[2 * k]     = pairs[2 * (size_t)ids[k]];
+            o[2 * k + 1] = pairs[2 * (size_t)ids[k] + 1];
+        }
+        out.set(o);
+        return true;
+    }
     if (name.rfind("contact_", 0) == 0)
     {
         S.read_status();
@@ -561,6 +578,13 @@ lcsx_status lcsx_solver_set_timing(lcsx_solver* s, int level)
     if (!s || level < 0 || level > 2) return arg_error("invalid arguments");
     s->s.timing      = level >= 1;
     s->s.timing_spmv = level >= 2;
+    return LCSX_OK;
+}
+
+lcsx_status lcsx_solver_set_graphs(lcsx_solver* s, int on)
+{
+    if (!s || (on != 0 && on != 1)) return arg_error("invalid arguments");
+    s->s.use_graphs = on == 1;
     return LCSX_OK;
 }
 

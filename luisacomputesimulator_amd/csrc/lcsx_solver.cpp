@@ -3,6 +3,7 @@
 // reduced to one status read per Newton iteration plus one per PCG chunk.
 #include "lcsx_solver.h"
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 
@@ -55,7 +56,15 @@ hipEvent_t              g_open = nullptr;
 
 Solver::~Solver()
 {
+    clear_graphs();
     if (stream) (void)hipStreamDestroy(stream);
+}
+
+void Solver::clear_graphs()
+{
+    for (auto& kv : pcg_graphs) (void)hipGraphExecDestroy(kv.second);
+    pcg_graphs.clear();
+    graph_epoch++;
 }
 
 unsigned Solver::pair_grid(uint32_t cap) const { return std::max(1u, std::min(kPairGridMax, grid_for(cap, 256))); }
@@ -115,6 +124,7 @@ void Solver::flush_events()
 
 void Solver::alloc_collision(uint32_t vf, uint32_t ee, uint32_t nc)
 {
+    clear_graphs();
     cap_vf = std::max(vf, 1024u);
     cap_ee = std::max(ee, 1024u);
     cap_c  = std::max(nc, 1024u);
@@ -143,6 +153,7 @@ void Solver::alloc_collision(uint32_t vf, uint32_t ee, uint32_t nc)
 
 void Solver::init()
 {
+    clear_graphs();
     S.build(meshes);
     setup_ready = true;
     int ndev = 0;
@@ -527,24 +538,68 @@ int Solver::pcg_run(PcgArgs& a, const SellView& A, const ContactView& cv, const 
         // ending at the residual replacements
         int chunk = std::min(std::max(kPcgChunk, a.check_every > 1 ? (int)a.check_every : 0), max_iters - issued);
         if (replace_every > 0) chunk = std::min(chunk, replace_every - issued % replace_every);
-        for (int k = 0; k < chunk; k++)
+        if (timing_spmv || !use_graphs)
         {
-            PcgArgs b = a;
-            b.p_old   = p[par].p;
-            b.p_new   = p[1 - par].p;
-            b.k       = (uint32_t)(issued + k);
-            hipEvent_t e0 = nullptr, e1 = nullptr;
-            if (timing_spmv)
+            for (int k = 0; k < chunk; k++)
             {
-                e0 = ev_get();
-                e1 = ev_get();
-                spmv_events.push_back({e0, e1});
+                PcgArgs b = a;
+                b.p_old   = p[par].p;
+                b.p_new   = p[1 - par].p;
+                b.k       = (uint32_t)(issued + k);
+                hipEvent_t e0 = nullptr, e1 = nullptr;
+                if (timing_spmv)
+                {
+                    e0 = ev_get();
+                    e1 = ev_get();
+                    spmv_events.push_back({e0, e1});
+                }
+                if (vc.ptr) launch_pcg_contact_t(b, cv, vc, mc, stream);
+                launch_pcg_spmv(b, A, vc, mc, stream, e0, e1);
+                LCSX_HIP(hipGetLastError());
+                launch_pcg_update(b, stream);
+                par = 1 - par;
             }
-            if (vc.ptr) launch_pcg_contact_t(b, cv, vc, mc, stream);
-            launch_pcg_spmv(b, A, vc, mc, stream, e0, e1);
-            LCSX_HIP(hipGetLastError());
-            launch_pcg_update(b, stream);
-            par = 1 - par;
+        }
+        else
+        {
+            // the slot-chunk grid rounded up to a power of two (blocks past the scene tables' chunk
+            // count exit at once), so one graph serves Newton iterations with similar contact counts
+            unsigned g = 0;
+            if (vc.ptr)
+            {
+                g = 1;
+                while (g < mc) g <<= 1;
+                g = std::min(g, max_chunks(slots_per_contact * cap_c));
+            }
+            const GraphKey key{graph_epoch, issued, chunk, par, g, vc.ptr != nullptr, A.val_h != nullptr};
+            auto           it = pcg_graphs.find(key);
+            if (it == pcg_graphs.end())
+            {
+                const auto t0 = std::chrono::steady_clock::now();
+                if (pcg_graphs.size() >= 64) clear_graphs();
+                hipGraph_t     graph = nullptr;
+                hipGraphExec_t exec  = nullptr;
+                // the captured kernels size their slot work from the device tables, not nslots
+                VertContacts vg = vc;
+                vg.nslots       = 0u;
+                LCSX_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+                enqueue_pcg_chunk(a, A, cv, vg, g, issued, chunk, par);
+                LCSX_HIP(hipStreamEndCapture(stream, &graph));
+                const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(graph);
+                LCSX_HIP(e);
+                it = pcg_graphs.emplace(GraphKey{graph_epoch, issued, chunk, par, g, vc.ptr != nullptr, A.val_h != nullptr},
+                                        exec).first;
+                if (timing)
+                {
+                    // host time of capture + instantiation ("graph_capture_host", not device time)
+                    auto& t = times["graph_capture_host"];
+                    t.first += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+                    t.second += 1;
+                }
+            }
+            LCSX_HIP(hipGraphLaunch(it->second, stream));
+            par = (par + chunk) & 1;
         }
         issued += chunk;
         const bool replace = replace_every > 0 && issued % replace_every == 0 && issued < max_iters;
@@ -603,6 +658,23 @@ int Solver::pcg_run(PcgArgs& a, const SellView& A, const ContactView& cv, const 
     }
     account_spmv((uint32_t)most, vc.nslots);
     return most;
+}
+
+// The launches of PCG iterations [k0, k0 + len) (the direct form of a captured chunk).
+void Solver::enqueue_pcg_chunk(const PcgArgs& a, const SellView& A, const ContactView& cv, const VertContacts& vc,
+                               unsigned mc, int k0, int len, int par)
+{
+    for (int k = 0; k < len; k++)
+    {
+        PcgArgs b = a;
+        b.p_old   = p[par].p;
+        b.p_new   = p[1 - par].p;
+        b.k       = (uint32_t)(k0 + k);
+        if (vc.ptr) launch_pcg_contact_t(b, cv, vc, mc, stream);
+        launch_pcg_spmv(b, A, vc, mc, stream, nullptr, nullptr);
+        launch_pcg_update(b, stream);
+        par = 1 - par;
+    }
 }
 
 // Timed SpMV launches: only the first `iters` did work (later ones exit on the converged flag).

@@ -2,6 +2,7 @@
 #pragma once
 #include <map>
 #include <string>
+#include <tuple>
 #include <vector>
 #include "../../include/lcsx.h"
 #include "lcsx_common.h"
@@ -134,6 +135,29 @@ struct Solver
     void       ev_put(hipEvent_t e);
     bool       timing_spmv = false;  // hipEvent pair around every SpMV launch (measurement mode)
     std::vector<std::pair<hipEvent_t, hipEvent_t>> spmv_events;
+
+    // PCG chunks captured as hipGraphs and replayed (one graph launch per convergence-test interval
+    // instead of three kernel launches per iteration), when enabled. A graph bakes in its kernels'
+    // arguments: the key holds what varies between chunks, and graph_epoch changes with every
+    // (re)allocation of a buffer the captured launches name. Not used while SpMV launches are timed.
+    struct GraphKey
+    {
+        uint64_t epoch;
+        int      k0, len, par;
+        unsigned grid;
+        int      contacts, lp;
+        bool     operator<(const GraphKey& o) const
+        {
+            return std::tie(epoch, k0, len, par, grid, contacts, lp) <
+                   std::tie(o.epoch, o.k0, o.len, o.par, o.grid, o.contacts, o.lp);
+        }
+    };
+    std::map<GraphKey, hipGraphExec_t> pcg_graphs;
+    uint64_t graph_epoch = 0;
+    bool     use_graphs  = false;
+    void     clear_graphs();
+    void     enqueue_pcg_chunk(const PcgArgs& a, const SellView& A, const ContactView& cv, const VertContacts& vc,
+                               unsigned mc, int k0, int len, int par);
 };
 
 }  // namespace lcsx
