@@ -144,8 +144,9 @@ def batched_leg(args, precision, sequential):
 
 def coupling_leg(args):
     """Cloth-rigid coupling (129 x 129 cloth pinned at its corners, 24 affine-body boxes): Newton
-    steps/s of all frames in FP32 and in mixed precision, and the position error of both against
-    the FP64 oracle over the first frames (a bounded CPU sample)."""
+    steps/s of all frames in FP32 and in mixed precision with FP16 and with BF16 block storage, and
+    the position error of each against the FP64 oracle over the first frames (a bounded CPU
+    sample)."""
     import numpy as np
 
     import oracle
@@ -154,7 +155,7 @@ def coupling_leg(args):
     out = {"workload": "129x129 cloth pinned at 4 corners + 24 affine-body boxes (0.1 m, solid, density 500, "
                        "stiffness 1e8), seed 0", "frames": args.coupling_frames}
     traj = {}
-    for prec in ("fp32", "mixed"):
+    for prec in ("fp32", "mixed", "mixed_bf16"):
         sc = scenes.cloth_rigid_coupling(129, 24, seed=0)
         sc.params.precision = prec
         g = NewtonSolver.from_scene(sc)
@@ -174,10 +175,11 @@ def coupling_leg(args):
         traj[prec] = xs
         print(f"[bench] coupling {prec}: {out[prec]}", file=sys.stderr, flush=True)
     out["mixed_speedup_vs_fp32"] = out["mixed"]["newton_steps_per_s"] / out["fp32"]["newton_steps_per_s"]
+    out["mixed_bf16_speedup_vs_fp32"] = out["mixed_bf16"]["newton_steps_per_s"] / out["fp32"]["newton_steps_per_s"]
     if args.coupling_oracle_frames > 0:
         sc = scenes.cloth_rigid_coupling(129, 24, seed=0)
         o64 = oracle.Oracle(sc, 64)
-        err = {"fp32": [], "mixed": []}
+        err = {"fp32": [], "mixed": [], "mixed_bf16": []}
         t0 = time.perf_counter()
         for f in range(args.coupling_oracle_frames):
             o64.step()
@@ -185,8 +187,8 @@ def coupling_leg(args):
             scale = np.abs(x64).max()
             for prec in err:
                 err[prec].append(float(np.abs(traj[prec][f] - x64).max() / scale))
-            print(f"[bench] coupling FP64 oracle frame {f}: fp32 {err['fp32'][-1]:.2e} mixed {err['mixed'][-1]:.2e}",
-                  file=sys.stderr, flush=True)
+            print(f"[bench] coupling FP64 oracle frame {f}: fp32 {err['fp32'][-1]:.2e} mixed {err['mixed'][-1]:.2e} "
+                  f"mixed_bf16 {err['mixed_bf16'][-1]:.2e}", file=sys.stderr, flush=True)
         out["rel_error_vs_fp64_per_frame"] = err
         out["oracle_seconds"] = time.perf_counter() - t0
     return out

@@ -94,7 +94,8 @@ typedef struct lcsx_params
     float newton_tol;           /* converged when max|dx| < newton_tol * dt                  */
     int   precision;            /* 0: FP32 (the reference's path); 1: mixed precision: FP16 block
                                    storage with per-block power-of-two scale in the PCG SpMV inside
-                                   FP32 iterative refinement, FP64 CCD, distances and barrier   */
+                                   FP32 iterative refinement, FP64 CCD, distances and barrier;
+                                   2: the same with BF16 block storage                          */
 } lcsx_params;
 
 /* Scalars of the last Newton iteration / frame. */

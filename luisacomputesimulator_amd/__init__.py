@@ -140,7 +140,7 @@ def params_to_c(p: Params) -> lcsx_params:
     c.damping_cloth = p.damping_cloth
     c.bending_scaling = p.bending_scaling
     c.newton_tol = p.newton_tol
-    c.precision = {"fp32": 0, "mixed": 1}[p.precision]
+    c.precision = {"fp32": 0, "mixed": 1, "mixed_bf16": 2}[p.precision]
     return c
 
 

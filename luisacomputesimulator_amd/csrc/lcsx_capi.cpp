@@ -333,7 +333,8 @@ void lcsx_material_default(lcsx_material* m)
 lcsx_status lcsx_solver_create(const lcsx_params* params, int device, lcsx_solver** out)
 {
     if (!out) return arg_error("out is NULL");
-    if (params && params->precision != 0 && params->precision != 1) return arg_error("precision must be 0 (FP32) or 1 (mixed)");
+    if (params && (params->precision < 0 || params->precision > 2))
+        return arg_error("precision must be 0 (FP32), 1 (mixed, FP16 blocks) or 2 (mixed, BF16 blocks)");
     auto* h = new lcsx_solver();
     if (params) h->s.prm = *params;
     else lcsx_params_default(&h->s.prm);

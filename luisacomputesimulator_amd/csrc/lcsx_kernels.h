@@ -59,6 +59,7 @@ struct SellView
     uint32_t        nblocks;
     const __half*   val_h;  // mixed precision: 9 planes of nblocks, block scaled by 2^-exp
     const int8_t*   val_e;  // per block exponent; kKeepFp32: the block is read from val
+    int             bf16;   // val_h holds BF16 bit patterns instead of FP16
 };
 
 // Exponent of a block that stays FP32 in the mixed-precision SpMV (packed exponents lie in [-100, 100]).
@@ -276,7 +277,7 @@ void launch_pcg_spmv(const PcgArgs& a, const SellView& A, const VertContacts& vc
 void launch_pcg_update(const PcgArgs& a, hipStream_t s);
 // Mixed precision: FP16 copy of the off-diagonal blocks with a power-of-two scale per block.
 void launch_pack_fp16(uint32_t nslices, uint32_t n, const uint32_t* sl_off, uint32_t nblocks, const float* val,
-                      const uint8_t* row_kind, __half* val_h, int8_t* val_e, hipStream_t s);
+                      const uint8_t* row_kind, __half* val_h, int8_t* val_e, int bf16, hipStream_t s);
 // Residual replacement: r = b - y, z = Minv r, per-block r.z with TwoSum compensation into
 // part_rz[block] (sum + error) and part_res[2 block] (sum, error).
 void launch_residual(const SegView& g, const float* b, const float* y, const float* minv, float* r, float* z,

@@ -1038,7 +1038,7 @@ __global__ __launch_bounds__(kB) void k_pcg_init(PcgArgs a)
 // 64-slot chunks (balanced whatever the distribution of contacts over rows), adding each row's
 // sum into the contact accumulator qc (folded into q by the update). p.q is reduced over both
 // parts in the same launch and the last block sets alpha = r.z / p.q (precond_cg.cpp:24-31).
-template <bool LP>
+template <int LP>
 __global__ __launch_bounds__(kB) void k_spmv_pcg(PcgArgs a, SellView A, VertContacts vc)
 {
     __shared__ float sh[4];
@@ -1084,10 +1084,20 @@ __global__ __launch_bounds__(kB) void k_spmv_pcg(PcgArgs a, SellView A, VertCont
                 const int8_t ex = LP ? A.val_e[b] : kKeepFp32;
                 if (ex != kKeepFp32)
                 {
-                    // FP16 entries times the block's power of two (exact)
+                    // FP16 / BF16 entries times the block's power of two (exact)
                     float sc = __int_as_float((127 + (int)ex) << 23);
+                    if (LP == 2)
+                    {
+                        const uint16_t* vb = reinterpret_cast<const uint16_t*>(A.val_h);
 #pragma unroll
-                    for (int cc = 0; cc < 9; cc++) M.m[cc] = __half2float(A.val_h[cc * (size_t)A.nblocks + b]) * sc;
+                        for (int cc = 0; cc < 9; cc++)
+                            M.m[cc] = __uint_as_float((uint32_t)vb[cc * (size_t)A.nblocks + b] << 16) * sc;
+                    }
+                    else
+                    {
+#pragma unroll
+                        for (int cc = 0; cc < 9; cc++) M.m[cc] = __half2float(A.val_h[cc * (size_t)A.nblocks + b]) * sc;
+                    }
                 }
                 else
                 {
@@ -1116,6 +1126,13 @@ __global__ __launch_bounds__(kB) void k_spmv_pcg(PcgArgs a, SellView A, VertCont
     if (threadIdx.x == 0) a.part_pq[blockIdx.x] = bs;
 }
 
+// BF16 of a finite float, rounded to nearest even (the upper 16 bits of the rounded float).
+LCSX_D uint16_t float_to_bf16_rn(float x)
+{
+    const uint32_t u = __float_as_uint(x);
+    return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+
 // Mixed precision: FP16 copy of the off-diagonal blocks, each block scaled by 2^-e with 2^e the
 // smallest power of two above its largest magnitude (entries in (-1, 1), 11 significant bits).
 // One thread per SELL row (rows past n are slice padding). An affine body's rows keep their FP32
@@ -1124,7 +1141,7 @@ __global__ __launch_bounds__(kB) void k_spmv_pcg(PcgArgs a, SellView A, VertCont
 __global__ __launch_bounds__(kB) void k_pack_fp16(uint32_t nslices, uint32_t n, const uint32_t* __restrict__ sl_off,
                                                    uint32_t nb, const float* __restrict__ val,
                                                    const uint8_t* __restrict__ row_kind, __half* __restrict__ vh,
-                                                   int8_t* __restrict__ ve)
+                                                   int8_t* __restrict__ ve, int bf16)
 {
     uint32_t r = blockIdx.x * kB + threadIdx.x;
     if (r >= nslices * 64u) return;
@@ -1149,8 +1166,17 @@ __global__ __launch_bounds__(kB) void k_pack_fp16(uint32_t nslices, uint32_t n, 
         if (m > 0.0f) frexpf(m, &e);  // m = f 2^e, f in [0.5, 1)
         e        = max(-100, min(100, e));
         float sc = __int_as_float((127 - e) << 23);
+        if (bf16)
+        {
 #pragma unroll
-        for (int c = 0; c < 9; c++) vh[c * (size_t)nb + b] = __float2half_rn(v[c] * sc);
+            for (int c = 0; c < 9; c++)
+                reinterpret_cast<uint16_t*>(vh)[c * (size_t)nb + b] = float_to_bf16_rn(v[c] * sc);
+        }
+        else
+        {
+#pragma unroll
+            for (int c = 0; c < 9; c++) vh[c * (size_t)nb + b] = __float2half_rn(v[c] * sc);
+        }
         ve[b] = (int8_t)e;
     }
 }
@@ -1559,22 +1585,31 @@ void launch_pcg_spmv(const PcgArgs& a, const SellView& A, const VertContacts& vc
                      hipEvent_t start, hipEvent_t stop)
 {
     uint32_t g = a.seg.n_rb + (vc.ptr ? max_chunks : 0u);
-    if (A.val_h)
+    if (A.val_h && A.bf16)
     {
         if (start)
-            hipExtLaunchKernelGGL(k_spmv_pcg<true>, dim3(g), dim3(kB), 0, s, start, stop, 0, a, A, vc);
+            hipExtLaunchKernelGGL(k_spmv_pcg<2>, dim3(g), dim3(kB), 0, s, start, stop, 0, a, A, vc);
         else
-            k_spmv_pcg<true><<<g, kB, 0, s>>>(a, A, vc);
+            k_spmv_pcg<2><<<g, kB, 0, s>>>(a, A, vc);
+    }
+    else if (A.val_h)
+    {
+        if (start)
+            hipExtLaunchKernelGGL(k_spmv_pcg<1>, dim3(g), dim3(kB), 0, s, start, stop, 0, a, A, vc);
+        else
+            k_spmv_pcg<1><<<g, kB, 0, s>>>(a, A, vc);
     }
     else if (start)
-        hipExtLaunchKernelGGL(k_spmv_pcg<false>, dim3(g), dim3(kB), 0, s, start, stop, 0, a, A, vc);
+        hipExtLaunchKernelGGL(k_spmv_pcg<0>, dim3(g), dim3(kB), 0, s, start, stop, 0, a, A, vc);
     else
-        k_spmv_pcg<false><<<g, kB, 0, s>>>(a, A, vc);
+        k_spmv_pcg<0><<<g, kB, 0, s>>>(a, A, vc);
 }
 void launch_pack_fp16(uint32_t nslices, uint32_t n, const uint32_t* sl_off, uint32_t nblocks, const float* val,
-                      const uint8_t* row_kind, __half* val_h, int8_t* val_e, hipStream_t s)
+                      const uint8_t* row_kind, __half* val_h, int8_t* val_e, int bf16, hipStream_t s)
 {
-    if (nblocks) k_pack_fp16<<<grid_for(nslices * 64u, kB), kB, 0, s>>>(nslices, n, sl_off, nblocks, val, row_kind, val_h, val_e);
+    if (nblocks)
+        k_pack_fp16<<<grid_for(nslices * 64u, kB), kB, 0, s>>>(nslices, n, sl_off, nblocks, val, row_kind, val_h, val_e,
+                                                               bf16);
 }
 void launch_residual(const SegView& g, const float* b, const float* y, const float* minv, float* r, float* z,
                      float* part_rz, float* part_res, hipStream_t s)

@@ -191,7 +191,7 @@ void Solver::init()
     scene_stats.assign(ns, lcsx_stats{});
     verts_per_scene.assign(ns, 0);
     for (uint32_t sc = 0; sc < ns; sc++) verts_per_scene[sc] = S.sc_verts[sc];
-    if (prm.precision == 1)
+    if (prm.precision >= 1)
     {
         sell_h.alloc(9 * (size_t)std::max(S.nblocks, 1u));
         sell_e.alloc(std::max(S.nblocks, 1u));
@@ -307,10 +307,13 @@ SegView Solver::seg_view(bool with_chunks) const
     return g;
 }
 
-SellView Solver::sell_view() const { return SellView{sl_off.p, sell_col.p, sell_val.p, diag_mat.p, S.nblocks, nullptr, nullptr}; }
+SellView Solver::sell_view() const
+{
+    return SellView{sl_off.p, sell_col.p, sell_val.p, diag_mat.p, S.nblocks, nullptr, nullptr, 0};
+}
 SellView Solver::sell_view_lp() const
 {
-    return SellView{sl_off.p, sell_col.p, sell_val.p, diag_mat.p, S.nblocks, sell_h.p, sell_e.p};
+    return SellView{sl_off.p, sell_col.p, sell_val.p, diag_mat.p, S.nblocks, sell_h.p, sell_e.p, prm.precision == 2};
 }
 
 // ------------------------------------------------------------------ stages
@@ -411,9 +414,9 @@ void Solver::material()
         b.sell_val  = sell_val.p;
         launch_abd_material(b, stream);
     }
-    if (prm.precision == 1)
+    if (prm.precision >= 1)
         launch_pack_fp16(S.nslices, S.nv, sl_off.p, S.nblocks, sell_val.p, S.bodies.empty() ? nullptr : row_kind_d.p,
-                         sell_h.p, sell_e.p, stream);
+                         sell_h.p, sell_e.p, prm.precision == 2, stream);
     toc("material");
 }
 
@@ -477,7 +480,7 @@ void Solver::dcd(bool from_ccd_pairs)
     a.offset = offset.p;
     a.dhat   = dhat.p;
     a.kappa  = prm.stiffness_collision;
-    a.fp64   = prm.precision == 1;
+    a.fp64   = prm.precision >= 1;
     a.out    = out;
     a.vert_body = vert_body_d.p;
     a.pairs = pairs_vf.p;
@@ -734,7 +737,7 @@ void Solver::pcg()
     a.check_every = (uint32_t)std::max(1, prm.pcg_check_every);
     a.qc       = vc.ptr ? vc_qc.p : nullptr;
     a.seg      = seg_view(vc.ptr != nullptr);
-    const bool mixed = prm.precision == 1;
+    const bool mixed = prm.precision >= 1;
     // replacements at multiples of the test interval, so the test after one sees the true residual
     const int every = (int)a.check_every;
     const int repl  = mixed ? every * ((kMixedSweep + every - 1) / every) : 0;
@@ -772,7 +775,7 @@ void Solver::ccd()
         c.sst      = sst.p;
         c.vert_scene = vert_scene_d.p;
         c.vert_body  = vert_body_d.p;
-        c.fp64     = prm.precision == 1;
+        c.fp64     = prm.precision >= 1;
         c.budget   = kCcdFirstPassIters;
         launch_fill_u32(ccd_rest_count.p, 2, 0u, stream);
         for (int rest = 0; rest < 2; rest++)

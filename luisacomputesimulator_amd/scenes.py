@@ -109,8 +109,9 @@ class Params:
     damping_cloth: float = 2.0
     bending_scaling: float = 1.0
     newton_tol: float = 1e-2
-    # "fp32" (the reference's path) or "mixed" (FP16 blocks + FP32 refinement, FP64 CCD and barrier);
-    # not part of pack(): the oracle runs FP32 or its FP64 twin
+    # "fp32" (the reference's path), "mixed" (FP16 blocks + FP32 refinement, FP64 CCD and barrier)
+    # or "mixed_bf16" (the same with BF16 blocks); not part of pack(): the oracle runs FP32 or its
+    # FP64 twin
     precision: str = "fp32"
 
     def pack(self) -> List[float]:

@@ -45,10 +45,11 @@ def test_precision_parameter_is_validated():
     p = lcsx_params()
     lib.lcsx_params_default(C.byref(p))
     assert p.precision == 0
-    p.precision = 2
+    p.precision = 3
     h = C.c_void_p()
     assert lib.lcsx_solver_create(C.byref(p), 0, C.byref(h)) != 0
     assert b"precision" in lib.lcsx_last_error()
     NewtonSolver(scenes.Params(precision="mixed"))  # accepted
+    NewtonSolver(scenes.Params(precision="mixed_bf16"))
     with pytest.raises(KeyError):
         NewtonSolver(scenes.Params(precision="fp16"))
