@@ -263,14 +263,31 @@ bool load_scene_json(const std::string& path, const std::string& mesh_dir, lcsx_
     for (const json::Value& sh : shells->a)
     {
         if (sh.kind != json::Value::Obj) continue;
-        const json::Value* st = sh.get("shell_type");
-        if (st && st->kind == json::Value::Str && st->s != "Cloth" && st->s != "stretch_model")
+        // shell_type and material.type (app_simulation_demo_config.cpp:362-375, 397-420, 540-575):
+        // Cloth (also "stretch_model") or Rigid; the material's "type", when given, must agree
+        const json::Value* st    = sh.get("shell_type");
+        std::string        stype = st && st->kind == json::Value::Str ? st->s : "Cloth";
+        if (stype == "stretch_model") stype = "Cloth";
+        const json::Value* mat = sh.get("material");
+        if (mat && mat->kind == json::Value::Arr && !mat->a.empty()) mat = &mat->a[0];
+        const json::Value* mt = mat && mat->kind == json::Value::Obj ? mat->get("type") : nullptr;
+        if (mt && mt->kind == json::Value::Str)
         {
-            err = "shell_type " + st->s + " is not supported by this build (cloth only)";
+            if (!st) stype = mt->s;
+            else if (mt->s != stype)
+            {
+                err = "shell_type " + stype + " and material type " + mt->s + " disagree";
+                return false;
+            }
+        }
+        if (stype != "Cloth" && stype != "Rigid")
+        {
+            err = "shell_type " + stype + " is not supported by this build (Cloth and Rigid)";
             return false;
         }
         MeshDesc m;
         lcsx_material_default(&m.mat);
+        if (stype == "Rigid") m.mat.body = 1;  // RigidMaterial: is_shell defaults to true
         const json::Value* mn = sh.get("model_name");
         if (!mn || mn->kind != json::Value::Str)
         {
@@ -294,17 +311,24 @@ bool load_scene_json(const std::string& path, const std::string& mesh_dir, lcsx_
         else vec3(s, sc);
         std::vector<float> model = m.x;  // pins are selected on model coordinates
         model_transform(tr, rot, sc, m.x);
-        const json::Value* mat = sh.get("material");
-        if (mat && mat->kind == json::Value::Arr && !mat->a.empty()) mat = &mat->a[0];
         if (mat && mat->kind == json::Value::Obj)
         {
             num(mat->get("thickness"), m.mat.thickness);
-            num(mat->get("youngs_modulus"), m.mat.youngs_modulus);
-            num(mat->get("poisson_ratio"), m.mat.poisson_ratio);
-            num(mat->get("area_bending_stiffness"), m.mat.area_bending_stiffness);
             num(mat->get("mass"), m.mat.mass);
             num(mat->get("density"), m.mat.density);
             num(mat->get("d_hat"), m.mat.d_hat);
+            if (stype == "Rigid")
+            {
+                num(mat->get("stiffness"), m.mat.rigid_stiffness);
+                const json::Value* isv = mat->get("is_shell");
+                if (isv && isv->kind == json::Value::Bool && !isv->b) m.mat.body = 2;  // rigid solid
+            }
+            else
+            {
+                num(mat->get("youngs_modulus"), m.mat.youngs_modulus);
+                num(mat->get("poisson_ratio"), m.mat.poisson_ratio);
+                num(mat->get("area_bending_stiffness"), m.mat.area_bending_stiffness);
+            }
         }
         const json::Value* fps = sh.get("fixed_points");
         if (fps && fps->kind == json::Value::Arr)

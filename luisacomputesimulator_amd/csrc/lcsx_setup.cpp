@@ -6,6 +6,7 @@
 //   elements / CSR / offsets Solver/Initializer/init_sim_data.cpp:19-48, 189-480, 760-896
 #include "lcsx_setup.h"
 #include <algorithm>
+#include <stdexcept>
 #include <cmath>
 #include <map>
 #include <numeric>
@@ -156,6 +157,10 @@ void Setup::build(const std::vector<MeshDesc>& meshes)
         const bool            rigid = m.mat.body != 0;
         std::vector<uint32_t> E, H;
         topology(m.tris, E, H);
+        // init_mesh_data.cpp:396-421: a rigid mesh (shell or solid) must be closed, i.e. every
+        // edge is a hinge of two triangles
+        if (rigid && H.size() / 4 != E.size() / 2)
+            throw std::runtime_error("mesh " + std::to_string(mi) + ": a rigid body's mesh must be closed");
         uint32_t mv = (uint32_t)(m.x.size() / 3);
         rest_x.insert(rest_x.end(), m.x.begin(), m.x.end());
         vert_mesh.insert(vert_mesh.end(), mv, (uint32_t)mi);

@@ -430,7 +430,9 @@ def model_transform(x: np.ndarray, translation=None, rotation=None, scale=None) 
 
 def from_json(path: str, mesh_dir: str) -> Scene:
     """A scene from the reference's scene JSON (Application/app_simulation_demo_config.cpp:174-745):
-    scene keys, cloth shells with their OBJ mesh, model transform, material and fixed points.
+    scene keys, cloth and rigid shells with their OBJ mesh, model transform, material and fixed
+    points. A "Rigid" shell is an affine body: a rigid shell unless its material sets
+    "is_shell": false (a rigid solid); "stiffness" is its orthogonality stiffness.
     The model transform is translate * Rx * Ry * Rz * scale (Core/affine_position.h:41-98),
     evaluated in float32 in the C loader's order."""
     import json
@@ -463,11 +465,26 @@ def from_json(path: str, mesh_dir: str) -> Scene:
         m = sh.get("material")
         if isinstance(m, list) and m:
             m = m[0]
+        stype = sh.get("shell_type", m.get("type", "Cloth") if isinstance(m, dict) else "Cloth")
+        stype = "Cloth" if stype == "stretch_model" else stype
+        if isinstance(m, dict) and "type" in m and "shell_type" in sh and m["type"] != stype:
+            raise ValueError(f"shell_type {stype} and material type {m['type']} disagree")
+        if stype not in ("Cloth", "Rigid"):
+            raise ValueError(f"shell_type {stype} is not supported (Cloth and Rigid)")
+        keys = ["thickness", "mass", "density", "d_hat"]
+        if stype == "Rigid":
+            mat.body = "rigid_shell"
+        else:
+            keys += ["youngs_modulus", "poisson_ratio", "area_bending_stiffness"]
         if isinstance(m, dict):
-            for key in ("thickness", "youngs_modulus", "poisson_ratio", "area_bending_stiffness", "mass",
-                        "density", "d_hat"):
-                if isinstance(m.get(key), (int, float)):
+            for key in keys:
+                if isinstance(m.get(key), (int, float)) and not isinstance(m.get(key), bool):
                     setattr(mat, key, float(m[key]))
+            if stype == "Rigid":
+                if isinstance(m.get("stiffness"), (int, float)) and not isinstance(m.get("stiffness"), bool):
+                    mat.rigid_stiffness = float(m["stiffness"])
+                if m.get("is_shell") is False:
+                    mat.body = "rigid_solid"
         fixed = []
         for fp in sh.get("fixed_points", []):
             fi = fp.get("fixed_info", {})

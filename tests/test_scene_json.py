@@ -68,3 +68,58 @@ def test_rotation_cylinder_7k_contents():
     h = s.read("hinges").reshape(-1, 4)
     assert len(e) == len(h) + (len(e) - len(h))  # hinges = interior edges
     assert len(h) > 0.9 * len(e)
+
+
+PER_ROW = {"rest_x": 3, "is_fixed": 1, "vert_area": 1, "vert_mass": 1, "offset": 1, "dhat": 1}
+
+
+@pytest.mark.parametrize("name,bodies", [("rigid_test_solid", ["rigid_shell", "rigid_shell"]),
+                                         ("cloth_rigid_coupling_drop", ["cloth", "rigid_shell"])])
+def test_rigid_scene_json(name, bodies):
+    """Scenes with "shell_type": "Rigid" (app_simulation_demo_config.cpp:362-375, 540-575): each
+    rigid shell becomes an affine body (a rigid shell by default, RigidMaterial::is_shell) whose
+    four rows (p, a0, a1, a2) follow the scene's vertices, starting at (0, I); the setup's vertex
+    rows equal the oracle's."""
+    path = os.path.join(SCENE_DIR, name + ".json")
+    sc = scenes.from_json(path, SCENE_DIR)
+    assert [m.material.body for m in sc.meshes] == bodies
+    s = load_product(path)
+    o = oracle.Oracle(sc)
+    nb = sum(b != "cloth" for b in bodies)
+    nv = sum(len(m.verts) for m in sc.meshes)
+    assert s.num_verts == nv + 4 * nb
+    for key in STATIC:
+        got, ref = s.read(key), o.get(key)
+        if key in PER_ROW:
+            w = PER_ROW[key]
+            assert len(got) == w * (nv + 4 * nb) and len(ref) == w * nv, key
+            got = got[: w * nv]
+        assert got.shape == ref.shape, key
+        assert np.array_equal(got, ref.astype(got.dtype)), key
+    q0 = s.read("rest_x").reshape(-1, 3)[nv:].reshape(nb, 4, 3)
+    assert np.array_equal(q0, np.tile(np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], np.float32), (nb, 1, 1)))
+
+
+def test_rigid_shell_must_be_closed():
+    """The reference rejects a rigid shell on an open mesh (init_mesh_data.cpp:404-412); so does
+    the loader, before any device is needed."""
+    path = os.path.join(SCENE_DIR, "rigid_test_shell.json")
+    s = NewtonSolver()
+    s.load_scene_json(path, SCENE_DIR)
+    with pytest.raises(LcsxError, match="closed"):
+        s.init_solver()
+
+
+def test_rigid_material_fields_from_json(tmp_path):
+    """A Rigid material's stiffness and is_shell (false: a rigid solid) reach the Python mirror."""
+    import json
+    js = {"shells": [{"model_name": "cube.obj", "shell_type": "Rigid", "scale": 0.1,
+                      "material": {"is_shell": False, "stiffness": 1e8, "density": 500.0}}]}
+    p = tmp_path / "solid.json"
+    p.write_text(json.dumps(js))
+    sc = scenes.from_json(str(p), SCENE_DIR)
+    m = sc.meshes[0].material
+    assert (m.body, m.rigid_stiffness, m.density) == ("rigid_solid", 1e8, 500.0)
+    s = load_product(str(p))
+    assert s.num_verts == 8 + 4
+    assert np.all(s.read("offset")[:8] == 0.0)  # a rigid solid has no contact thickness
