@@ -194,6 +194,50 @@ def coupling_leg(args):
     return out
 
 
+def twist_error_leg(args):
+    """Twisting sheet at a size the FP64 oracle finishes in minutes (n x n, edges turned at -/+ pi
+    rad/s, 120 frames): relative position error of the GPU's FP32, mixed (FP16) and mixed (BF16)
+    forms, and of the FP32 oracle, against the FP64 oracle at frames 30, 60 and 120."""
+    import numpy as np
+
+    import oracle
+    from luisacomputesimulator_amd import NewtonSolver, scenes
+
+    n, marks = args.twist_error_n, (30, 60, 120)
+    out = {"workload": f"twisting sheet {n}x{n}, edges turned at -/+ pi rad/s, 120 frames", "frames": list(marks)}
+    xs = {}
+    for prec in ("fp32", "mixed", "mixed_bf16"):
+        sc = scenes.twisting_sheet(n)
+        sc.params.precision = prec
+        g = NewtonSolver.from_scene(sc)
+        t0, newton, xs[prec] = time.perf_counter(), 0, {}
+        for f in range(1, marks[-1] + 1):
+            g.physics_step_gpu(1)
+            newton += g.stats()["newton_iters"]
+            if f in marks:
+                xs[prec][f] = g.get_simulation_results().copy()
+        out[prec] = {"newton_iters": newton, "seconds": time.perf_counter() - t0}
+        print(f"[bench] twist {n} {prec}: {newton} Newton iterations", file=sys.stderr, flush=True)
+    sc = scenes.twisting_sheet(n)
+    o64, o32 = oracle.Oracle(sc, 64), oracle.Oracle(sc, 32)
+    err = {k: [] for k in ("fp32", "mixed", "mixed_bf16", "oracle_fp32")}
+    t0 = time.perf_counter()
+    for f in range(1, marks[-1] + 1):
+        o64.step()
+        o32.step()
+        if f in marks:
+            x64 = o64.get("x").reshape(-1, 3)
+            scale = np.abs(x64).max()
+            for prec in ("fp32", "mixed", "mixed_bf16"):
+                err[prec].append(float(np.abs(xs[prec][f] - x64).max() / scale))
+            err["oracle_fp32"].append(float(np.abs(o32.get("x").reshape(-1, 3) - x64).max() / scale))
+            print(f"[bench] twist {n} frame {f} vs FP64: " + ", ".join(f"{k} {v[-1]:.2e}" for k, v in err.items()),
+                  file=sys.stderr, flush=True)
+    out["rel_error_vs_fp64"] = err
+    out["oracle_seconds"] = time.perf_counter() - t0
+    return out
+
+
 def cpu_baseline(scene):
     """The CPU oracle (FP32, one thread) on frame 0 of the same scene: Newton iterations per second."""
     import oracle
@@ -228,7 +272,9 @@ def main():
     ap.add_argument("--batch-warmup", type=int, default=16, help="batched leg: frames before timing")
     ap.add_argument("--coupling-frames", type=int, default=60, help="cloth-rigid leg: GPU frames (0: skip)")
     ap.add_argument("--legs", default="tube,batch,coupling",
-                    help="comma list of legs: tube (the headline), batch, coupling")
+                    help="comma list of legs: tube (the headline), batch, coupling, twist_error (not run by "
+                         "default: its FP64 oracle takes minutes)")
+    ap.add_argument("--twist-error-n", type=int, default=65, help="twist_error leg: vertices per side")
     ap.add_argument("--coupling-oracle-frames", type=int, default=24,
                     help="cloth-rigid leg: frames compared with the FP64 oracle")
     ap.add_argument("--profile-pcg", action="store_true",
@@ -306,6 +352,8 @@ def main():
             line["batched"] = {"fp32": batched_leg(args, precisions[0], sequential=False)}
         if "coupling" in legs and args.coupling_frames > 0:
             line["coupling"] = coupling_leg(args)
+        if "twist_error" in legs:
+            line["twist_error"] = twist_error_leg(args)
         print(json.dumps(line), flush=True)
         return
     res = {}
@@ -361,6 +409,8 @@ def main():
             line["batched"]["mixed"] = batched_leg(args, "mixed", sequential=False)
     if "coupling" in legs and args.coupling_frames > 0:
         line["coupling"] = coupling_leg(args)
+    if "twist_error" in legs:
+        line["twist_error"] = twist_error_leg(args)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(sc)
     print(json.dumps(line), flush=True)
