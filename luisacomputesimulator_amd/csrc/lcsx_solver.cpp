@@ -25,6 +25,7 @@ struct EventPool
     {
         std::string name;
         hipEvent_t  a, b;
+        bool        count = true;  // false: adds its time to the name's current call
     };
     std::vector<hipEvent_t> free_;
     std::vector<Rec>        pending;
@@ -57,6 +58,9 @@ hipEvent_t              g_open = nullptr;
 Solver::~Solver()
 {
     clear_graphs();
+    if (side) (void)hipStreamDestroy(side);
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (ev_join) (void)hipEventDestroy(ev_join);
     if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -81,7 +85,15 @@ void Solver::toc(const char* name)
     if (!timing || !g_open) return;
     hipEvent_t b = g_pool->get();
     LCSX_HIP(hipEventRecord(b, stream));
-    g_pool->pending.push_back({name, g_open, b});
+    g_pool->pending.push_back({name, g_open, b, true});
+    g_open = nullptr;
+}
+void Solver::toc_part(const char* name)
+{
+    if (!timing || !g_open) return;
+    hipEvent_t b = g_pool->get();
+    LCSX_HIP(hipEventRecord(b, stream));
+    g_pool->pending.push_back({name, g_open, b, false});
     g_open = nullptr;
 }
 
@@ -113,7 +125,7 @@ void Solver::flush_events()
             {
                 auto& t = times[r.name];
                 t.first += ms;
-                t.second += 1;
+                if (r.count) t.second += 1;
             }
             g_pool->free_.push_back(r.a);
             g_pool->free_.push_back(r.b);
@@ -164,6 +176,9 @@ void Solver::init()
     }
     LCSX_HIP(hipSetDevice(device));
     if (!stream) LCSX_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    if (!side) LCSX_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));  // default priority, as stream
+    if (!ev_fork) LCSX_HIP(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+    if (!ev_join) LCSX_HIP(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
     const uint32_t nv = S.nv;
     for (auto* b : {&x, &v, &x_step, &x_iter, &x_tilde, &dx, &cgB, &r, &z, &p[0], &p[1], &q, &spmv_tmp}) b->alloc(3 * (size_t)nv);
     diag.alloc(9 * (size_t)nv);
@@ -420,50 +435,58 @@ void Solver::material()
     toc("material");
 }
 
+void Solver::query_faces(bool ccd)
+{
+    if (S.nf == 0) return;
+    const float* x0 = ccd ? x_iter.p : x.p;
+    Status*      st = status.p;
+    tree_face.refit(x0, x.p, offset.p, stream);
+    QueryArgs q{};
+    q.vert_scene = vert_scene_d.p;
+    q.active     = active_d.p;
+    q.row_kind   = row_kind_d.p;
+    q.mode       = QueryVerts;
+    q.n_queries  = S.nv;
+    q.x0         = x0;
+    q.x1         = x.p;
+    q.offset     = offset.p;
+    q.dhat       = dhat.p;
+    q.pairs      = pairs_vf.p;
+    q.capacity   = cap_vf;
+    q.count      = ccd ? &st->n_vf_ccd : &st->n_vf;
+    q.overflow   = &st->overflow;
+    lbvh_query(tree_face, q, stream);
+}
+
+void Solver::query_edges(bool ccd)
+{
+    if (S.ne == 0) return;
+    const float* x0 = ccd ? x_iter.p : x.p;
+    Status*      st = status.p;
+    tree_edge.refit(x0, x.p, offset.p, stream);
+    QueryArgs q{};
+    q.vert_scene = vert_scene_d.p;
+    q.active     = active_d.p;
+    q.row_kind   = row_kind_d.p;
+    q.mode       = QueryEdges;
+    q.n_queries  = S.ne;
+    q.x0         = x0;
+    q.x1         = x.p;
+    q.edges      = edges.p;
+    q.offset     = offset.p;
+    q.dhat       = dhat.p;
+    q.pairs      = pairs_ee.p;
+    q.capacity   = cap_ee;
+    q.count      = ccd ? &st->n_ee_ccd : &st->n_ee;
+    q.overflow   = &st->overflow;
+    lbvh_query(tree_edge, q, stream);
+}
+
 void Solver::broad_phase(bool ccd)
 {
     tic();
-    const float* x0 = ccd ? x_iter.p : x.p;
-    Status*      st = status.p;
-    if (S.nf > 0)
-    {
-        tree_face.refit(x0, x.p, offset.p, stream);
-        QueryArgs q{};
-        q.vert_scene = vert_scene_d.p;
-        q.active     = active_d.p;
-        q.row_kind   = row_kind_d.p;
-        q.mode      = QueryVerts;
-        q.n_queries = S.nv;
-        q.x0        = x0;
-        q.x1        = x.p;
-        q.offset    = offset.p;
-        q.dhat      = dhat.p;
-        q.pairs     = pairs_vf.p;
-        q.capacity  = cap_vf;
-        q.count     = ccd ? &st->n_vf_ccd : &st->n_vf;
-        q.overflow  = &st->overflow;
-        lbvh_query(tree_face, q, stream);
-    }
-    if (S.ne > 0)
-    {
-        tree_edge.refit(x0, x.p, offset.p, stream);
-        QueryArgs q{};
-        q.vert_scene = vert_scene_d.p;
-        q.active     = active_d.p;
-        q.row_kind   = row_kind_d.p;
-        q.mode      = QueryEdges;
-        q.n_queries = S.ne;
-        q.x0        = x0;
-        q.x1        = x.p;
-        q.edges     = edges.p;
-        q.offset    = offset.p;
-        q.dhat      = dhat.p;
-        q.pairs     = pairs_ee.p;
-        q.capacity  = cap_ee;
-        q.count     = ccd ? &st->n_ee_ccd : &st->n_ee;
-        q.overflow  = &st->overflow;
-        lbvh_query(tree_edge, q, stream);
-    }
+    query_faces(ccd);
+    query_edges(ccd);
     last_bp = ccd ? 2 : 1;
     toc(ccd ? "broadphase_ccd" : "broadphase_dcd");
 }
@@ -747,9 +770,8 @@ void Solver::pcg()
     fold_stats();
 }
 
-void Solver::ccd()
+void Solver::ccd_ground()
 {
-    tic();
     LsArgs l{};
     l.nv        = S.nv;
     l.x_iter    = x_iter.p;
@@ -764,40 +786,78 @@ void Solver::ccd()
     l.seg       = seg_view(false);
     l.row_kind  = row_kind_d.p;
     launch_ground_ccd_maxdx(l, stream);
+}
+
+// One ACCD pass over the swept candidates of one kind (0 VF, 1 EE): the first pass runs every
+// pair with the iteration budget and lists those that need more; the rest pass runs the listed
+// pairs with the full budget (a small grid: grid-stride over the list).
+void Solver::ccd_pass(int kind, bool rest, hipStream_t s)
+{
+    CcdArgs c{};
+    c.x0            = x_iter.p;
+    c.x1            = x.p;
+    c.faces         = faces.p;
+    c.edges         = edges.p;
+    c.offset        = offset.p;
+    c.sst           = sst.p;
+    c.vert_scene    = vert_scene_d.p;
+    c.vert_body     = vert_body_d.p;
+    c.fp64          = prm.precision >= 1;
+    c.budget        = kCcdFirstPassIters;
+    c.rest          = rest ? 1 : 0;
+    c.kind          = kind;
+    c.pairs         = kind == 0 ? pairs_vf.p : pairs_ee.p;
+    c.pair_count    = kind == 0 ? &status.p->n_vf_ccd : &status.p->n_ee_ccd;
+    c.pair_capacity = kind == 0 ? cap_vf : cap_ee;
+    c.rest_list     = kind == 0 ? ccd_rest_vf.p : ccd_rest_ee.p;
+    c.rest_count    = ccd_rest_count.p + kind;
+    launch_ccd(c, rest ? std::min(1024u, pair_grid(c.pair_capacity)) : pair_grid(c.pair_capacity), s);
+}
+
+// CCD over the last swept broad phase (the "ccd" stage): ground bound, then each kind's passes.
+void Solver::ccd()
+{
+    tic();
+    ccd_ground();
     if (prm.use_ccd_linesearch && prm.use_self_collision)
     {
-        CcdArgs c{};
-        c.x0       = x_iter.p;
-        c.x1       = x.p;
-        c.faces    = faces.p;
-        c.edges    = edges.p;
-        c.offset   = offset.p;
-        c.sst      = sst.p;
-        c.vert_scene = vert_scene_d.p;
-        c.vert_body  = vert_body_d.p;
-        c.fp64     = prm.precision >= 1;
-        c.budget   = kCcdFirstPassIters;
         launch_fill_u32(ccd_rest_count.p, 2, 0u, stream);
-        for (int rest = 0; rest < 2; rest++)
-        {
-            c.rest          = rest;
-            c.kind          = 0;
-            c.pairs         = pairs_vf.p;
-            c.pair_count    = &status.p->n_vf_ccd;
-            c.pair_capacity = cap_vf;
-            c.rest_list     = ccd_rest_vf.p;
-            c.rest_count    = ccd_rest_count.p;
-            launch_ccd(c, pair_grid(cap_vf), stream);
-            c.kind          = 1;
-            c.pairs         = pairs_ee.p;
-            c.pair_count    = &status.p->n_ee_ccd;
-            c.pair_capacity = cap_ee;
-            c.rest_list     = ccd_rest_ee.p;
-            c.rest_count    = ccd_rest_count.p + 1;
-            launch_ccd(c, pair_grid(cap_ee), stream);
-        }
+        for (int kind = 0; kind < 2; kind++) ccd_pass(kind, false, stream);
+        for (int kind = 0; kind < 2; kind++) ccd_pass(kind, true, stream);
     }
     toc("ccd");
+}
+
+// The line search's swept broad phase and CCD, interleaved: the vertex-face query and first pass,
+// then the vertex-face rest pass on the side stream while the edge tree is refit and queried and
+// the edge-edge passes run; the stream waits for the side stream before the step is applied.
+// The rest pass is a handful of long ACCD loops (one lane each), which otherwise leave the GPU
+// idle; the step bound is a minimum over pairs, so the order of the passes does not change it.
+void Solver::swept_ccd()
+{
+    tic();
+    ccd_ground();
+    launch_fill_u32(ccd_rest_count.p, 2, 0u, stream);
+    toc_part("ccd");
+    tic();
+    query_faces(true);
+    toc_part("broadphase_ccd");
+    tic();
+    ccd_pass(0, false, stream);
+    LCSX_HIP(hipEventRecord(ev_fork, stream));
+    LCSX_HIP(hipStreamWaitEvent(side, ev_fork, 0));
+    ccd_pass(0, true, side);
+    LCSX_HIP(hipEventRecord(ev_join, side));
+    toc_part("ccd");
+    tic();
+    query_edges(true);
+    toc("broadphase_ccd");
+    tic();
+    ccd_pass(1, false, stream);
+    ccd_pass(1, true, stream);
+    LCSX_HIP(hipStreamWaitEvent(stream, ev_join, 0));
+    toc("ccd");
+    last_bp = 2;
 }
 
 bool Solver::grow_if_needed(bool ccd_only)
@@ -817,9 +877,9 @@ void Solver::line_search()
     launch_reset_line_search(status.p, sst.p, S.nscenes, stream);
     launch_apply_dx(S.nv, x_iter.p, dx.p, x.p, nullptr, 1.0f, g, stream);
     rigid_positions();
-    if (prm.use_ccd_linesearch && prm.use_self_collision) broad_phase(true);
     ccd_pairs_fresh = prm.use_ccd_linesearch && prm.use_self_collision;
-    ccd();
+    if (ccd_pairs_fresh) swept_ccd();
+    else ccd();
     launch_apply_dx(S.nv, x_iter.p, dx.p, x.p, sst.p, 1.0f, g, stream);
     rigid_positions();
     launch_pin_delta((uint32_t)S.fixed_verts.size(), fixed_ids.p, x.p, x_tilde.p, sst.p, vert_scene_d.p, stream);

@@ -18,6 +18,8 @@ struct Solver
     lcsx_params           prm{};
     int                   device = 0;
     hipStream_t           stream = nullptr;
+    hipStream_t           side   = nullptr;  // second stream (default priority) for overlapped passes
+    hipEvent_t            ev_fork = nullptr, ev_join = nullptr;
     std::vector<MeshDesc> meshes;
     Setup                 S;
     bool                  inited = false;
@@ -98,11 +100,16 @@ struct Solver
     void iter_begin();
     void material();
     void broad_phase(bool ccd);
+    void query_faces(bool ccd);  // vertex-triangle candidates (refit of the face tree + query)
+    void query_edges(bool ccd);  // edge-edge candidates
     void dcd(bool from_ccd_pairs = false);
     void update_contacts(bool reuse_ccd_pairs = false);
     void contact_assemble();
     void pcg();
     void ccd();
+    void ccd_ground();
+    void ccd_pass(int kind, bool rest, hipStream_t s);
+    void swept_ccd();  // the line search's broad phase + CCD with the VF rest pass on `side`
     void line_search();
     bool newton_iteration();  // returns false if it must be redone (buffer growth)
     void frame_end();
@@ -130,6 +137,7 @@ struct Solver
     // timing helpers
     void tic();
     void toc(const char* name);
+    void toc_part(const char* name);  // adds to the name's current call (closed by a toc)
     void       flush_events();
     hipEvent_t ev_get();
     void       ev_put(hipEvent_t e);
