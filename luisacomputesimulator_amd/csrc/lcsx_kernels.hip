@@ -969,6 +969,20 @@ __global__ __launch_bounds__(kB) void k_contact_t_pcg(PcgArgs a, ContactView c, 
     vc.tc[k]          = contact_t1(c, k, pf);
 }
 
+// One scene: one thread per contact (grid-stride), no slot chunks to walk. A chunk of 256 slots
+// holds about 64 contacts, so the chunk form above keeps three lanes in four idle.
+__global__ __launch_bounds__(kB) void k_contact_t_pcg_one(PcgArgs a, ContactView c, VertContacts vc)
+{
+    __shared__ float sh[4];
+    if (block_flag(&a.s[0].converged)) return;
+    IterScalars sc = iter_scalars(a, 0u, sh);
+    if (sc.stop) return;
+    const float    beta = sc.beta;
+    const uint32_t n    = min(*c.count, c.capacity);
+    auto           pf   = [&](uint32_t v) { return ld3(a.z, v) + beta * ld3(a.p_old, v); };
+    for (uint32_t k = blockIdx.x * kB + threadIdx.x; k < n; k += gridDim.x * kB) vc.tc[k] = contact_t1(c, k, pf);
+}
+
 __global__ __launch_bounds__(kB) void k_contact_t_plain(ContactView c, const float* __restrict__ p, float* __restrict__ tc)
 {
     contact_t(c, tc, [&](uint32_t v) { return ld3(p, v); });
@@ -1579,7 +1593,10 @@ void launch_pcg_init(const PcgArgs& a, hipStream_t s) { k_pcg_init<<<a.seg.n_rb,
 void launch_pcg_contact_t(const PcgArgs& a, const ContactView& c, const VertContacts& vc, unsigned max_chunks,
                           hipStream_t s)
 {
-    if (max_chunks) k_contact_t_pcg<<<max_chunks, kB, 0, s>>>(a, c, vc);
+    if (!max_chunks) return;
+    // contacts <= slots / 4 <= 64 max_chunks
+    if (a.seg.nscenes == 1) k_contact_t_pcg_one<<<max_chunks / 4u + 1u, kB, 0, s>>>(a, c, vc);
+    else k_contact_t_pcg<<<max_chunks, kB, 0, s>>>(a, c, vc);
 }
 void launch_pcg_spmv(const PcgArgs& a, const SellView& A, const VertContacts& vc, unsigned max_chunks, hipStream_t s,
                      hipEvent_t start, hipEvent_t stop)
