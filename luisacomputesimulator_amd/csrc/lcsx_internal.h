@@ -94,6 +94,9 @@ struct Lbvh
     std::vector<uint32_t> level_ptr_h;
     uint32_t              levels = 0, tail_level = 0;  // deepest level; deepest level refit in one workgroup
     bool                  use_levels = false;          // false: one-pass ticket refit (very tall trees)
+    // packed keys: largest element id under each child (false) or largest sorted leaf index
+    // (true, for sorted self-queries)
+    int                   sorted_keys = 0;
     uint32_t*         sorted_to_orig = nullptr;  // points into vals[?] after construct
 
     // seg0: nseg + 1 leaf offsets (null: one segment of all leaves)
@@ -137,6 +140,11 @@ struct QueryArgs
     uint32_t        capacity;
     uint32_t*       count;
     uint32_t*       overflow;  // set to 1 if a traversal stack overflowed (may be null)
+    // QueryEdges on a tree refit with sorted_keys: thread i queries edge order[i] (the sorted
+    // leaf order) and keeps the leaves after leaf i; valid when the pair test is symmetric
+    // (every vertex has the same offset and the same d_hat)
+    int             sorted_self;
+    const uint32_t* order;
 };
 void lbvh_query(const Lbvh& t, const QueryArgs& a, hipStream_t s);
 

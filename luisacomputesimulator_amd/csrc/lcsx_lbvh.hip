@@ -313,7 +313,7 @@ __global__ __launch_bounds__(kBlock) void k_refit(uint32_t n, uint32_t ni, int a
                                                    const float* __restrict__ x0, const float* __restrict__ x1,
                                                    const float* __restrict__ pad, const uint32_t* __restrict__ parent,
                                                    const uint2* __restrict__ children, NodeBox* nodes,
-                                                   float4* __restrict__ packed, uint32_t* counters)
+                                                   float4* __restrict__ packed, uint32_t* counters, int sorted_keys)
 {
     uint32_t l = blockIdx.x * kBlock + threadIdx.x;
     if (l >= n) return;
@@ -359,14 +359,24 @@ __global__ __launch_bounds__(kBlock) void k_refit(uint32_t n, uint32_t ni, int a
         // child codes (leaves keep (element, element) in children[]) and the largest element id
         // under each child (an inner child's record was published before its ticket)
         uint32_t ca, cb, ma, mb;
-        if (c.x >= ni) ca = 0x80000000u | (ma = children[c.x].x);
+        if (c.x >= ni)
+        {
+            ma = children[c.x].x;
+            ca = 0x80000000u | ma;
+            if (sorted_keys) ma = c.x - ni;
+        }
         else
         {
             ca = c.x;
             const float4* C = packed + 4 * (size_t)c.x;
             ma              = max(__float_as_uint(C[1].w), __float_as_uint(C[3].w));
         }
-        if (c.y >= ni) cb = 0x80000000u | (mb = children[c.y].x);
+        if (c.y >= ni)
+        {
+            mb = children[c.y].x;
+            cb = 0x80000000u | mb;
+            if (sorted_keys) mb = c.y - ni;
+        }
         else
         {
             cb = c.y;
@@ -491,8 +501,10 @@ LCSX_D NodeBox leaf_box(uint32_t l, int arity, const uint32_t* __restrict__ prim
 }
 
 // fn_refit_tree_aabb, lbvh.cpp:702-754, for inner node p whose children are final: its box, and
-// its packed record (child boxes, child codes, largest element id under each child).
-LCSX_D void inner_box(uint32_t p, uint32_t ni, const uint2* __restrict__ children, NodeBox* nodes, float4* packed)
+// its packed record (child boxes, child codes, and under each child the largest element id, or
+// with sorted_keys the largest sorted leaf index, i.e. the end of its leaf range).
+LCSX_D void inner_box(uint32_t p, uint32_t ni, const uint2* __restrict__ children, NodeBox* nodes, float4* packed,
+                      int sorted_keys)
 {
     uint2   c = children[p];
     NodeBox A = nodes[c.x], B = nodes[c.y];
@@ -501,14 +513,24 @@ LCSX_D void inner_box(uint32_t p, uint32_t ni, const uint2* __restrict__ childre
     U.hi     = make_float4(smax(A.hi.x, B.hi.x), smax(A.hi.y, B.hi.y), smax(A.hi.z, B.hi.z), 0.0f);
     nodes[p] = U;
     uint32_t ca, cb, ma, mb;
-    if (c.x >= ni) ca = 0x80000000u | (ma = children[c.x].x);
+    if (c.x >= ni)
+    {
+        ma = children[c.x].x;
+        ca = 0x80000000u | ma;
+        if (sorted_keys) ma = c.x - ni;
+    }
     else
     {
         ca              = c.x;
         const float4* C = packed + 4 * (size_t)c.x;
         ma              = max(__float_as_uint(C[1].w), __float_as_uint(C[3].w));
     }
-    if (c.y >= ni) cb = 0x80000000u | (mb = children[c.y].x);
+    if (c.y >= ni)
+    {
+        mb = children[c.y].x;
+        cb = 0x80000000u | mb;
+        if (sorted_keys) mb = c.y - ni;
+    }
     else
     {
         cb              = c.y;
@@ -534,21 +556,21 @@ __global__ __launch_bounds__(kBlock) void k_refit_leaves(uint32_t n, uint32_t ni
 
 __global__ __launch_bounds__(kBlock) void k_refit_level(const uint32_t* __restrict__ order, uint32_t b, uint32_t e,
                                                          uint32_t ni, const uint2* __restrict__ children, NodeBox* nodes,
-                                                         float4* packed)
+                                                         float4* packed, int sorted_keys)
 {
     uint32_t k = b + blockIdx.x * kBlock + threadIdx.x;
-    if (k < e) inner_box(order[k], ni, children, nodes, packed);
+    if (k < e) inner_box(order[k], ni, children, nodes, packed, sorted_keys);
 }
 
 // Depths h1 down to 0 in one workgroup, a barrier between levels.
 __global__ __launch_bounds__(1024) void k_refit_tail(const uint32_t* __restrict__ order, const uint32_t* __restrict__ level_ptr,
                                                       uint32_t h1, uint32_t ni, const uint2* __restrict__ children,
-                                                      NodeBox* nodes, float4* packed)
+                                                      NodeBox* nodes, float4* packed, int sorted_keys)
 {
     for (int h = (int)h1; h >= 0; h--)
     {
         for (uint32_t k = level_ptr[h] + threadIdx.x; k < level_ptr[h + 1]; k += 1024u)
-            inner_box(order[k], ni, children, nodes, packed);
+            inner_box(order[k], ni, children, nodes, packed, sorted_keys);
         __threadfence_block();
         __syncthreads();
     }
@@ -596,8 +618,13 @@ __global__ __launch_bounds__(kQueryBlock) void k_query(QueryArgs a, const float4
     __shared__ uint32_t stack[kStack][kQueryBlock];
     __shared__ uint2    buf[kQueryBlock / 64][kWaveBuf];
     const uint32_t      tid = threadIdx.x, wv = tid >> 6;
-    const uint32_t      q   = blockIdx.x * kQueryBlock + tid;
-    bool                active = q < a.n_queries;
+    const uint32_t      qi  = blockIdx.x * kQueryBlock + tid;
+    bool                active = qi < a.n_queries;
+    // sorted self-query (edges against their own tree, packed keys = sorted leaf indices): thread
+    // qi queries the edge at sorted leaf qi and keeps leaves after it in sorted order
+    const bool          self = a.mode == QueryEdges && a.sorted_self;
+    const uint32_t      q    = (self && active) ? a.order[qi] : qi;
+    const uint32_t      fk   = self ? qi : q;  // filter key, compared with the packed keys
     f3                  lo = mk(0.0f, 0.0f, 0.0f), hi = mk(0.0f, 0.0f, 0.0f);
     bool                filt = false;
     if (active)
@@ -669,7 +696,7 @@ __global__ __launch_bounds__(kQueryBlock) void k_query(QueryArgs a, const float4
                     NodeBox  B = nodes[nd];
                     uint32_t o = children[nd].x;
                     r0 = make_float4(B.lo.x, B.lo.y, B.lo.z, __uint_as_float(0x80000000u | o));
-                    r1 = make_float4(B.hi.x, B.hi.y, B.hi.z, __uint_as_float(o));
+                    r1 = make_float4(B.hi.x, B.hi.y, B.hi.z, __uint_as_float(self ? nd - n_inner : o));
                     const float inf = __int_as_float(0x7f800000);
                     r2 = make_float4(inf, inf, inf, __uint_as_float(0x80000000u));
                     r3 = make_float4(-inf, -inf, -inf, 0.0f);
@@ -677,8 +704,8 @@ __global__ __launch_bounds__(kQueryBlock) void k_query(QueryArgs a, const float4
                 uint32_t      c0 = __float_as_uint(r0.w), c1 = __float_as_uint(r2.w);
                 // with the edge filter (keep q < element) a subtree whose largest element id is
                 // <= q holds no pair: it is not descended (leaf: the largest id is the element)
-                bool h0 = overlap4(r0, r1, lo, hi) && (!filt || q < __float_as_uint(r1.w));
-                bool h1 = overlap4(r2, r3, lo, hi) && (!filt || q < __float_as_uint(r3.w));
+                bool h0 = overlap4(r0, r1, lo, hi) && (!filt || fk < __float_as_uint(r1.w));
+                bool h1 = overlap4(r2, r3, lo, hi) && (!filt || fk < __float_as_uint(r3.w));
                 bool l0 = (c0 & 0x80000000u) != 0u, l1 = (c1 & 0x80000000u) != 0u;
                 bool brk = false;
                 if (h0 && !l0)
@@ -713,8 +740,9 @@ __global__ __launch_bounds__(kQueryBlock) void k_query(QueryArgs a, const float4
         {
             uint64_t lt  = lanemask_lt();
             uint32_t off = nbuf + (uint32_t)(__popcll(b0 & lt) + __popcll(b1 & lt));
-            if (e0) buf[wv][off++] = make_uint2(q, o0);
-            if (e1) buf[wv][off] = make_uint2(q, o1);
+            // a sorted self-query stores each pair as (lower id, higher id), as the element-id filter does
+            if (e0) buf[wv][off++] = self ? make_uint2(min(q, o0), max(q, o0)) : make_uint2(q, o0);
+            if (e1) buf[wv][off] = self ? make_uint2(min(q, o1), max(q, o1)) : make_uint2(q, o1);
             nbuf += cnt;
             if (nbuf > kWaveBuf - 128u)
             {
@@ -890,15 +918,18 @@ void Lbvh::refit(const float* x0, const float* x1, const float* pad, hipStream_t
         {
             uint32_t b = level_ptr_h[h], e = level_ptr_h[h + 1];
             if (e > b)
-                k_refit_level<<<grid_for(e - b, kBlock), kBlock, 0, s>>>(order.p, b, e, ni, children.p, nodes.p, packed.p);
+                k_refit_level<<<grid_for(e - b, kBlock), kBlock, 0, s>>>(order.p, b, e, ni, children.p, nodes.p, packed.p,
+                                                                         sorted_keys);
         }
-        if (top >= 0) k_refit_tail<<<1, 1024, 0, s>>>(order.p, level_ptr.p, (uint32_t)top, ni, children.p, nodes.p, packed.p);
+        if (top >= 0)
+            k_refit_tail<<<1, 1024, 0, s>>>(order.p, level_ptr.p, (uint32_t)top, ni, children.p, nodes.p, packed.p,
+                                            sorted_keys);
         LCSX_HIP(hipGetLastError());
         return;
     }
     k_fill_u32<<<grid_for((uint32_t)counters.n, kBlock), kBlock, 0, s>>>(counters.p, (uint32_t)counters.n, 0u);
     k_refit<<<grid_for(n, kBlock), kBlock, 0, s>>>(n, ni, arity, prims, sorted_to_orig, x0, x1, pad, parent.p, children.p,
-                                                   nodes.p, packed.p, counters.p);
+                                                   nodes.p, packed.p, counters.p, sorted_keys);
     LCSX_HIP(hipGetLastError());
 }
 

@@ -216,6 +216,12 @@ void Solver::init()
     upload_state_constants();
     tree_face.init(3, faces.p, S.nf, S.sc_f0.data(), ns);
     tree_edge.init(2, edges.p, S.ne, S.sc_e0.data(), ns);
+    // one offset and one d_hat for every vertex: the edge-edge pair test is symmetric, and the
+    // edge tree is queried by itself in sorted leaf order (each pair found once, from its first leaf)
+    sorted_ee = true;
+    for (uint32_t i = 0; i < S.nv; i++)
+        if (S.row_kind[i] < 2 && (S.offset[i] != S.offset[0] || S.dhat[i] != S.dhat[0])) sorted_ee = false;
+    tree_edge.sorted_keys = sorted_ee ? 1 : 0;
     alloc_collision(24 * nv, 24 * S.ne, 12 * nv);
     vc_cnt.alloc(nv + 1);
     ccd_rest_count.alloc(2);
@@ -479,6 +485,8 @@ void Solver::query_edges(bool ccd)
     q.capacity   = cap_ee;
     q.count      = ccd ? &st->n_ee_ccd : &st->n_ee;
     q.overflow   = &st->overflow;
+    q.sorted_self = sorted_ee ? 1 : 0;
+    q.order       = tree_edge.sorted_to_orig;
     lbvh_query(tree_edge, q, stream);
 }
 

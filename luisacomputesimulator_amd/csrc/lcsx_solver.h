@@ -44,6 +44,7 @@ struct Solver
     DBuf<float4>   sf_Dm, sf_c, bh_c;
     DBuf<float>    face_stage, hinge_stage;
     Lbvh           tree_face, tree_edge;
+    bool           sorted_ee = false;  // edge-edge candidates by the sorted self-query
     // collision buffers
     DBuf<uint32_t> pairs_vf, pairs_ee;
     uint32_t       cap_vf = 0, cap_ee = 0, cap_c = 0;
