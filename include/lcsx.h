@@ -232,6 +232,14 @@ lcsx_status lcsx_solver_set_timing(lcsx_solver* s, int level);
  * launches are the default. Timing level 2 always launches directly (its events are attached to
  * the SpMV launches). */
 lcsx_status lcsx_solver_set_graphs(lcsx_solver* s, int on);
+/* Autotuner (no counterpart in the reference): times the runtime mappings on the current state
+ * and keeps the fastest of each: the CCD first-pass iteration budget (8..128), the PCG launch
+ * mode (direct or captured graphs) and, for a single scene, the contact t_c kernel form (one
+ * thread per contact or per 256-slot chunk). No choice changes a result. The state (positions,
+ * velocities, statistics, timings) is restored. report (may be NULL) receives
+ * "name[setting]=ms;...;chosen:..." (truncated to report_len). Best called after a few frames,
+ * on a state with contacts. */
+lcsx_status lcsx_solver_autotune(lcsx_solver* s, char* report, size_t report_len);
 
 /* ----------------------------------------------- standalone LBVH (device pointers)
  * LBVH over faces (arity 3) or edges (arity 2), lbvh.h:166-304. All pointers are device

@@ -89,6 +89,7 @@ _SIGS = {
     "lcsx_solver_kernel_times": (C.c_int, [C.c_void_p, C.c_char_p, C.c_size_t, C.c_int]),
     "lcsx_solver_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "lcsx_solver_set_graphs": (C.c_int, [C.c_void_p, C.c_int]),
+    "lcsx_solver_autotune": (C.c_int, [C.c_void_p, C.c_char_p, C.c_size_t]),
     "lcsx_lbvh_create": (C.c_int, [C.c_int, C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]),
     "lcsx_lbvh_destroy": (C.c_int, [C.c_void_p]),
     "lcsx_lbvh_construct": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
@@ -341,6 +342,20 @@ class NewtonSolver:
     def set_graphs(self, on: bool) -> None:
         """Replay captured hipGraphs of the PCG iterations, or launch each kernel (the default)."""
         _check(self.lib.lcsx_solver_set_graphs(self.h, int(on)))
+
+    def autotune(self) -> Dict[str, object]:
+        """Times the runtime mappings on the current state and keeps the fastest (results are
+        unchanged by any of them); returns the timings (ms) and the chosen settings."""
+        buf = C.create_string_buffer(4096)
+        _check(self.lib.lcsx_solver_autotune(self.h, buf, len(buf)))
+        out: Dict[str, object] = {}
+        for item in buf.value.decode().split(";"):
+            if item.startswith("chosen:"):
+                out["chosen"] = {k: int(v) for k, v in (kv.split("=") for kv in item[7:].split(","))}
+            elif "=" in item:
+                k, v = item.rsplit("=", 1)
+                out[k] = float(v)
+        return out
 
     def kernel_times(self, reset: bool = False) -> Dict[str, tuple]:
         buf = C.create_string_buffer(8192)

@@ -582,6 +582,20 @@ lcsx_status lcsx_solver_set_timing(lcsx_solver* s, int level)
     return LCSX_OK;
 }
 
+lcsx_status lcsx_solver_autotune(lcsx_solver* s, char* report, size_t report_len)
+{
+    if (!s) return arg_error("invalid arguments");
+    if (!s->s.inited) { set_error("autotune before init"); return LCSX_ERR_STATE; }
+    LCSX_GUARD(
+        const std::string r = s->s.autotune();
+        if (report && report_len)
+        {
+            std::strncpy(report, r.c_str(), report_len - 1);
+            report[report_len - 1] = 0;
+        }
+        return LCSX_OK;)
+}
+
 lcsx_status lcsx_solver_set_graphs(lcsx_solver* s, int on)
 {
     if (!s || (on != 0 && on != 1)) return arg_error("invalid arguments");

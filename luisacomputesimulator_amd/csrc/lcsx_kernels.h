@@ -271,7 +271,7 @@ void launch_pcg_init(const PcgArgs& a, hipStream_t s);
 // max_chunks: an upper bound of the contact-slot chunks (slots / 256 + scenes); blocks past the
 // actual count return at once.
 void launch_pcg_contact_t(const PcgArgs& a, const ContactView& c, const VertContacts& vc, unsigned max_chunks,
-                          hipStream_t s);
+                          bool per_contact, hipStream_t s);
 void launch_pcg_spmv(const PcgArgs& a, const SellView& A, const VertContacts& vc, unsigned max_chunks, hipStream_t s,
                      hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
 void launch_pcg_update(const PcgArgs& a, hipStream_t s);

@@ -1591,11 +1591,11 @@ void launch_contact_assemble(const ContactView& c, const VertContacts& vc, float
 }
 void launch_pcg_init(const PcgArgs& a, hipStream_t s) { k_pcg_init<<<a.seg.n_rb, kB, 0, s>>>(a); }
 void launch_pcg_contact_t(const PcgArgs& a, const ContactView& c, const VertContacts& vc, unsigned max_chunks,
-                          hipStream_t s)
+                          bool per_contact, hipStream_t s)
 {
     if (!max_chunks) return;
     // contacts <= slots / 4 <= 64 max_chunks
-    if (a.seg.nscenes == 1) k_contact_t_pcg_one<<<max_chunks / 4u + 1u, kB, 0, s>>>(a, c, vc);
+    if (per_contact && a.seg.nscenes == 1) k_contact_t_pcg_one<<<max_chunks / 4u + 1u, kB, 0, s>>>(a, c, vc);
     else k_contact_t_pcg<<<max_chunks, kB, 0, s>>>(a, c, vc);
 }
 void launch_pcg_spmv(const PcgArgs& a, const SellView& A, const VertContacts& vc, unsigned max_chunks, hipStream_t s,

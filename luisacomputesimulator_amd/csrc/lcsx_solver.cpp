@@ -13,9 +13,6 @@ namespace lcsx
 namespace
 {
 constexpr unsigned kPairGridMax = 4096;
-// ACCD iterations per pair in the first CCD pass; pairs that need more are run again in a
-// second pass over a compact list, so long iterations do not hold whole wavefronts
-constexpr int      kCcdFirstPassIters = 32;
 constexpr int      kPcgChunk    = 10;
 constexpr int      kMixedSweep  = 25;  // PCG iterations between residual replacements (mixed precision)
 
@@ -587,7 +584,7 @@ int Solver::pcg_run(PcgArgs& a, const SellView& A, const ContactView& cv, const 
                     e1 = ev_get();
                     spmv_events.push_back({e0, e1});
                 }
-                if (vc.ptr) launch_pcg_contact_t(b, cv, vc, mc, stream);
+                if (vc.ptr) launch_pcg_contact_t(b, cv, vc, mc, contact_t_per_contact, stream);
                 launch_pcg_spmv(b, A, vc, mc, stream, e0, e1);
                 LCSX_HIP(hipGetLastError());
                 launch_pcg_update(b, stream);
@@ -605,7 +602,8 @@ int Solver::pcg_run(PcgArgs& a, const SellView& A, const ContactView& cv, const 
                 while (g < mc) g <<= 1;
                 g = std::min(g, max_chunks(slots_per_contact * cap_c));
             }
-            const GraphKey key{graph_epoch, issued, chunk, par, g, vc.ptr != nullptr, A.val_h != nullptr};
+            const GraphKey key{graph_epoch, issued, chunk, par, g, vc.ptr != nullptr, A.val_h != nullptr,
+                               contact_t_per_contact};
             auto           it = pcg_graphs.find(key);
             if (it == pcg_graphs.end())
             {
@@ -622,8 +620,7 @@ int Solver::pcg_run(PcgArgs& a, const SellView& A, const ContactView& cv, const 
                 const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
                 (void)hipGraphDestroy(graph);
                 LCSX_HIP(e);
-                it = pcg_graphs.emplace(GraphKey{graph_epoch, issued, chunk, par, g, vc.ptr != nullptr, A.val_h != nullptr},
-                                        exec).first;
+                it = pcg_graphs.emplace(key, exec).first;
                 if (timing)
                 {
                     // host time of capture + instantiation ("graph_capture_host", not device time)
@@ -704,7 +701,7 @@ void Solver::enqueue_pcg_chunk(const PcgArgs& a, const SellView& A, const Contac
         b.p_old   = p[par].p;
         b.p_new   = p[1 - par].p;
         b.k       = (uint32_t)(k0 + k);
-        if (vc.ptr) launch_pcg_contact_t(b, cv, vc, mc, stream);
+        if (vc.ptr) launch_pcg_contact_t(b, cv, vc, mc, contact_t_per_contact, stream);
         launch_pcg_spmv(b, A, vc, mc, stream, nullptr, nullptr);
         launch_pcg_update(b, stream);
         par = 1 - par;
@@ -811,7 +808,7 @@ void Solver::ccd_pass(int kind, bool rest, hipStream_t s)
     c.vert_scene    = vert_scene_d.p;
     c.vert_body     = vert_body_d.p;
     c.fp64          = prm.precision >= 1;
-    c.budget        = kCcdFirstPassIters;
+    c.budget        = ccd_budget;
     c.rest          = rest ? 1 : 0;
     c.kind          = kind;
     c.pairs         = kind == 0 ? pairs_vf.p : pairs_ee.p;
@@ -1048,6 +1045,124 @@ void Solver::spmv_dx()
 {
     launch_spmv(sell_view(), contact_view(), vert_contacts(), pair_grid(cap_c), dx.p, spmv_tmp.p, S.nv, stream);
     LCSX_HIP(hipMemcpyAsync(dx.p, spmv_tmp.p, dx.n * sizeof(float), hipMemcpyDeviceToDevice, stream));
+}
+
+// Times the runtime mappings on the current state and keeps the fastest of each: the CCD
+// first-pass budget (iterations per pair before a pair moves to the full-budget pass), the PCG
+// launch mode (direct or captured graphs) and, with one scene, the form of the contact t_c
+// kernel (one thread per contact or per slot chunk). None of them changes a result: the step
+// bound is a minimum over the same pairs, and the PCG runs the same arithmetic. The state
+// (positions, velocities, statistics, timings) is restored afterwards.
+std::string Solver::autotune()
+{
+    DBuf<float> sx, sv;
+    sx.alloc(x.n);
+    sv.alloc(v.n);
+    LCSX_HIP(hipMemcpyAsync(sx.p, x.p, x.n * sizeof(float), hipMemcpyDeviceToDevice, stream));
+    LCSX_HIP(hipMemcpyAsync(sv.p, v.p, v.n * sizeof(float), hipMemcpyDeviceToDevice, stream));
+    const auto saved_times = times;
+    const auto saved_stats = stats;
+    const auto saved_scene = scene_stats;
+    const bool saved_timing = timing, saved_spmv = timing_spmv;
+    timing      = false;
+    timing_spmv = false;
+    hipEvent_t e0, e1;
+    LCSX_HIP(hipEventCreate(&e0));
+    LCSX_HIP(hipEventCreate(&e1));
+    auto time_ms = [&](auto&& fn, int reps) {
+        std::vector<float> t;
+        for (int r = 0; r < reps; r++)
+        {
+            LCSX_HIP(hipEventRecord(e0, stream));
+            fn();
+            LCSX_HIP(hipEventRecord(e1, stream));
+            LCSX_HIP(hipEventSynchronize(e1));
+            float ms = 0.0f;
+            LCSX_HIP(hipEventElapsedTime(&ms, e0, e1));
+            t.push_back(ms);
+        }
+        std::sort(t.begin(), t.end());
+        return t[t.size() / 2];
+    };
+    std::string out;
+    char        buf[160];
+    // a Newton system at the current state
+    active_h.assign(S.nscenes, 1);
+    active_d.upload(active_h, stream);
+    frame_begin();
+    predict();
+    iter_begin();
+    material();
+    launch_reset_status(status.p, sst.p, S.nscenes, stream);
+    update_contacts(false);
+    read_status();
+    if (grow_if_needed(false))
+    {
+        launch_reset_status(status.p, sst.p, S.nscenes, stream);
+        update_contacts(false);
+    }
+    contact_assemble();
+    // PCG: launch mode x contact t_c form
+    float best = 1e30f;
+    bool  bg = use_graphs, bc = contact_t_per_contact;
+    for (int graphs = 0; graphs < 2; graphs++)
+        for (int per_contact = 1; per_contact >= (S.nscenes == 1 ? 0 : 1); per_contact--)
+        {
+            use_graphs            = graphs != 0;
+            contact_t_per_contact = per_contact != 0;
+            pcg();  // first run: captures graphs, warms caches
+            const float ms = time_ms([&] { pcg(); }, 3);
+            std::snprintf(buf, sizeof(buf), "pcg[graphs=%d,per_contact=%d]=%.4f;", graphs, per_contact, ms);
+            out += buf;
+            if (ms < best)
+            {
+                best = ms;
+                bg   = use_graphs;
+                bc   = contact_t_per_contact;
+            }
+        }
+    use_graphs            = bg;
+    contact_t_per_contact = bc;
+    pcg();
+    // line search: CCD first-pass budget
+    if (prm.use_ccd_linesearch && prm.use_self_collision)
+    {
+        best        = 1e30f;
+        int bb      = ccd_budget;
+        for (int budget : {8, 16, 32, 64, 128})
+        {
+            ccd_budget = budget;
+            line_search();
+            read_status();
+            if (grow_if_needed(true)) line_search();
+            const float ms = time_ms([&] { line_search(); }, 3);
+            std::snprintf(buf, sizeof(buf), "line_search[ccd_budget=%d]=%.4f;", budget, ms);
+            out += buf;
+            if (ms < best)
+            {
+                best = ms;
+                bb   = budget;
+            }
+        }
+        ccd_budget = bb;
+    }
+    std::snprintf(buf, sizeof(buf), "chosen:ccd_budget=%d,graphs=%d,per_contact=%d", ccd_budget, use_graphs ? 1 : 0,
+                  contact_t_per_contact ? 1 : 0);
+    out += buf;
+    // restore the state
+    LCSX_HIP(hipMemcpyAsync(x.p, sx.p, x.n * sizeof(float), hipMemcpyDeviceToDevice, stream));
+    LCSX_HIP(hipMemcpyAsync(v.p, sv.p, v.n * sizeof(float), hipMemcpyDeviceToDevice, stream));
+    LCSX_HIP(hipStreamSynchronize(stream));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    ccd_pairs_fresh = false;
+    vc_ready        = false;
+    timing          = saved_timing;
+    timing_spmv     = saved_spmv;
+    times           = saved_times;
+    stats           = saved_stats;
+    scene_stats     = saved_scene;
+    return out;
 }
 
 }  // namespace lcsx

@@ -154,16 +154,22 @@ struct Solver
         uint64_t epoch;
         int      k0, len, par;
         unsigned grid;
-        int      contacts, lp;
+        int      contacts, lp, per_contact;
         bool     operator<(const GraphKey& o) const
         {
-            return std::tie(epoch, k0, len, par, grid, contacts, lp) <
-                   std::tie(o.epoch, o.k0, o.len, o.par, o.grid, o.contacts, o.lp);
+            return std::tie(epoch, k0, len, par, grid, contacts, lp, per_contact) <
+                   std::tie(o.epoch, o.k0, o.len, o.par, o.grid, o.contacts, o.lp, o.per_contact);
         }
     };
     std::map<GraphKey, hipGraphExec_t> pcg_graphs;
     uint64_t graph_epoch = 0;
     bool     use_graphs  = false;
+    // runtime mappings chosen by autotune() (defaults: the measured best on the 88K-vertex tube)
+    // ACCD iterations per pair in the first CCD pass; pairs that need more run again in a second
+    // pass over a compact list, so long iterations do not hold whole wavefronts
+    int         ccd_budget            = 32;
+    bool        contact_t_per_contact = true;  // one scene: t_c by one thread per contact
+    std::string autotune();
     void     clear_graphs();
     void     enqueue_pcg_chunk(const PcgArgs& a, const SellView& A, const ContactView& cv, const VertContacts& vc,
                                unsigned mc, int k0, int len, int par);

@@ -1,0 +1,47 @@
+"""The autotuner: it picks among runtime mappings that do not change results, and restores the
+solver's state, so a tuned solver's frames equal an untuned one's."""
+import numpy as np
+import pytest
+
+from luisacomputesimulator_amd import NewtonSolver, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def test_autotune_keeps_results_and_state(require_gpu):
+    """Square cloth without contact: after two frames, autotune on one of two identical solvers;
+    its state is unchanged and the next frames are bit-identical to the untuned solver's."""
+    a, b = NewtonSolver.from_scene(scenes.square_cloth(33)), NewtonSolver.from_scene(scenes.square_cloth(33))
+    for g in (a, b):
+        g.physics_step_gpu(2)
+    x0, v0 = a.get_simulation_results(), a.get_velocities()
+    rep = a.autotune()
+    print(f"[gpu-autotune] {rep}", flush=True)
+    assert set(rep["chosen"]) == {"ccd_budget", "graphs", "per_contact"}
+    assert rep["chosen"]["ccd_budget"] in (8, 16, 32, 64, 128)
+    assert np.array_equal(a.get_simulation_results(), x0) and np.array_equal(a.get_velocities(), v0)
+    for f in range(3):
+        a.physics_step_gpu(1)
+        b.physics_step_gpu(1)
+        assert a.stats()["pcg_iters_total"] == b.stats()["pcg_iters_total"]
+        assert np.array_equal(a.get_simulation_results(), b.get_simulation_results())
+
+
+def test_autotune_in_contact(require_gpu):
+    """Cloth just above a sphere (contacts from the first frame): the tuned solver tracks the
+    untuned one within 1e-4 relative per frame (contact slot order comes from atomics)."""
+    def make():
+        return NewtonSolver.from_scene(scenes.cloth_on_sphere(24, 0.02, -0.01, seed=4, sphere_subdiv=2, height=0.3015))
+
+    a, b = make(), make()
+    for g in (a, b):
+        g.physics_step_gpu(1)
+    rep = a.autotune()
+    print(f"[gpu-autotune] in contact: {rep}", flush=True)
+    for f in range(3):
+        a.physics_step_gpu(1)
+        b.physics_step_gpu(1)
+        xa, xb = a.get_simulation_results(), b.get_simulation_results()
+        err = np.abs(xa - xb).max() / np.abs(xb).max()
+        print(f"[gpu-autotune] frame {f}: rel diff {err:.3e}, contacts {a.stats()['num_contacts']}", flush=True)
+        assert err < 1e-4
