@@ -275,6 +275,7 @@ def main():
                     help="comma list of legs: tube (the headline), batch, coupling, twist_error (not run by "
                          "default: its FP64 oracle takes minutes)")
     ap.add_argument("--twist-error-n", type=int, default=65, help="twist_error leg: vertices per side")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default runtime mappings")
     ap.add_argument("--coupling-oracle-frames", type=int, default=24,
                     help="cloth-rigid leg: frames compared with the FP64 oracle")
     ap.add_argument("--profile-pcg", action="store_true",
@@ -317,6 +318,11 @@ def main():
             tw = time.perf_counter()
             g.physics_step_gpu(1)
             progress("warmup", f, g.stats(), time.perf_counter() - tw)
+        tuned = None
+        if not args.no_autotune:
+            # runtime mappings timed on the warm state (results do not depend on them)
+            tuned = g.autotune()["chosen"]
+            print(f"[bench] {tag} autotune: {tuned}", file=sys.stderr, flush=True)
         if args.profile_pcg:
             profile_pcg(g)
             return None
@@ -339,7 +345,7 @@ def main():
         spmv_ms, spmv_n = kt.get("spmv", (float("nan"), 0))
         slots = kt.get("spmv_slots", (0.0, 0))[0]
         nbytes = spmv_bytes(n_rows, n_blocks, slots, 23 if precision == "mixed" else 40)
-        return {"elapsed": elapsed, "newton": newton, "pcg": pcg, "kt": kt, "st": g.stats(), "n_rows": n_rows,
+        return {"elapsed": elapsed, "newton": newton, "pcg": pcg, "kt": kt, "st": g.stats(), "n_rows": n_rows, "tuned": tuned,
                 "n_blocks": n_blocks, "spmv_ms": spmv_ms, "spmv_n": spmv_n, "slots": slots, "nbytes": nbytes,
                 "achieved": nbytes / (spmv_ms * 1e-3) / 1e9 if spmv_n else None}
 
@@ -391,6 +397,7 @@ def main():
                      "contact_slots_per_launch": head["slots"],
                      "launches": head["spmv_n"]},
         "kernel_ms": {k: round(v[0], 4) for k, v in kt.items() if k != "spmv_slots"},
+        "autotuned": head["tuned"],
     }
     if "mixed" in res and precisions[0] != "mixed":
         m = res["mixed"]

@@ -235,7 +235,8 @@ lcsx_status lcsx_solver_set_graphs(lcsx_solver* s, int on);
 /* Autotuner (no counterpart in the reference): times the runtime mappings on the current state
  * and keeps the fastest of each: the CCD first-pass iteration budget (8..128), the PCG launch
  * mode (direct or captured graphs) and, for a single scene, the contact t_c kernel form (one
- * thread per contact or per 256-slot chunk). No choice changes a result. The state (positions,
+ * thread per contact or per 256-slot chunk); a setting changes only for a candidate more than 3%
+ * faster than the current one. No choice changes a result. The state (positions,
  * velocities, statistics, timings) is restored. report (may be NULL) receives
  * "name[setting]=ms;...;chosen:..." (truncated to report_len). Best called after a few frames,
  * on a state with contacts. */

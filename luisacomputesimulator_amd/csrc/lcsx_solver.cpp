@@ -1102,9 +1102,12 @@ std::string Solver::autotune()
         update_contacts(false);
     }
     contact_assemble();
-    // PCG: launch mode x contact t_c form
-    float best = 1e30f;
+    // PCG: launch mode x contact t_c form. A candidate replaces the current setting only when it
+    // is faster by more than kTuneMargin (timings repeat to within ~2%)
+    constexpr float kTuneMargin = 0.97f;
+    float best = 1e30f, cur = 1e30f;
     bool  bg = use_graphs, bc = contact_t_per_contact;
+    const bool g0 = use_graphs, c0 = contact_t_per_contact;
     for (int graphs = 0; graphs < 2; graphs++)
         for (int per_contact = 1; per_contact >= (S.nscenes == 1 ? 0 : 1); per_contact--)
         {
@@ -1114,6 +1117,7 @@ std::string Solver::autotune()
             const float ms = time_ms([&] { pcg(); }, 3);
             std::snprintf(buf, sizeof(buf), "pcg[graphs=%d,per_contact=%d]=%.4f;", graphs, per_contact, ms);
             out += buf;
+            if (use_graphs == g0 && contact_t_per_contact == c0) cur = ms;
             if (ms < best)
             {
                 best = ms;
@@ -1121,14 +1125,17 @@ std::string Solver::autotune()
                 bc   = contact_t_per_contact;
             }
         }
-    use_graphs            = bg;
-    contact_t_per_contact = bc;
+    const bool pcg_change = best < kTuneMargin * cur;
+    use_graphs            = pcg_change ? bg : g0;
+    contact_t_per_contact = pcg_change ? bc : c0;
     pcg();
     // line search: CCD first-pass budget
     if (prm.use_ccd_linesearch && prm.use_self_collision)
     {
-        best        = 1e30f;
-        int bb      = ccd_budget;
+        best         = 1e30f;
+        cur          = 1e30f;
+        const int b0 = ccd_budget;
+        int       bb = ccd_budget;
         for (int budget : {8, 16, 32, 64, 128})
         {
             ccd_budget = budget;
@@ -1138,13 +1145,14 @@ std::string Solver::autotune()
             const float ms = time_ms([&] { line_search(); }, 3);
             std::snprintf(buf, sizeof(buf), "line_search[ccd_budget=%d]=%.4f;", budget, ms);
             out += buf;
+            if (budget == b0) cur = ms;
             if (ms < best)
             {
                 best = ms;
                 bb   = budget;
             }
         }
-        ccd_budget = bb;
+        ccd_budget = best < kTuneMargin * cur ? bb : b0;
     }
     std::snprintf(buf, sizeof(buf), "chosen:ccd_budget=%d,graphs=%d,per_contact=%d", ccd_budget, use_graphs ? 1 : 0,
                   contact_t_per_contact ? 1 : 0);
