@@ -156,6 +156,19 @@ lcsx_status lcsx_solver_get_positions(lcsx_solver* s, float* out, size_t n_float
 lcsx_status lcsx_solver_get_velocities(lcsx_solver* s, float* out, size_t n_floats);
 /* SolverInterface state restore (load_state positions/velocities, solver_interface.cpp:291-412). */
 lcsx_status lcsx_solver_set_state(lcsx_solver* s, const float* x, const float* v, size_t n_floats);
+/* Checkpoints in the reference's ".state" text format (SolverInterface::
+ * save_current_frame_state_to_host / load_saved_state_from_host, solver_interface.cpp:206-410):
+ * "o position" and "o velocity" with one "v x y z" line per mesh vertex (registration order),
+ * then "o q" and "o qv" with four lines per affine body (p, a0, a1, a2). Values are written with
+ * 9 significant digits, so a save/load round trip is exact; files written by the reference (6
+ * digits) load too. load_state with frame >= 0 also sets the frame counter that drives the
+ * pinned-vertex animation; the file path is the caller's (the reference builds
+ * "<resources>/SimulationState/frame_<n>.state"). */
+lcsx_status lcsx_solver_save_state(lcsx_solver* s, const char* path);
+lcsx_status lcsx_solver_load_state(lcsx_solver* s, const char* path, int frame);
+/* The current positions as one OBJ file, an "o mesh_<i>" object per registered mesh with its
+ * triangles (SimMesh::saveToOBJ_combined, mesh_reader.cpp:681-750). */
+lcsx_status lcsx_solver_save_obj(lcsx_solver* s, const char* path);
 /* SolverInterface::update_pinned_verts_information (solver_interface.h:95-108): explicit targets
  * for the pinned vertices of one mesh (in selection order), replacing the animation. */
 lcsx_status lcsx_solver_set_pinned_targets(lcsx_solver* s, int mesh_id, const float* targets, int n_pinned);

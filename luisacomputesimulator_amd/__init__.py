@@ -89,6 +89,9 @@ _SIGS = {
     "lcsx_solver_kernel_times": (C.c_int, [C.c_void_p, C.c_char_p, C.c_size_t, C.c_int]),
     "lcsx_solver_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "lcsx_solver_set_graphs": (C.c_int, [C.c_void_p, C.c_int]),
+    "lcsx_solver_save_state": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "lcsx_solver_load_state": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int]),
+    "lcsx_solver_save_obj": (C.c_int, [C.c_void_p, C.c_char_p]),
     "lcsx_solver_autotune": (C.c_int, [C.c_void_p, C.c_char_p, C.c_size_t]),
     "lcsx_lbvh_create": (C.c_int, [C.c_int, C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]),
     "lcsx_lbvh_destroy": (C.c_int, [C.c_void_p]),
@@ -338,6 +341,18 @@ class NewtonSolver:
 
     def set_timing(self, level: int) -> None:
         _check(self.lib.lcsx_solver_set_timing(self.h, level))
+
+    def save_state(self, path: str) -> None:
+        """Writes the reference's ".state" checkpoint (positions, velocities, body q / qv)."""
+        _check(self.lib.lcsx_solver_save_state(self.h, os.fspath(path).encode()))
+
+    def load_state(self, path: str, frame: Optional[int] = None) -> None:
+        """Reads a ".state" checkpoint; with frame, also sets the frame counter."""
+        _check(self.lib.lcsx_solver_load_state(self.h, os.fspath(path).encode(), -1 if frame is None else int(frame)))
+
+    def save_obj(self, path: str) -> None:
+        """Writes the current positions as one OBJ with an object per mesh."""
+        _check(self.lib.lcsx_solver_save_obj(self.h, os.fspath(path).encode()))
 
     def set_graphs(self, on: bool) -> None:
         """Replay captured hipGraphs of the PCG iterations, or launch each kernel (the default)."""

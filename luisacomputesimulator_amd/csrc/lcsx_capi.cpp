@@ -1,5 +1,8 @@
 // extern "C" boundary of the lcsx library (declared in include/lcsx.h).
+#include <cstdio>
 #include <cstring>
+#include <fstream>
+#include <sstream>
 #include <exception>
 #include <string>
 #include "lcsx_io.h"
@@ -461,6 +464,152 @@ lcsx_status lcsx_solver_set_state(lcsx_solver* s, const float* x, const float* v
     LCSX_GUARD(if (x) s->s.x.upload(x, n, s->s.stream); if (v) s->s.v.upload(v, n, s->s.stream);
                LCSX_HIP(hipStreamSynchronize(s->s.stream)); return LCSX_OK;)
 }
+namespace
+{
+// Vertex rows (kind < 2) in row order, and the body rows (4 per body) in body order.
+void split_rows(const Solver& S, std::vector<uint32_t>& verts, std::vector<uint32_t>& body_rows)
+{
+    verts.clear();
+    body_rows.clear();
+    for (uint32_t i = 0; i < S.S.nv; i++)
+        if (S.S.row_kind[i] < 2) verts.push_back(i);
+    for (const auto& b : S.S.bodies)
+        for (uint32_t k = 0; k < 4; k++) body_rows.push_back(b.row0 + k);
+}
+}  // namespace
+
+// save_current_frame_state_to_host, solver_interface.cpp:206-276: sections "o position" and
+// "o velocity" (one "v x y z" line per mesh vertex), "o q" and "o qv" (four lines per affine body),
+// written with 9 significant digits so that a load restores the floats exactly.
+static lcsx_status save_state_impl(Solver& S, const char* path)
+{
+    std::vector<float> x(3 * (size_t)S.S.nv), v(x.size());
+    S.x.download(x.data(), x.size(), S.stream);
+    S.v.download(v.data(), v.size(), S.stream);
+    LCSX_HIP(hipStreamSynchronize(S.stream));
+    std::vector<uint32_t> verts, brows;
+    split_rows(S, verts, brows);
+    std::ofstream f(path);
+    if (!f) { set_error(std::string("cannot write ") + path); return LCSX_ERR_IO; }
+    char line[96];
+    auto put = [&](const char* head, const std::vector<float>& a, const std::vector<uint32_t>& rows) {
+        f << head << '\n';
+        for (uint32_t r : rows)
+        {
+            std::snprintf(line, sizeof(line), "v %.9g %.9g %.9g\n", a[3 * (size_t)r], a[3 * (size_t)r + 1], a[3 * (size_t)r + 2]);
+            f << line;
+        }
+    };
+    put("o position", x, verts);
+    put("o velocity", v, verts);
+    put("o q", x, brows);
+    put("o qv", v, brows);
+    if (!f) { set_error(std::string("write failed: ") + path); return LCSX_ERR_IO; }
+    return LCSX_OK;
+}
+
+lcsx_status lcsx_solver_save_state(lcsx_solver* s, const char* path)
+{
+    if (!s || !path) return arg_error("invalid arguments");
+    if (!s->s.inited) { set_error("save_state before init"); return LCSX_ERR_STATE; }
+    Solver& S = s->s;
+    LCSX_GUARD(return save_state_impl(S, path);)
+}
+
+// load_saved_state_from_host, solver_interface.cpp:277-410; "o qv" is matched before "o q"
+// (the reference tests "o q" first, so its q section also swallows the qv lines). Files with
+// fewer lines than the solver has rows leave the other rows as they are, as the reference does;
+// a section with more lines is an error. frame >= 0 also sets the frame counter (pinned targets).
+static lcsx_status load_state_impl(Solver& S, const char* path, int frame)
+{
+    std::ifstream f(path);
+    if (!f) { set_error(std::string("cannot read ") + path); return LCSX_ERR_IO; }
+    std::vector<float> x(3 * (size_t)S.S.nv), v(x.size());
+    S.x.download(x.data(), x.size(), S.stream);
+    S.v.download(v.data(), v.size(), S.stream);
+    LCSX_HIP(hipStreamSynchronize(S.stream));
+    std::vector<uint32_t> verts, brows;
+    split_rows(S, verts, brows);
+    int         sec = -1;
+    size_t      idx = 0;
+    std::string ln;
+    while (std::getline(f, ln))
+    {
+        if (ln.empty()) continue;
+        if (ln.rfind("o position", 0) == 0) { sec = 0; idx = 0; continue; }
+        if (ln.rfind("o velocity", 0) == 0) { sec = 1; idx = 0; continue; }
+        if (ln.rfind("o qv", 0) == 0) { sec = 3; idx = 0; continue; }
+        if (ln.rfind("o q", 0) == 0) { sec = 2; idx = 0; continue; }
+        if (ln[0] != 'v' || sec < 0) continue;
+        std::istringstream iss(ln.substr(1));
+        float a, b, c;
+        if (!(iss >> a >> b >> c)) { set_error("malformed line in " + std::string(path) + ": " + ln); return LCSX_ERR_IO; }
+        const std::vector<uint32_t>& rows = sec < 2 ? verts : brows;
+        if (idx >= rows.size()) { set_error("more lines than rows in a section of " + std::string(path)); return LCSX_ERR_IO; }
+        std::vector<float>& dst = (sec == 0 || sec == 2) ? x : v;
+        const size_t        r   = rows[idx++];
+        dst[3 * r] = a;
+        dst[3 * r + 1] = b;
+        dst[3 * r + 2] = c;
+    }
+    S.x.upload(x.data(), x.size(), S.stream);
+    S.v.upload(v.data(), v.size(), S.stream);
+    LCSX_HIP(hipStreamSynchronize(S.stream));
+    S.ccd_pairs_fresh = false;
+    if (frame >= 0) S.frame = frame;
+    return LCSX_OK;
+}
+
+lcsx_status lcsx_solver_load_state(lcsx_solver* s, const char* path, int frame)
+{
+    if (!s || !path) return arg_error("invalid arguments");
+    if (!s->s.inited) { set_error("load_state before init"); return LCSX_ERR_STATE; }
+    Solver& S = s->s;
+    LCSX_GUARD(return load_state_impl(S, path, frame);)
+}
+
+// saveToOBJ_combined, mesh_reader.cpp:681-750: one "o mesh_<i>" object per registered mesh with its
+// current vertex positions and its triangles, indices offset across meshes (1-based).
+static lcsx_status save_obj_impl(Solver& S, const char* path)
+{
+    std::vector<float> x(3 * (size_t)S.S.nv);
+    S.x.download(x.data(), x.size(), S.stream);
+    LCSX_HIP(hipStreamSynchronize(S.stream));
+    std::vector<uint32_t> verts, brows;
+    split_rows(S, verts, brows);
+    std::ofstream f(path);
+    if (!f) { set_error(std::string("cannot write ") + path); return LCSX_ERR_IO; }
+    f << "# Simulated by LuisaComputeSolver\n";
+    char   line[96];
+    size_t at = 0, prefix = 1;
+    for (size_t mi = 0; mi < S.meshes.size(); mi++)
+    {
+        const MeshDesc& m  = S.meshes[mi];
+        const size_t    mv = m.x.size() / 3;
+        f << "o mesh_" << mi << '\n';
+        for (size_t k = 0; k < mv; k++)
+        {
+            const size_t r = verts[at + k];
+            std::snprintf(line, sizeof(line), "v %.9g %.9g %.9g\n", x[3 * r], x[3 * r + 1], x[3 * r + 2]);
+            f << line;
+        }
+        for (size_t t = 0; t + 2 < m.tris.size(); t += 3)
+            f << "f " << prefix + m.tris[t] << ' ' << prefix + m.tris[t + 1] << ' ' << prefix + m.tris[t + 2] << '\n';
+        at += mv;
+        prefix += mv;
+    }
+    if (!f) { set_error(std::string("write failed: ") + path); return LCSX_ERR_IO; }
+    return LCSX_OK;
+}
+
+lcsx_status lcsx_solver_save_obj(lcsx_solver* s, const char* path)
+{
+    if (!s || !path) return arg_error("invalid arguments");
+    if (!s->s.inited) { set_error("save_obj before init"); return LCSX_ERR_STATE; }
+    Solver& S = s->s;
+    LCSX_GUARD(return save_obj_impl(S, path);)
+}
+
 lcsx_status lcsx_solver_set_pinned_targets(lcsx_solver* s, int mesh_id, const float* targets, int n_pinned)
 {
     if (!s || !s->s.inited || mesh_id < 0 || mesh_id >= (int)s->s.meshes.size() || !targets)
