@@ -109,6 +109,8 @@ class Params:
     damping_cloth: float = 2.0
     bending_scaling: float = 1.0
     newton_tol: float = 1e-2
+    # energy backtracking after the CCD bound (use_energy_linesearch, newton_solver.cpp:3803-3856)
+    use_energy_linesearch: bool = False
     # "fp32" (the reference's path), "mixed" (FP16 blocks + FP32 refinement, FP64 CCD and barrier)
     # or "mixed_bf16" (the same with BF16 blocks); not part of pack(): the oracle runs FP32 or its
     # FP64 twin
@@ -119,7 +121,7 @@ class Params:
                 self.pcg_check_every, self.pcg_rel_tol, self.pcg_abs_tol, *map(float, self.gravity),
                 float(self.use_floor), float(self.use_self_collision), float(self.use_ccd_linesearch),
                 self.floor_y, self.stiffness_collision, self.stiffness_dirichlet, self.damping_cloth,
-                self.bending_scaling, self.newton_tol]
+                self.bending_scaling, self.newton_tol, float(self.use_energy_linesearch)]
 
 
 @dataclasses.dataclass

@@ -13,7 +13,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liblcs_oracle.so")
 SOURCES = ["orc_capi.cpp"]
-HEADERS = ["orc_math.h", "orc_lbvh.h", "orc_collision.h", "orc_energy.h", "orc_scene.h"]
+HEADERS = ["orc_math.h", "orc_lbvh.h", "orc_collision.h", "orc_energy.h", "orc_abd.h", "orc_scene.h"]
 
 
 def build(force: bool = True) -> str:

@@ -4,6 +4,7 @@
 // cpu_baseline leg as the checker. The product never links or loads this library.
 #include <cstring>
 #include <string>
+#include <exception>
 #include "orc_scene.h"
 
 using namespace orc;
@@ -181,6 +182,7 @@ int orc_set_params(void* p, const double* a, int n)
     q.use_floor = (int)a[10]; q.use_self_collision = (int)a[11]; q.use_ccd = (int)a[12];
     q.floor_y = a[13]; q.kappa = a[14]; q.k_dirichlet = a[15]; q.damping = a[16];
     q.bending_scaling = a[17]; q.newton_tol = a[18];
+    q.use_energy_ls = n > 19 ? (int)a[19] : 0;
     Handle* h = (Handle*)p;
     h->s32.prm = q;
     h->s64.prm = q;
@@ -233,10 +235,32 @@ int orc_pin_method(void* p, int mesh, int method, double range, const double* an
     return (int)h->s32.meshes[mesh].pinned.size();
 }
 
+// -2: the stage failed (a non-finite energy or a failed energy line search)
 int orc_stage(void* p, const char* stage)
 {
     Handle* h = (Handle*)p;
-    return h->prec == 64 ? run_stage(h->s64, stage) : run_stage(h->s32, stage);
+    try
+    {
+        return h->prec == 64 ? run_stage(h->s64, stage) : run_stage(h->s32, stage);
+    }
+    catch (const std::exception&)
+    {
+        return -2;
+    }
+}
+
+// The total energy at the current positions (the energy line search's objective) and the two
+// energies of the last energy line search (at alpha = 0 and at the chosen alpha).
+int orc_energy(void* p, double* out3)
+{
+    Handle* h = (Handle*)p;
+    auto    f = [&](auto& s) {
+        out3[0] = s.energy();
+        out3[1] = s.ls_energy0;
+        out3[2] = s.ls_energy;
+    };
+    if (h->prec == 64) f(h->s64); else f(h->s32);
+    return 0;
 }
 
 // Writes element count and dtype code for a named array; returns 0 if found.

@@ -15,6 +15,7 @@
 //   Solver/SimulationSolver/newton_solver.cpp:3174-3427,3685-3884,4051-4345 driver and line search
 //   Solver/SimulationSolver/solver_interface.cpp:130-183 per-frame pre/post operations
 #pragma once
+#include <stdexcept>
 #include <map>
 #include <string>
 #include <vector>
@@ -54,6 +55,7 @@ struct Params
     double gravity[3] = {0, -9.8, 0};
     int    use_floor = 1, use_self_collision = 1, use_ccd = 1;
     double floor_y = 0.0, kappa = 1e7, k_dirichlet = 1e9, damping = 2.0, bending_scaling = 1.0, newton_tol = 1e-2;
+    int    use_energy_ls = 0;  // energy backtracking after the CCD bound (use_energy_linesearch)
 };
 
 template <class T>
@@ -123,7 +125,8 @@ struct Scene
         int      kind;  // -1 ground / none, 0 VF, 1 EE
         uint32_t a, b;  // (vertex, face) or (edge, edge)
     } ccd_arg{-1, 0, 0};
-    int pcg_iters = 0, newton_iters = 0, frame = 0;
+    int    pcg_iters = 0, newton_iters = 0, frame = 0;
+    double ls_energy0 = 0, ls_energy = 0;  // energies of the last energy line search (alpha 0, chosen alpha)
     T   pcg_rel = 0;
     bool converged = false, dirichlet_converged = false;
     std::vector<int> iters_log;
@@ -1079,7 +1082,75 @@ struct Scene
         rigid_positions();
     }
 
-    // newton_solver.cpp:3685-3884 (CCD line search, no energy backtracking)
+    // Total energy at the current x (solver_interface.cpp:499-873 for the elastic terms,
+    // narrow_phase.cpp:2964-3070 for the contacts of the iteration's contact list): each element
+    // in T, the sums in double. Terms: inertia of soft vertices (a pinned one times k_D),
+    // ground barrier (free vertices, rigid ones included), BW stretch + shear per stretch face,
+    // bending 0.5 k s h_bar dtheta^2 per hinge, affine-body inertia and orthogonality, and
+    // kappa a_bar b(d - thickness, d_hat) per listed contact within thickness + d_hat.
+    double energy() const
+    {
+        const T h = T(float(prm.dt)), h2inv = T(1) / (h * h), kD = T(float(prm.k_dirichlet));
+        double  e_in = 0, e_gr = 0, e_st = 0, e_be = 0, e_ab = 0, e_ct = 0;
+        for (uint32_t i = 0; i < nv; i++)
+        {
+            if (vert_body[i] != ~0u) continue;
+            T e = ((h2inv * dot(x[i] - x_tilde[i], x[i] - x_tilde[i])) * vert_mass[i]) / T(2);
+            if (is_fixed[i]) e = kD * e;
+            e_in += (double)e;
+        }
+        if (prm.use_floor)
+        {
+            const T kap = T(float(prm.kappa)), fy = T(float(prm.floor_y));
+            for (uint32_t i = 0; i < nv; i++)
+            {
+                if (is_fixed[i]) continue;
+                T dh = dhat[i], th = offset[i], dist = x[i].y - fy;
+                if (dist - th < dh) e_gr += (double)((kap * vert_area[i]) * barrier(dist - th, dh));
+            }
+        }
+        for (size_t f = 0; f < sf.size() / 3; f++)
+        {
+            const uint32_t* t = &sf[3 * f];
+            e_st += (double)bw_energy(x[t[0]], x[t[1]], x[t[2]], &sf_Dm[4 * f], sf_mu[f], sf_lambda[f], sf_area[f]);
+        }
+        const T scaling = T(float(prm.bending_scaling));
+        for (size_t e = 0; e < bh.size() / 4; e++)
+        {
+            const uint32_t* q = &bh[4 * e];
+            T               d = dihedral_angle(x[q[0]], x[q[1]], x[q[2]], x[q[3]]) - bh_angle[e];
+            e_be += (double)((((T(0.5f) * bh_k[e]) * scaling) * bh_hbar[e]) * d * d);
+        }
+        for (const Body& b : bodies)
+        {
+            V dq[4];
+            for (int k = 0; k < 4; k++) dq[k] = x[b.row0 + k] - x_tilde[b.row0 + k];
+            T e = 0;
+            for (int ii = 0; ii < 4; ii++)
+                for (int jj = 0; jj < 4; jj++) e = e + ((h2inv * dot(dq[ii], dq[jj])) * b.M[ii][jj]) / T(2);
+            if (b.fixed) e = e * kD;
+            T o = 0;
+            for (int ii = 0; ii < 3; ii++)
+                for (int jj = 0; jj < 3; jj++)
+                {
+                    T term = dot(x[b.row0 + 1 + ii], x[b.row0 + 1 + jj]) - (ii == jj ? T(1) : T(0));
+                    o      = o + term * term;
+                }
+            e_ab += (double)e + (double)(o * b.stiff);
+        }
+        for (const Contact& c : contacts)
+        {
+            V diff = ((c.w[0] * x[c.idx[0]] + c.w[1] * x[c.idx[1]]) + c.w[2] * x[c.idx[2]]) + c.w[3] * x[c.idx[3]];
+            T d    = std::sqrt(dot(diff, diff));
+            T dh   = T(0.5f) * (dhat[c.idx[0]] + dhat[c.idx[2]]);
+            T th   = offset[c.idx[0]] + offset[c.idx[2]];
+            if (d < th + dh) e_ct += (double)(c.area_kappa * barrier(d - th, dh));
+        }
+        return ((((e_in + e_gr) + e_st) + e_be) + e_ab) + e_ct;
+    }
+
+    // newton_solver.cpp:3685-3884: the CCD bound, then (use_energy_linesearch) halving from it
+    // while E(alpha) >= E(0) + Epsilon, at most 20 times (alpha below 1e-4 is a failure)
     void line_search()
     {
         apply_dx(T(1));
@@ -1100,6 +1171,24 @@ struct Scene
             return;
         }
         converged = false;
+        if (prm.use_energy_ls)
+        {
+            apply_dx(T(0));
+            const double e0 = energy();
+            apply_dx(alpha);
+            double e = energy();
+            if (!std::isfinite(e)) throw std::runtime_error("energy is not finite");
+            for (int c = 0; c < 20; c++)
+            {
+                if (e < e0 + (double)kAbdEpsilon) break;
+                alpha = alpha / T(2);
+                apply_dx(alpha);
+                e = energy();
+                if (alpha < T(1e-4f)) throw std::runtime_error("energy line search failed");
+            }
+            ls_energy0 = e0;
+            ls_energy  = e;
+        }
         apply_dx(alpha);
         pin_delta = T(-1e9f);
         for (uint32_t i = 0; i < nv; i++)

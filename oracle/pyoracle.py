@@ -34,6 +34,7 @@ def load() -> C.CDLL:
     lib.orc_set.argtypes = [vp, C.c_char_p, vp, C.c_int64]
     lib.orc_scalars.argtypes = [vp, dp, C.c_int]
     lib.orc_ccd_argmin.argtypes = [vp, C.POINTER(C.c_int64)]
+    lib.orc_energy.argtypes = [vp, dp]
     lib.orc_morton32.restype = C.c_uint32
     lib.orc_morton32.argtypes = [C.c_float, C.c_float, C.c_float]
     lib.orc_accd.restype = C.c_double
@@ -103,6 +104,8 @@ class Oracle:
 
     def stage(self, name: str) -> int:
         r = self.lib.orc_stage(self.h, name.encode())
+        if r == -2:
+            raise RuntimeError(f"oracle stage {name} failed (non-finite energy or failed energy line search)")
         if r < 0:
             raise ValueError(f"unknown oracle stage {name}")
         return r
@@ -130,6 +133,12 @@ class Oracle:
     def step(self, frames: int = 1) -> None:
         for _ in range(frames):
             self.stage("step")
+
+    def energy(self) -> tuple:
+        """(E at the current x, E(0) and E(alpha) of the last energy line search)."""
+        out = np.zeros(3, dtype=np.float64)
+        self.lib.orc_energy(self.h, out.ctypes.data_as(C.POINTER(C.c_double)))
+        return float(out[0]), float(out[1]), float(out[2])
 
     def ccd_argmin(self) -> tuple:
         """(kind, a, b) of the pair that bounded the last CCD: kind -1 none/ground, 0 VF, 1 EE."""
