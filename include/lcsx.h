@@ -96,6 +96,8 @@ typedef struct lcsx_params
                                    storage with per-block power-of-two scale in the PCG SpMV inside
                                    FP32 iterative refinement, FP64 CCD, distances and barrier;
                                    2: the same with BF16 block storage                          */
+    int   use_energy_linesearch; /* 1: after the CCD bound, halve the step (at most 20 times) while
+                                   the energy does not decrease (newton_solver.cpp:3803-3856)  */
 } lcsx_params;
 
 /* Scalars of the last Newton iteration / frame. */
@@ -213,7 +215,7 @@ lcsx_status lcsx_solver_get_scene_stats(lcsx_solver* s, int scene, lcsx_stats* o
  */
 lcsx_status lcsx_solver_run_stage(lcsx_solver* s, const char* stage);
 /* Named solver buffers, converted to a canonical host layout: float arrays are FP32, index
- * arrays uint32, morton keys uint64. Off-diagonal blocks are returned in the reference's CSR slot
+ * arrays uint32, morton keys uint64, energies double. Off-diagonal blocks are returned in the reference's CSR slot
  * order (row-major 3x3 per block). count_out receives the element count; with out == NULL only
  * the count is reported. Names:
  *   state      x, v, x_step, x_iter, x_tilde, dx, cgB, diag, offdiag, toi
@@ -227,6 +229,8 @@ lcsx_status lcsx_solver_run_stage(lcsx_solver* s, const char* stage);
  *              contact_k (k1, k2, kappa * area), contact_w, contact_bary, ccd_rest_vf /
  *              ccd_rest_ee (the last line search's candidate pairs whose ACCD ran past the first
  *              pass's iteration budget)
+ *   energy     the total energy of each scene at the current x over the current contact list
+ *              (double per scene; the energy line search's objective)
  * lcsx_solver_write accepts x, v, x_step, x_iter, x_tilde, dx and cgB. */
 lcsx_status lcsx_solver_read(lcsx_solver* s, const char* name, void* out, size_t bytes, size_t* count_out);
 lcsx_status lcsx_solver_write(lcsx_solver* s, const char* name, const void* in, size_t bytes);

@@ -47,7 +47,8 @@ class lcsx_params(C.Structure):
                 ("pcg_abs_tol", C.c_float), ("gravity", C.c_float * 3), ("use_floor", C.c_int),
                 ("use_self_collision", C.c_int), ("use_ccd_linesearch", C.c_int), ("floor_y", C.c_float),
                 ("stiffness_collision", C.c_float), ("stiffness_dirichlet", C.c_float), ("damping_cloth", C.c_float),
-                ("bending_scaling", C.c_float), ("newton_tol", C.c_float), ("precision", C.c_int)]
+                ("bending_scaling", C.c_float), ("newton_tol", C.c_float), ("precision", C.c_int),
+                ("use_energy_linesearch", C.c_int)]
 
 
 class lcsx_stats(C.Structure):
@@ -145,6 +146,7 @@ def params_to_c(p: Params) -> lcsx_params:
     c.bending_scaling = p.bending_scaling
     c.newton_tol = p.newton_tol
     c.precision = {"fp32": 0, "mixed": 1, "mixed_bf16": 2}[p.precision]
+    c.use_energy_linesearch = int(p.use_energy_linesearch)
     return c
 
 
@@ -168,12 +170,13 @@ def anim_to_c(a: Anim) -> lcsx_fixed_anim:
     return c
 
 
-_NP = {"f": np.float32, "u": np.uint32, "q": np.uint64, "b": np.uint8}
+_NP = {"f": np.float32, "u": np.uint32, "q": np.uint64, "b": np.uint8, "d": np.float64}
 # dtype of each readable buffer (see lcsx_solver_read)
 _DTYPES: Dict[str, str] = {n: "u" for n in ("faces", "edges", "hinges", "csr", "stretch_faces", "bending_hinges",
                                              "fixed_verts", "pairs_vf", "pairs_ee", "contact_idx", "contact_type",
                                              "ccd_rest_vf", "ccd_rest_ee")}
 _DTYPES["is_fixed"] = "b"
+_DTYPES["energy"] = "d"
 
 
 def _dtype_of(name: str):

@@ -212,6 +212,12 @@ bool read_named(Solver& S, const std::string& name, Blob& out)
         out.set(dl(vf ? S.pairs_vf : S.pairs_ee, 2 * (size_t)n, s));
         return true;
     }
+    if (name == "energy")
+    {
+        // total energy of each scene at the current x, over the current contact list (doubles)
+        out.set(S.energy());
+        return true;
+    }
     if (name == "ccd_rest_vf" || name == "ccd_rest_ee")
     {
         // the candidate pairs of the last line search whose ACCD ran past the first pass's budget
@@ -319,6 +325,7 @@ void lcsx_params_default(lcsx_params* p)
     p->bending_scaling      = 1.0f;
     p->newton_tol           = 1e-2f;
     p->precision            = 0;
+    p->use_energy_linesearch = 0;
 }
 void lcsx_material_default(lcsx_material* m)
 {

@@ -269,7 +269,12 @@ LCSX_HD double accd(const d3 a0[4], const d3 a1[4], double offset, int kind)
     return accd_budget(a0, a1, offset, kind, 0x7fffffff, &done);
 }
 
-// cipc_kernel.hpp:12-38 on d - thickness, in double
+// cipc_kernel.hpp:12-38 on d - thickness, in double: the barrier, its first and second derivatives
+LCSX_HD double barrier(double d, double dh)
+{
+    double dm = d - dh;
+    return -dm * dm * log(d / dh);
+}
 LCSX_HD double barrier_d1(double d, double dh) { return (dh - d) * ((2.0 * log(d / dh) - dh / d) + 1.0); }
 LCSX_HD double barrier_d2(double d, double dh)
 {

@@ -251,6 +251,7 @@ bool load_scene_json(const std::string& path, const std::string& mesh_dir, lcsx_
     get_bool("use_floor", prm.use_floor);
     get_bool("use_self_collision", prm.use_self_collision);
     get_bool("use_ccd_linesearch", prm.use_ccd_linesearch);
+    get_bool("use_energy_linesearch", prm.use_energy_linesearch);
     num(root.get("implicit_dt"), prm.implicit_dt);
     num(root.get("stiffness_collision"), prm.stiffness_collision);
     num(root.get("stiffness_dirichlet"), prm.stiffness_dirichlet);

@@ -166,6 +166,33 @@ struct CcdArgs
     const uint32_t* vert_body;   // per row: body id or ~0u; pairs within one body are skipped
 };
 
+struct BodyDev;
+// Energy line search: the total energy per scene (sums in double).
+struct EnergyArgs
+{
+    uint32_t        nv;  // rows
+    const float*    x;
+    const float*    x_tilde;
+    const float*    mass;
+    const uint8_t*  fixed;
+    const uint8_t*  row_kind;
+    const uint32_t* vert_scene;
+    const float*    offset;
+    const float*    dhat;
+    const float*    vert_area;
+    float           h2inv, kD, kappa, floor_y, scaling;
+    int             use_floor, fp64;
+    uint32_t        nsf, nbh, nb;
+    const uint32_t* sf;
+    const float4*   sf_Dm;
+    const float4*   sf_c;   // area, mu, lambda
+    const uint32_t* bh;
+    const float4*   bh_c;   // rest angle, h_bar, stiffness
+    const BodyDev*  bodies;
+    ContactView     c;
+    double*         out;    // per scene, accumulated
+};
+
 struct LsArgs
 {
     uint32_t       nv;
@@ -276,6 +303,7 @@ void launch_pcg_spmv(const PcgArgs& a, const SellView& A, const VertContacts& vc
                      hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
 void launch_pcg_update(const PcgArgs& a, hipStream_t s);
 // Mixed precision: FP16 copy of the off-diagonal blocks with a power-of-two scale per block.
+void launch_energy(const EnergyArgs& a, unsigned contact_grid, uint32_t nscenes, hipStream_t s);
 void launch_pack_fp16(uint32_t nslices, uint32_t n, const uint32_t* sl_off, uint32_t nblocks, const float* val,
                       const uint8_t* row_kind, __half* val_h, int8_t* val_e, int bf16, hipStream_t s);
 // Residual replacement: r = b - y, z = Minv r, per-block r.z with TwoSum compensation into

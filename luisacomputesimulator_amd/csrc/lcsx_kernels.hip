@@ -1099,7 +1099,7 @@ __global__ __launch_bounds__(kB) void k_spmv_pcg(PcgArgs a, SellView A, VertCont
                 if (ex != kKeepFp32)
                 {
                     // FP16 / BF16 entries times the block's power of two (exact)
-                    float sc = __int_as_float((127 + (int)ex) << 23);
+                    float sc = __uint_as_float((uint32_t)(127 + (int)ex) << 23);
                     if (LP == 2)
                     {
                         const uint16_t* vb = reinterpret_cast<const uint16_t*>(A.val_h);
@@ -1139,6 +1139,157 @@ __global__ __launch_bounds__(kB) void k_spmv_pcg(PcgArgs a, SellView A, VertCont
     float bs = block_sum(loc, sh);
     if (threadIdx.x == 0) a.part_pq[blockIdx.x] = bs;
 }
+
+__global__ void k_fill_f64(double* a, uint32_t n)
+{
+    uint32_t i = blockIdx.x * kB + threadIdx.x;
+    if (i < n) a[i] = 0.0;
+}
+
+// ---------------------------------------------------------------- energy (line search)
+// solver_interface.cpp:499-873 (inertia, ground, stretch face, bending, affine-body inertia and
+// orthogonality) and narrow_phase.cpp:2964-3070 (the iteration's contact list at the current x).
+// Each element in FP32 as the reference (the barrier terms in FP64 in the mixed form), the sums
+// in double: per block when the block's elements are of one scene, else per element.
+LCSX_D void scene_add(double e, uint32_t scene, double* __restrict__ out)
+{
+    __shared__ uint32_t s0;
+    __shared__ double   sd[kB / 64];
+    if (threadIdx.x == 0) s0 = kInvalid;
+    __syncthreads();
+    if (scene != kInvalid) s0 = scene;  // any one of the block's scenes
+    __syncthreads();
+    const uint32_t g = s0;
+    if (__syncthreads_and(scene == kInvalid || scene == g))
+    {
+        for (int off = 32; off > 0; off >>= 1) e += __shfl_xor(e, off);
+        if (lane_id() == 0) sd[threadIdx.x >> 6] = e;
+        __syncthreads();
+        if (threadIdx.x == 0 && g != kInvalid) atomicAdd(out + g, ((sd[0] + sd[1]) + sd[2]) + sd[3]);
+    }
+    else if (scene != kInvalid && e != 0.0)
+        atomicAdd(out + scene, e);
+}
+
+__global__ __launch_bounds__(kB) void k_energy_rows(EnergyArgs a)
+{
+    const uint32_t i  = blockIdx.x * kB + threadIdx.x;
+    double         e  = 0.0;
+    uint32_t       sc = kInvalid;
+    if (i < a.nv && a.row_kind[i] < 2u)
+    {
+        sc = a.vert_scene[i];
+        const f3 xi = ld3(a.x, i);
+        if (a.row_kind[i] == 0u)
+        {
+            f3    d  = xi - ld3(a.x_tilde, i);
+            float ei = ((a.h2inv * dot(d, d)) * a.mass[i]) / 2.0f;
+            if (a.fixed[i]) ei = a.kD * ei;
+            e += (double)ei;
+        }
+        if (a.use_floor && !a.fixed[i])
+        {
+            const float dist = xi.y - a.floor_y, th = a.offset[i], dh = a.dhat[i];
+            if (dist - th < dh)
+            {
+                const float stiff = a.kappa * a.vert_area[i];
+                e += a.fp64 ? (double)stiff * fp64::barrier((double)dist - (double)th, (double)dh)
+                            : (double)(stiff * barrier_e(dist - th, dh));
+            }
+        }
+    }
+    scene_add(e, sc, a.out);
+}
+
+__global__ __launch_bounds__(kB) void k_energy_faces(EnergyArgs a)
+{
+    const uint32_t f  = blockIdx.x * kB + threadIdx.x;
+    double         e  = 0.0;
+    uint32_t       sc = kInvalid;
+    if (f < a.nsf)
+    {
+        const uint32_t v0 = a.sf[3 * f], v1 = a.sf[3 * f + 1], v2 = a.sf[3 * f + 2];
+        sc                = a.vert_scene[v0];
+        const float4 D = a.sf_Dm[f], c = a.sf_c[f];
+        const f3     e0 = ld3(a.x, v1) - ld3(a.x, v0), e1 = ld3(a.x, v2) - ld3(a.x, v0);
+        const f3     Fu = e0 * D.x + e1 * D.y, Fv = e0 * D.z + e1 * D.w;
+        const float  su = sqrtf(dot(Fu, Fu)) - 1.0f, sv = sqrtf(dot(Fv, Fv)) - 1.0f, i6 = dot(Fu, Fv);
+        e = (double)(c.x * (0.5f * c.y * (su * su + sv * sv) + 0.5f * c.z * (i6 * i6)));
+    }
+    scene_add(e, sc, a.out);
+}
+
+__global__ __launch_bounds__(kB) void k_energy_hinges(EnergyArgs a)
+{
+    const uint32_t h  = blockIdx.x * kB + threadIdx.x;
+    double         e  = 0.0;
+    uint32_t       sc = kInvalid;
+    if (h < a.nbh)
+    {
+        const uint32_t* q = a.bh + 4 * (size_t)h;
+        sc                = a.vert_scene[q[0]];
+        const float4 c    = a.bh_c[h];
+        const float  d    = dihedral_angle(ld3(a.x, q[0]), ld3(a.x, q[1]), ld3(a.x, q[2]), ld3(a.x, q[3])) - c.x;
+        e                 = (double)(((((0.5f * c.z) * a.scaling) * c.y) * d) * d);
+    }
+    scene_add(e, sc, a.out);
+}
+
+__global__ __launch_bounds__(kB) void k_energy_bodies(EnergyArgs a)
+{
+    const uint32_t b  = blockIdx.x * kB + threadIdx.x;
+    double         e  = 0.0;
+    uint32_t       sc = kInvalid;
+    if (b < a.nb)
+    {
+        const BodyDev& B = a.bodies[b];
+        sc               = a.vert_scene[B.row0];
+        f3 dq[4], A[3];
+        for (int k = 0; k < 4; k++) dq[k] = ld3(a.x, B.row0 + k) - ld3(a.x_tilde, B.row0 + k);
+        for (int k = 0; k < 3; k++) A[k] = ld3(a.x, B.row0 + 1 + k);
+        float ei = 0.0f;
+        for (int ii = 0; ii < 4; ii++)
+            for (int jj = 0; jj < 4; jj++) ei = ei + ((a.h2inv * dot(dq[ii], dq[jj])) * B.M[4 * ii + jj]) / 2.0f;
+        if (B.fixed) ei = ei * a.kD;
+        float o = 0.0f;
+        for (int ii = 0; ii < 3; ii++)
+            for (int jj = 0; jj < 3; jj++)
+            {
+                float t = dot(A[ii], A[jj]) - (ii == jj ? 1.0f : 0.0f);
+                o       = o + t * t;
+            }
+        e = (double)ei + (double)(o * B.stiff);
+    }
+    scene_add(e, sc, a.out);
+}
+
+__global__ __launch_bounds__(kB) void k_energy_contacts(EnergyArgs a)
+{
+    const uint32_t n = min(*a.c.count, a.c.capacity);
+    for (uint32_t base = blockIdx.x * kB; base < n; base += gridDim.x * kB)
+    {
+        const uint32_t k  = base + threadIdx.x;
+        double         e  = 0.0;
+        uint32_t       sc = kInvalid;
+        if (k < n)
+        {
+            const uint4  I  = a.c.idx[k];
+            const float4 w  = a.c.w[k];
+            const float  ak = a.c.aux[k].y;
+            const uint32_t i0 = I.x & 0x7fffffffu, i1 = I.y & 0x7fffffffu, i2 = I.z & 0x7fffffffu, i3 = I.w & 0x7fffffffu;
+            sc                = a.vert_scene[i0];
+            const f3    diff = ((w.x * ld3(a.x, i0) + w.y * ld3(a.x, i1)) + w.z * ld3(a.x, i2)) + w.w * ld3(a.x, i3);
+            const float d    = sqrtf(dot(diff, diff));
+            const float dh   = 0.5f * (a.dhat[i0] + a.dhat[i2]);
+            const float th   = a.offset[i0] + a.offset[i2];
+            if (d < th + dh)
+                e = a.fp64 ? (double)ak * fp64::barrier((double)d - (double)th, (double)dh)
+                           : (double)(ak * barrier_e(d - th, dh));
+        }
+        scene_add(e, sc, a.out);
+    }
+}
+
 
 // BF16 of a finite float, rounded to nearest even (the upper 16 bits of the rounded float).
 LCSX_D uint16_t float_to_bf16_rn(float x)
@@ -1620,6 +1771,15 @@ void launch_pcg_spmv(const PcgArgs& a, const SellView& A, const VertContacts& vc
         hipExtLaunchKernelGGL(k_spmv_pcg<0>, dim3(g), dim3(kB), 0, s, start, stop, 0, a, A, vc);
     else
         k_spmv_pcg<0><<<g, kB, 0, s>>>(a, A, vc);
+}
+void launch_energy(const EnergyArgs& a, unsigned contact_grid, uint32_t nscenes, hipStream_t s)
+{
+    k_fill_f64<<<grid_for(nscenes, kB), kB, 0, s>>>(a.out, nscenes);
+    if (a.nv) k_energy_rows<<<grid_for(a.nv, kB), kB, 0, s>>>(a);
+    if (a.nsf) k_energy_faces<<<grid_for(a.nsf, kB), kB, 0, s>>>(a);
+    if (a.nbh) k_energy_hinges<<<grid_for(a.nbh, kB), kB, 0, s>>>(a);
+    if (a.nb) k_energy_bodies<<<grid_for(a.nb, kB), kB, 0, s>>>(a);
+    if (a.c.count && contact_grid) k_energy_contacts<<<contact_grid, kB, 0, s>>>(a);
 }
 void launch_pack_fp16(uint32_t nslices, uint32_t n, const uint32_t* sl_off, uint32_t nblocks, const float* val,
                       const uint8_t* row_kind, __half* val_h, int8_t* val_e, int bf16, hipStream_t s)

@@ -14,6 +14,7 @@ namespace
 {
 constexpr unsigned kPairGridMax = 4096;
 constexpr int      kPcgChunk    = 10;
+constexpr float    kFltEpsilon  = 1.19209290e-7f;  // Epsilon of the energy test (constant_value.h:12)
 constexpr int      kMixedSweep  = 25;  // PCG iterations between residual replacements (mixed precision)
 
 struct EventPool
@@ -198,6 +199,7 @@ void Solver::init()
     active_h.assign(ns, 1);
     active_d.upload(active_h, stream);
     sst.alloc(ns);
+    energy_d.alloc(ns);
     host_sst.assign(ns, SceneStatus{});
     host_pcg.assign(ns, PcgState{});
     scene_stats.assign(ns, lcsx_stats{});
@@ -887,7 +889,116 @@ void Solver::line_search()
     else ccd();
     launch_apply_dx(S.nv, x_iter.p, dx.p, x.p, sst.p, 1.0f, g, stream);
     rigid_positions();
+    if (prm.use_energy_linesearch) energy_line_search();
     launch_pin_delta((uint32_t)S.fixed_verts.size(), fixed_ids.p, x.p, x_tilde.p, sst.p, vert_scene_d.p, stream);
+}
+
+// Total energy of every scene at the current x (element energies in FP32, barrier terms in FP64
+// in the mixed form, sums in double).
+std::vector<double> Solver::energy()
+{
+    EnergyArgs a{};
+    a.nv         = S.nv;
+    a.x          = x.p;
+    a.x_tilde    = x_tilde.p;
+    a.mass       = vert_mass.p;
+    a.fixed      = is_fixed.p;
+    a.row_kind   = row_kind_d.p;
+    a.vert_scene = vert_scene_d.p;
+    a.offset     = offset.p;
+    a.dhat       = dhat.p;
+    a.vert_area  = vert_area.p;
+    const float h = prm.implicit_dt;
+    a.h2inv      = 1.0f / (h * h);
+    a.kD         = prm.stiffness_dirichlet;
+    a.kappa      = prm.stiffness_collision;
+    a.floor_y    = prm.floor_y;
+    a.scaling    = prm.bending_scaling;
+    a.use_floor  = prm.use_floor;
+    a.fp64       = prm.precision >= 1;
+    a.nsf        = (uint32_t)(S.sf.size() / 3);
+    a.sf         = sf.p;
+    a.sf_Dm      = sf_Dm.p;
+    a.sf_c       = sf_c.p;
+    a.nbh        = (uint32_t)(S.bh.size() / 4);
+    a.bh         = bh.p;
+    a.bh_c       = bh_c.p;
+    a.nb         = (uint32_t)S.bodies.size();
+    a.bodies     = bodies_d.p;
+    if (prm.use_self_collision) a.c = contact_view();
+    a.out = energy_d.p;
+    launch_energy(a, prm.use_self_collision ? pair_grid(cap_c) : 0u, S.nscenes, stream);
+    std::vector<double> e(S.nscenes);
+    LCSX_HIP(hipMemcpyAsync(e.data(), energy_d.p, e.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+    LCSX_HIP(hipStreamSynchronize(stream));
+    return e;
+}
+
+// newton_solver.cpp:3803-3856, per scene: scenes still iterating whose |dx| is not below the
+// convergence tolerance start from the CCD bound and halve their step while E(alpha) >= E(0) +
+// Epsilon, at most 20 times; a step below 1e-4 is a failure (the reference stops with an error).
+void Solver::energy_line_search()
+{
+    tic();
+    read_status();
+    const uint32_t       ns  = S.nscenes;
+    const float          tol = prm.newton_tol * prm.implicit_dt;
+    std::vector<float>   alpha(ns);
+    std::vector<uint8_t> search(ns, 0);
+    bool                 any = false;
+    for (uint32_t s = 0; s < ns; s++)
+    {
+        alpha[s]  = ord2f(host_sst[s].toi_bits);
+        search[s] = (active_h.empty() || active_h[s]) &&
+                    !(__builtin_bit_cast(float, host_sst[s].maxdx_bits) < tol);
+        any       = any || search[s];
+    }
+    if (!any)
+    {
+        toc("energy_line_search");
+        return;
+    }
+    const SegView g   = seg_view(false);
+    auto          put = [&](const std::vector<float>& al) {
+        for (uint32_t s = 0; s < ns; s++) host_sst[s].toi_bits = f2ord(al[s]);
+        sst.upload(host_sst, stream);
+        launch_apply_dx(S.nv, x_iter.p, dx.p, x.p, sst.p, 1.0f, g, stream);
+        rigid_positions();
+    };
+    std::vector<float> zero = alpha;
+    for (uint32_t s = 0; s < ns; s++)
+        if (search[s]) zero[s] = 0.0f;
+    put(zero);
+    const std::vector<double> e0 = energy();
+    put(alpha);
+    std::vector<double> e = energy();
+    for (uint32_t s = 0; s < ns; s++)
+        if (search[s] && !std::isfinite(e[s]))
+        {
+            set_error("energy is not finite in scene " + std::to_string(s));
+            throw HipFailure();
+        }
+    for (int c = 0; c < 20; c++)
+    {
+        bool more = false;
+        for (uint32_t s = 0; s < ns; s++)
+        {
+            if (!search[s]) continue;
+            if (e[s] < e0[s] + (double)kFltEpsilon) { search[s] = 0; continue; }
+            alpha[s] *= 0.5f;
+            more = true;
+        }
+        if (!more) break;
+        put(alpha);
+        e = energy();
+        for (uint32_t s = 0; s < ns; s++)
+            if (search[s] && alpha[s] < 1e-4f)
+            {
+                set_error("energy line search failed in scene " + std::to_string(s));
+                throw HipFailure();
+            }
+    }
+    toc("energy_line_search");
 }
 
 bool Solver::newton_iteration()

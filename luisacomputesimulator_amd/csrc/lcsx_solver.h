@@ -112,6 +112,9 @@ struct Solver
     void ccd_pass(int kind, bool rest, hipStream_t s);
     void swept_ccd();  // the line search's broad phase + CCD with the VF rest pass on `side`
     void line_search();
+    std::vector<double> energy();  // per scene
+    void                energy_line_search();
+    DBuf<double>        energy_d;
     bool newton_iteration();  // returns false if it must be redone (buffer growth)
     void frame_end();
     void step();

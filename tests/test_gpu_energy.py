@@ -1,0 +1,111 @@
+"""GPU energy (the energy line search's objective) and the energy line search, against the oracle."""
+import numpy as np
+import pytest
+
+import oracle
+from luisacomputesimulator_amd import NewtonSolver, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def staged(sc, prec=32):
+    g, o = NewtonSolver.from_scene(sc), oracle.Oracle(sc, prec)
+    for s in ("frame_begin", "predict", "iter_begin", "material", "update_contacts", "contact_assemble"):
+        g.stage(s)
+        o.stage(s)
+    return g, o
+
+
+@pytest.mark.parametrize("name", ["square_floor", "two_sheets", "bodies"])
+def test_energy_matches_oracle(require_gpu, name):
+    """Same state, same contact list: the GPU's total energy equals the FP32 oracle's to 1e-5
+    relative (element energies in FP32 in both, summed in double in another order)."""
+    if name == "square_floor":
+        sc = scenes.square_cloth(17)
+        sc.meshes[0].verts[:, 1] = np.float32(1.4e-3)
+    elif name == "two_sheets":
+        sc = scenes.small_two_sheets(9, gap=1.2e-3)
+    else:
+        sc = scenes.cloth_rigid_coupling(n=9, boxes=2, seed=3, box_region=((-0.25, 0.25), (0.32, 0.5), (-0.25, 0.25)))
+    g, o = staged(sc)
+    rng = np.random.default_rng(5)
+    x = o.get("x").reshape(-1, 3)
+    # perturbations below the sheets' clearance (a pair inside its offsets has no barrier energy)
+    x = x + ((1e-5 if name == "two_sheets" else 2e-4) * rng.standard_normal(x.shape)).astype(np.float32)
+    g.write("x", x)
+    o.set("x", x)
+    eg = float(g.read("energy")[0])
+    eo = o.energy()[0]
+    print(f"[gpu-energy] {name}: gpu {eg:.9e} oracle {eo:.9e} contacts {len(o.get('contact_idx')) // 4}", flush=True)
+    assert eo > 0
+    assert abs(eg - eo) <= 1e-5 * abs(eo)
+
+
+@pytest.mark.parametrize("scale,alpha", [(3.0, 0.5), (6.0, 0.25)])
+def test_energy_line_search_halves_an_overshoot(require_gpu, scale, alpha):
+    """Square cloth, one Newton system solved on the GPU; its direction scaled by 3 (or 6)
+    overshoots the energy's minimum along it, so E(1) > E(0) and the search halves the step once
+    (twice) to alpha = 0.5 (0.25), as the oracle does from the same state and direction; the
+    positions then agree to FP32 rounding."""
+    sc = scenes.square_cloth(17)
+    sc.params.use_energy_linesearch = True
+    g, o = staged(sc)
+    g.stage("pcg")
+    d = scale * g.read("dx").reshape(-1, 3)
+    g.write("dx", d)
+    o.set("dx", d)
+    g.stage("line_search")
+    o.stage("line_search")
+    sco = o.scalars()
+    _, e0, e = o.energy()
+    ag = float(g.read("toi")[0])  # the step taken (the stage leaves the stats of the last full iteration)
+    print(f"[gpu-energy] x{scale}: gpu alpha {ag}, oracle alpha {sco['alpha']}, E0 {e0:.6e} E {e:.6e}", flush=True)
+    assert ag == alpha and sco["alpha"] == alpha
+    assert e < e0
+    xg, xo = g.read("x").reshape(-1, 3), o.get("x").reshape(-1, 3)
+    assert np.abs(xg - xo).max() <= 1e-6 * np.abs(xo).max()
+
+
+def test_energy_line_search_frames(require_gpu):
+    """Square cloth falling onto the floor, six frames with the energy line search on: positions
+    track the oracle's (same search) to 1e-4 relative, the bar of the frames without it."""
+    sc = scenes.square_cloth(33)
+    sc.params.use_energy_linesearch = True
+    g, o = NewtonSolver.from_scene(sc), oracle.Oracle(sc, 32)
+    for f in range(6):
+        g.physics_step_gpu(1)
+        o.step()
+        xg, xo = g.get_simulation_results(), o.get("x").reshape(-1, 3)
+        err = np.abs(xg - xo).max() / np.abs(xo).max()
+        print(f"[gpu-energy] square33 frame {f}: rel err {err:.3e}, {g.stats()}", flush=True)
+        assert err < 1e-4
+    assert g.get_simulation_results()[:, 1].min() > 0.0
+
+
+def test_energy_line_search_contact_frame_vs_fp64(require_gpu):
+    """Two sheets in self-contact with the energy line search on, one frame from the oracle's
+    state after two frames. The PCG stops at its cap here, so FP32 results depend on summation
+    order (the FP32 oracle differs from its FP64 twin by ~2e-2 after this frame); bar: the GPU's error against the
+    FP64 twin is at most three times the FP32 oracle's (plus 1e-6 relative)."""
+    sc = scenes.small_two_sheets(9, gap=1.6e-3)
+    sc.params.use_energy_linesearch = True
+    g, o, o64 = NewtonSolver.from_scene(sc), oracle.Oracle(sc, 32), oracle.Oracle(sc, 64)
+    for _ in range(2):
+        o.step()
+        o64.step()
+        g.physics_step_gpu(1)
+    x0, v0 = o.get("x_outer"), o.get("v_outer")
+    g.set_state(x0, v0)
+    o64.set("x_outer", x0)
+    o64.set("v_outer", v0)
+    g.physics_step_gpu(1)
+    o.step()
+    o64.step()
+    x64 = o64.get("x").reshape(-1, 3)
+    scale = np.abs(x64).max()
+    err_gpu = np.abs(g.get_simulation_results() - x64).max() / scale
+    err_o32 = np.abs(o.get("x").reshape(-1, 3) - x64).max() / scale
+    print(f"[gpu-energy] two sheets frame vs FP64: gpu {err_gpu:.3e}, FP32 oracle {err_o32:.3e}, {g.stats()}",
+          flush=True)
+    assert np.isfinite(g.get_simulation_results()).all()
+    assert err_gpu <= 3.0 * err_o32 + 1e-6
