@@ -98,6 +98,9 @@ typedef struct lcsx_params
                                    2: the same with BF16 block storage                          */
     int   use_energy_linesearch; /* 1: after the CCD bound, halve the step (at most 20 times) while
                                    the energy does not decrease (newton_solver.cpp:3803-3856)  */
+    int   contact_energy_type;  /* 1: IPC log barrier (the reference's default); 0: quadratic
+                                   penalty 1/2 kappa area C^2, C = thickness + d_hat - d
+                                   (SceneParams::contact_energy_type, scene_params.h:89)        */
 } lcsx_params;
 
 /* Scalars of the last Newton iteration / frame. */

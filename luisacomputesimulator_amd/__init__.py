@@ -48,7 +48,8 @@ class lcsx_params(C.Structure):
                 ("use_self_collision", C.c_int), ("use_ccd_linesearch", C.c_int), ("floor_y", C.c_float),
                 ("stiffness_collision", C.c_float), ("stiffness_dirichlet", C.c_float), ("damping_cloth", C.c_float),
                 ("bending_scaling", C.c_float), ("newton_tol", C.c_float), ("precision", C.c_int),
-                ("use_energy_linesearch", C.c_int)]
+                ("use_energy_linesearch", C.c_int),
+                ("contact_energy_type", C.c_int)]
 
 
 class lcsx_stats(C.Structure):
@@ -147,6 +148,7 @@ def params_to_c(p: Params) -> lcsx_params:
     c.newton_tol = p.newton_tol
     c.precision = {"fp32": 0, "mixed": 1, "mixed_bf16": 2}[p.precision]
     c.use_energy_linesearch = int(p.use_energy_linesearch)
+    c.contact_energy_type = int(p.contact_energy_type)
     return c
 
 

@@ -326,6 +326,7 @@ void lcsx_params_default(lcsx_params* p)
     p->newton_tol           = 1e-2f;
     p->precision            = 0;
     p->use_energy_linesearch = 0;
+    p->contact_energy_type   = 1;
 }
 void lcsx_material_default(lcsx_material* m)
 {
@@ -345,6 +346,8 @@ lcsx_status lcsx_solver_create(const lcsx_params* params, int device, lcsx_solve
     if (!out) return arg_error("out is NULL");
     if (params && (params->precision < 0 || params->precision > 2))
         return arg_error("precision must be 0 (FP32), 1 (mixed, FP16 blocks) or 2 (mixed, BF16 blocks)");
+    if (params && (params->contact_energy_type < 0 || params->contact_energy_type > 1))
+        return arg_error("contact_energy_type must be 0 (quadratic) or 1 (log barrier)");
     auto* h = new lcsx_solver();
     if (params) h->s.prm = *params;
     else lcsx_params_default(&h->s.prm);

@@ -282,5 +282,39 @@ LCSX_HD double barrier_d2(double d, double dh)
     return ((r + 2.0) * r - 2.0 * log(d / dh)) - 3.0;
 }
 
+// the contact energy forms of lcsx_physics.h (ground_k, ground_e, pair_k, pair_e), in double
+LCSX_HD double ground_e(int quad, double dist, double th, double dh, double stiff)
+{
+    if (quad)
+    {
+        double q = (dist - th) - dh;
+        return (0.5 * stiff) * (q * q);
+    }
+    return stiff * barrier(dist - th, dh);
+}
+LCSX_HD void pair_k(int quad, double d, double th, double dh, double aa, double kappa, double& k1, double& k2)
+{
+    if (quad)
+    {
+        double C = (th + dh) - d, s = aa * kappa;
+        k1       = s * C;
+        k2       = s;
+    }
+    else
+    {
+        k1 = ((-aa) * kappa) * barrier_d1(d - th, dh);
+        k2 = (aa * kappa) * barrier_d2(d - th, dh);
+    }
+}
+LCSX_HD double pair_e(int quad, double d, double th, double dh, double stiff)
+{
+    if (quad)
+    {
+        double C = (th + dh) - d;
+        return ((0.5 * stiff) * C) * C;
+    }
+    return stiff * barrier(d - th, dh);
+}
+
 }  // namespace fp64
 }  // namespace lcsx

@@ -258,6 +258,15 @@ bool load_scene_json(const std::string& path, const std::string& mesh_dir, lcsx_
     float tmp;
     if (num(root.get("nonlinear_iter_count"), tmp)) prm.nonlinear_iter_count = (int)tmp;
     if (num(root.get("pcg_iter_count"), tmp)) prm.pcg_iter_count = (int)tmp;
+    if (num(root.get("contact_energy_type"), tmp))
+    {
+        if (tmp != 0.0f && tmp != 1.0f)
+        {
+            err = "contact_energy_type must be 0 (quadratic) or 1 (log barrier)";
+            return false;
+        }
+        prm.contact_energy_type = (int)tmp;
+    }
     vec3(root.get("gravity"), prm.gravity);
     const json::Value* shells = root.get("shells");
     if (!shells || shells->kind != json::Value::Arr) return true;

@@ -113,6 +113,7 @@ struct VertArgs
     const float*    dhat;
     float           h2inv, kD, kappa, floor_y;
     int             use_floor;
+    int             quadratic;  // contact energy form: 1 quadratic penalty, 0 log barrier
     const uint32_t* v_sf_ptr;
     const uint32_t* v_sf;
     const uint32_t* v_bh_ptr;
@@ -138,6 +139,7 @@ struct DcdArgs
     const float*    dhat;
     float           kappa;
     int             fp64;  // mixed-precision mode: distances and barrier in double
+    int             quadratic;  // contact energy form: 1 quadratic penalty, 0 log barrier
     ContactBuf      out;
     const uint32_t* vert_body;  // per row: body id or ~0u; pairs within one body are skipped
 };
@@ -182,6 +184,7 @@ struct EnergyArgs
     const float*    vert_area;
     float           h2inv, kD, kappa, floor_y, scaling;
     int             use_floor, fp64;
+    int             quadratic;  // contact energy form: 1 quadratic penalty, 0 log barrier
     uint32_t        nsf, nbh, nb;
     const uint32_t* sf;
     const float4*   sf_Dm;
@@ -231,6 +234,7 @@ struct AbdArgs
     const float*   area;
     float          h2inv, kD, kappa, floor_y;
     int            use_floor;
+    int            quadratic;  // contact energy form: 1 quadratic penalty, 0 log barrier
     float*         cgB;
     float*         diag;
     float*         diag_mat;

@@ -232,9 +232,10 @@ __global__ __launch_bounds__(kB) void k_vert_assemble(VertArgs a)
         float dh = a.dhat[i], th = a.offset[i], dist = a.x[3 * (size_t)i + 1] - a.floor_y;
         if (dist - th < dh)
         {
-            float stiff = a.kappa * a.area[i];
-            grad.y      = grad.y - stiff * barrier_d1(dist - th, dh);
-            H.m[4]      = H.m[4] + stiff * barrier_d2(dist - th, dh);
+            float k1, k2;
+            ground_k(a.quadratic, dist, th, dh, a.kappa * a.area[i], k1, k2);
+            grad.y = grad.y - k1;
+            H.m[4] = H.m[4] + k2;
         }
     }
     for (uint32_t p = a.v_sf_ptr[i]; p < a.v_sf_ptr[i + 1]; p++)
@@ -403,9 +404,8 @@ __global__ __launch_bounds__(kB) void k_abd_material(AbdArgs a)
             float dist = a.x[3 * (size_t)v + 1] - a.floor_y, th = a.offset[v], dh = a.dhat[v];
             if (!(dist - th < dh)) continue;
             const f3    n     = mk(0.0f, 1.0f, 0.0f);
-            float       stiff = a.kappa * a.area[v];
-            float       k1    = stiff * barrier_d1(dist - th, dh);
-            float       k2    = stiff * barrier_d2(dist - th, dh);
+            float       k1, k2;
+            ground_k(a.quadratic, dist, th, dh, a.kappa * a.area[v], k1, k2);
             f3          gr    = k1 * n;
             m3          hs    = m3_scale(k2, m3_outer(n, n));
             const float w[4]  = {1.0f, a.body_X[3 * (size_t)v], a.body_X[3 * (size_t)v + 1], a.body_X[3 * (size_t)v + 2]};
@@ -497,9 +497,10 @@ __global__ __launch_bounds__(kB) void k_dcd_vf(DcdArgs a)
                 double d  = sqrt(d2);
                 d3     nd = X / d;
                 nrm       = mk((float)nd.x, (float)nd.y, (float)nd.z);
-                double aa = 0.5 * ((double)a.area_a[vid] + (double)a.area_b[fid]);
-                k1        = (float)(((-aa) * (double)a.kappa) * fp64::barrier_d1(d - th, dh));
-                k2        = (float)((aa * (double)a.kappa) * fp64::barrier_d2(d - th, dh));
+                double aa = 0.5 * ((double)a.area_a[vid] + (double)a.area_b[fid]), dk1, dk2;
+                fp64::pair_k(a.quadratic, d, th, dh, aa, (double)a.kappa, dk1, dk2);
+                k1        = (float)dk1;
+                k2        = (float)dk2;
                 ak        = (float)(aa * (double)a.kappa);
                 emit      = true;
             }
@@ -517,8 +518,7 @@ __global__ __launch_bounds__(kB) void k_dcd_vf(DcdArgs a)
                 float d  = sqrtf(d2);
                 nrm      = X / d;
                 float aa = 0.5f * (a.area_a[vid] + a.area_b[fid]);
-                k1       = ((-aa) * a.kappa) * barrier_d1(d - th, dh);
-                k2       = (aa * a.kappa) * barrier_d2(d - th, dh);
+                pair_k(a.quadratic, d, th, dh, aa, a.kappa, k1, k2);
                 ak       = aa * a.kappa;
                 emit     = true;
             }
@@ -558,9 +558,10 @@ __global__ __launch_bounds__(kB) void k_dcd_ee(DcdArgs a)
                 double d  = sqrt(d2);
                 d3     nd = X / d;
                 nrm       = mk((float)nd.x, (float)nd.y, (float)nd.z);
-                double aa = 0.5 * ((double)a.area_a[l] + (double)a.area_b[r]);
-                k1        = (float)(((-aa) * (double)a.kappa) * fp64::barrier_d1(d - th, dh));
-                k2        = (float)((aa * (double)a.kappa) * fp64::barrier_d2(d - th, dh));
+                double aa = 0.5 * ((double)a.area_a[l] + (double)a.area_b[r]), dk1, dk2;
+                fp64::pair_k(a.quadratic, d, th, dh, aa, (double)a.kappa, dk1, dk2);
+                k1        = (float)dk1;
+                k2        = (float)dk2;
                 ak        = (float)(aa * (double)a.kappa);
                 emit      = true;
             }
@@ -578,8 +579,7 @@ __global__ __launch_bounds__(kB) void k_dcd_ee(DcdArgs a)
                 float d  = sqrtf(d2);
                 nrm      = X / d;
                 float aa = 0.5f * (a.area_a[l] + a.area_b[r]);
-                k1       = ((-aa) * a.kappa) * barrier_d1(d - th, dh);
-                k2       = (aa * a.kappa) * barrier_d2(d - th, dh);
+                pair_k(a.quadratic, d, th, dh, aa, a.kappa, k1, k2);
                 ak       = aa * a.kappa;
                 emit     = true;
             }
@@ -1193,8 +1193,8 @@ __global__ __launch_bounds__(kB) void k_energy_rows(EnergyArgs a)
             if (dist - th < dh)
             {
                 const float stiff = a.kappa * a.vert_area[i];
-                e += a.fp64 ? (double)stiff * fp64::barrier((double)dist - (double)th, (double)dh)
-                            : (double)(stiff * barrier_e(dist - th, dh));
+                e += a.fp64 ? fp64::ground_e(a.quadratic, (double)dist, (double)th, (double)dh, (double)stiff)
+                            : (double)ground_e(a.quadratic, dist, th, dh, stiff);
             }
         }
     }
@@ -1283,8 +1283,8 @@ __global__ __launch_bounds__(kB) void k_energy_contacts(EnergyArgs a)
             const float dh   = 0.5f * (a.dhat[i0] + a.dhat[i2]);
             const float th   = a.offset[i0] + a.offset[i2];
             if (d < th + dh)
-                e = a.fp64 ? (double)ak * fp64::barrier((double)d - (double)th, (double)dh)
-                           : (double)(ak * barrier_e(d - th, dh));
+                e = a.fp64 ? fp64::pair_e(a.quadratic, (double)d, (double)th, (double)dh, (double)ak)
+                           : (double)pair_e(a.quadratic, d, th, dh, ak);
         }
         scene_add(e, sc, a.out);
     }

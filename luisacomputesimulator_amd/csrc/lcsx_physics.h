@@ -266,6 +266,59 @@ LCSX_HD float barrier_e(float d, float dh)
     return -dm * dm * logf(d / dh);
 }
 
+// ---------------------------------------------------------------- contact energy forms
+// SceneParams::contact_energy_type (scene_params.h:89): 1 the log barrier above on d - thickness,
+// 0 the quadratic penalty E = 1/2 k C^2, C = thickness + d_hat - d.
+// Ground (newton_solver.cpp:916-925): k1 = dE/d(dist), k2 = its derivative; stiff = kappa * area.
+LCSX_HD void ground_k(int quad, float dist, float th, float dh, float stiff, float& k1, float& k2)
+{
+    if (quad)
+    {
+        k1 = stiff * ((dist - th) - dh);
+        k2 = stiff;
+    }
+    else
+    {
+        k1 = stiff * barrier_d1(dist - th, dh);
+        k2 = stiff * barrier_d2(dist - th, dh);
+    }
+}
+// solver_interface.cpp:577-584
+LCSX_HD float ground_e(int quad, float dist, float th, float dh, float stiff)
+{
+    if (quad)
+    {
+        float q = (dist - th) - dh;
+        return (0.5f * stiff) * (q * q);
+    }
+    return stiff * barrier_e(dist - th, dh);
+}
+// Pairs (narrow_phase.cpp:873-885, 1031-1043): k1 = -dE/dd, k2 = d2E/dd2, stiffness aa * kappa.
+LCSX_HD void pair_k(int quad, float d, float th, float dh, float aa, float kappa, float& k1, float& k2)
+{
+    if (quad)
+    {
+        float C = (th + dh) - d, s = aa * kappa;
+        k1      = s * C;
+        k2      = s;
+    }
+    else
+    {
+        k1 = ((-aa) * kappa) * barrier_d1(d - th, dh);
+        k2 = (aa * kappa) * barrier_d2(d - th, dh);
+    }
+}
+// narrow_phase.cpp:3022-3030 (stiff: the pair's stored aa * kappa)
+LCSX_HD float pair_e(int quad, float d, float th, float dh, float stiff)
+{
+    if (quad)
+    {
+        float C = (th + dh) - d;
+        return ((0.5f * stiff) * C) * C;
+    }
+    return stiff * barrier_e(d - th, dh);
+}
+
 // ---------------------------------------------------------------- Baraff-Witkin membrane
 // F = [x1 - x0, x2 - x0] Dm^-1, Dm^-1 column-major (d0, d1, d2, d3): Fu = e0 d0 + e1 d1,
 // Fv = e0 d2 + e1 d3. Output: gradient per vertex and the 9x9 Hessian as 3x3 blocks

@@ -398,6 +398,7 @@ void Solver::material()
     a.kappa    = prm.stiffness_collision;
     a.floor_y  = prm.floor_y;
     a.use_floor = prm.use_floor;
+    a.quadratic = prm.contact_energy_type == 0;
     a.v_sf_ptr = v_sf_ptr.p;
     a.v_sf     = v_sf.p;
     a.v_bh_ptr = v_bh_ptr.p;
@@ -428,6 +429,7 @@ void Solver::material()
         b.kappa     = a.kappa;
         b.floor_y   = a.floor_y;
         b.use_floor = a.use_floor;
+        b.quadratic = a.quadratic;
         b.cgB       = cgB.p;
         b.diag      = diag.p;
         b.diag_mat  = diag_mat.p;
@@ -511,6 +513,7 @@ void Solver::dcd(bool from_ccd_pairs)
     a.dhat   = dhat.p;
     a.kappa  = prm.stiffness_collision;
     a.fp64   = prm.precision >= 1;
+    a.quadratic = prm.contact_energy_type == 0;
     a.out    = out;
     a.vert_body = vert_body_d.p;
     a.pairs = pairs_vf.p;
@@ -916,6 +919,7 @@ std::vector<double> Solver::energy()
     a.scaling    = prm.bending_scaling;
     a.use_floor  = prm.use_floor;
     a.fp64       = prm.precision >= 1;
+    a.quadratic  = prm.contact_energy_type == 0;
     a.nsf        = (uint32_t)(S.sf.size() / 3);
     a.sf         = sf.p;
     a.sf_Dm      = sf_Dm.p;

@@ -111,6 +111,9 @@ class Params:
     newton_tol: float = 1e-2
     # energy backtracking after the CCD bound (use_energy_linesearch, newton_solver.cpp:3803-3856)
     use_energy_linesearch: bool = False
+    # contact energy form (contact_energy_type, scene_params.h:89): 1 IPC log barrier, 0 quadratic
+    # penalty 1/2 kappa area C^2 with C = thickness + d_hat - d
+    contact_energy_type: int = 1
     # "fp32" (the reference's path), "mixed" (FP16 blocks + FP32 refinement, FP64 CCD and barrier)
     # or "mixed_bf16" (the same with BF16 blocks); not part of pack(): the oracle runs FP32 or its
     # FP64 twin
@@ -121,7 +124,8 @@ class Params:
                 self.pcg_check_every, self.pcg_rel_tol, self.pcg_abs_tol, *map(float, self.gravity),
                 float(self.use_floor), float(self.use_self_collision), float(self.use_ccd_linesearch),
                 self.floor_y, self.stiffness_collision, self.stiffness_dirichlet, self.damping_cloth,
-                self.bending_scaling, self.newton_tol, float(self.use_energy_linesearch)]
+                self.bending_scaling, self.newton_tol, float(self.use_energy_linesearch),
+                float(self.contact_energy_type)]
 
 
 @dataclasses.dataclass
@@ -447,10 +451,12 @@ def from_json(path: str, mesh_dir: str) -> Scene:
                       ("stiffness_dirichlet", "stiffness_dirichlet")):
         if isinstance(js.get(key), (int, float)) and not isinstance(js.get(key), bool):
             setattr(prm, attr, float(js[key]))
-    for key in ("nonlinear_iter_count", "pcg_iter_count"):
+    for key in ("nonlinear_iter_count", "pcg_iter_count", "contact_energy_type"):
         if isinstance(js.get(key), (int, float)) and not isinstance(js.get(key), bool):
             setattr(prm, key, int(js[key]))
-    for key in ("use_floor", "use_self_collision", "use_ccd_linesearch"):
+    if prm.contact_energy_type not in (0, 1):
+        raise ValueError("contact_energy_type must be 0 (quadratic) or 1 (log barrier)")
+    for key in ("use_floor", "use_self_collision", "use_ccd_linesearch", "use_energy_linesearch"):
         if isinstance(js.get(key), bool):
             setattr(prm, key, js[key])
     if isinstance(js.get("gravity"), list):

@@ -197,14 +197,15 @@ inline void abd_orthogonality(const V3<T> q[4], T stiff, BodyGH<T>& o)
 // fn_evaluate_abd_ground_collision for one rigid vertex at x (barrier on y - floor), lifted to
 // the body with weights (1, X): o.g[ii] += w_ii k1 n, o.H[ii, jj] += w_ii w_jj k2 n n^T.
 template <class T>
-inline void abd_ground_vertex(V3<T> x, V3<T> X, T floor_y, T dhat, T thickness, T area, T kappa, BodyGH<T>& o)
+inline void abd_ground_vertex(V3<T> x, V3<T> X, T floor_y, T dhat, T thickness, T area, T kappa, BodyGH<T>& o,
+                              int quad = 0)
 {
     T dist = x.y - floor_y;
     if (!(dist - thickness < dhat)) return;
     const V3<T> n(T(0), T(1), T(0));
     T           stiff = kappa * area;
-    T           k1    = stiff * barrier_d1(dist - thickness, dhat);
-    T           k2    = stiff * barrier_d2(dist - thickness, dhat);
+    T           k1, k2;
+    ground_k(quad, dist, thickness, dhat, stiff, k1, k2);
     V3<T>       grad  = k1 * n;
     M3<T>       hess  = k2 * outer(n, n);
     const T     w[4]  = {T(1), X.x, X.y, X.z};

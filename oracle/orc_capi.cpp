@@ -171,7 +171,7 @@ void orc_free(void* p) { delete (Handle*)p; }
 
 // params order: dt, newton_min, newton_max, pcg_max, pcg_check_every, pcg_rel_tol, pcg_abs_tol,
 // gx, gy, gz, use_floor, use_self_collision, use_ccd, floor_y, kappa, k_dirichlet, damping,
-// bending_scaling, newton_tol
+// bending_scaling, newton_tol, use_energy_linesearch, contact_energy_type (1 barrier, 0 quadratic)
 int orc_set_params(void* p, const double* a, int n)
 {
     if (n < 19) return -1;
@@ -183,6 +183,7 @@ int orc_set_params(void* p, const double* a, int n)
     q.floor_y = a[13]; q.kappa = a[14]; q.k_dirichlet = a[15]; q.damping = a[16];
     q.bending_scaling = a[17]; q.newton_tol = a[18];
     q.use_energy_ls = n > 19 ? (int)a[19] : 0;
+    q.contact_quadratic = n > 20 ? (int)a[20] == 0 : 0;
     Handle* h = (Handle*)p;
     h->s32.prm = q;
     h->s64.prm = q;

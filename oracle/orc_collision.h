@@ -261,4 +261,52 @@ template <class T> inline T barrier_d2(T d, T dh)
     return ((r + T(2)) * r - T(2) * std::log(d / dh)) - T(3);
 }
 
+// Contact energy forms (SceneParams::contact_energy_type, scene_params.h:89): 1 the log barrier on
+// d - thickness, 0 the quadratic penalty E = 1/2 k C^2 with C = thickness + d_hat - d.
+// Ground, stiffness kappa * area: k1 = dE/d(dist), k2 its derivative (newton_solver.cpp:916-925),
+// energy solver_interface.cpp:577-584.
+template <class T> inline void ground_k(int quad, T dist, T th, T dh, T stiff, T& k1, T& k2)
+{
+    if (quad)
+    {
+        k1 = stiff * ((dist - th) - dh);
+        k2 = stiff;
+        return;
+    }
+    k1 = stiff * barrier_d1(dist - th, dh);
+    k2 = stiff * barrier_d2(dist - th, dh);
+}
+template <class T> inline T ground_e(int quad, T dist, T th, T dh, T stiff)
+{
+    if (quad)
+    {
+        T q = (dist - th) - dh;
+        return (T(0.5f) * stiff) * (q * q);
+    }
+    return stiff * barrier(dist - th, dh);
+}
+// Pairs, stiffness aa * kappa: k1 = -dE/dd, k2 = d2E/dd2 (narrow_phase.cpp:873-885, 1031-1043),
+// energy narrow_phase.cpp:3022-3030.
+template <class T> inline void pair_k(int quad, T d, T th, T dh, T aa, T kap, T& k1, T& k2)
+{
+    if (quad)
+    {
+        T C = (th + dh) - d, s = aa * kap;
+        k1  = s * C;
+        k2  = s;
+        return;
+    }
+    k1 = ((-aa) * kap) * barrier_d1(d - th, dh);
+    k2 = (aa * kap) * barrier_d2(d - th, dh);
+}
+template <class T> inline T pair_e(int quad, T d, T th, T dh, T stiff)
+{
+    if (quad)
+    {
+        T C = (th + dh) - d;
+        return ((T(0.5f) * stiff) * C) * C;
+    }
+    return stiff * barrier(d - th, dh);
+}
+
 }  // namespace orc

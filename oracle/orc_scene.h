@@ -56,6 +56,7 @@ struct Params
     int    use_floor = 1, use_self_collision = 1, use_ccd = 1;
     double floor_y = 0.0, kappa = 1e7, k_dirichlet = 1e9, damping = 2.0, bending_scaling = 1.0, newton_tol = 1e-2;
     int    use_energy_ls = 0;  // energy backtracking after the CCD bound (use_energy_linesearch)
+    int    contact_quadratic = 0;  // contact_energy_type == 0: quadratic penalty instead of the barrier
 };
 
 template <class T>
@@ -643,9 +644,8 @@ struct Scene
                 T dh = dhat[i], th = offset[i], dist = x[i].y - fy;
                 if (dist - th < dh)
                 {
-                    T stiff = kap * vert_area[i];
-                    T k1    = stiff * barrier_d1(dist - th, dh);
-                    T k2    = stiff * barrier_d2(dist - th, dh);
+                    T k1, k2;
+                    ground_k(prm.contact_quadratic, dist, th, dh, kap * vert_area[i], k1, k2);
                     cgB[i].y        = cgB[i].y - k1;
                     diag[i].m[1][1] = diag[i].m[1][1] + k2;
                 }
@@ -745,7 +745,7 @@ struct Scene
             for (uint32_t i = b.v0; i < b.v1; i++)
                 if (!is_fixed[i])
                     abd_ground_vertex(x[i], body_X[i], T(float(prm.floor_y)), dhat[i], offset[i], vert_area[i],
-                                      T(float(prm.kappa)), o);
+                                      T(float(prm.kappa)), o, prm.contact_quadratic);
         return o;
     }
     void body_terms(const Body& b, T h2inv, T kD)
@@ -869,8 +869,7 @@ struct Scene
                 c.type       = 4;
                 c.n          = X / d;
                 T aa         = T(0.5f) * (vert_area[vid] + face_area[fid]);
-                c.k1         = ((-aa) * kap) * barrier_d1(d - th, dh);
-                c.k2         = (aa * kap) * barrier_d2(d - th, dh);
+                pair_k(prm.contact_quadratic, d, th, dh, aa, kap, c.k1, c.k2);
                 c.area_kappa = aa * kap;
                 c.bary0      = b.x;
                 c.bary1      = b.y;
@@ -905,8 +904,7 @@ struct Scene
                 c.type       = 8;
                 c.n          = X / d;
                 T aa         = T(0.5f) * (edge_area[l] + edge_area[r]);
-                c.k1         = ((-aa) * kap) * barrier_d1(d - th, dh);
-                c.k2         = (aa * kap) * barrier_d2(d - th, dh);
+                pair_k(prm.contact_quadratic, d, th, dh, aa, kap, c.k1, c.k2);
                 c.area_kappa = aa * kap;
                 c.bary0      = c4.v[0];
                 c.bary1      = c4.v[2];
@@ -1106,7 +1104,7 @@ struct Scene
             {
                 if (is_fixed[i]) continue;
                 T dh = dhat[i], th = offset[i], dist = x[i].y - fy;
-                if (dist - th < dh) e_gr += (double)((kap * vert_area[i]) * barrier(dist - th, dh));
+                if (dist - th < dh) e_gr += (double)ground_e(prm.contact_quadratic, dist, th, dh, kap * vert_area[i]);
             }
         }
         for (size_t f = 0; f < sf.size() / 3; f++)
@@ -1144,7 +1142,7 @@ struct Scene
             T d    = std::sqrt(dot(diff, diff));
             T dh   = T(0.5f) * (dhat[c.idx[0]] + dhat[c.idx[2]]);
             T th   = offset[c.idx[0]] + offset[c.idx[2]];
-            if (d < th + dh) e_ct += (double)(c.area_kappa * barrier(d - th, dh));
+            if (d < th + dh) e_ct += (double)pair_e(prm.contact_quadratic, d, th, dh, c.area_kappa);
         }
         return ((((e_in + e_gr) + e_st) + e_be) + e_ab) + e_ct;
     }

@@ -53,3 +53,9 @@ def test_precision_parameter_is_validated():
     NewtonSolver(scenes.Params(precision="mixed_bf16"))
     with pytest.raises(KeyError):
         NewtonSolver(scenes.Params(precision="fp16"))
+    lib.lcsx_params_default(C.byref(p))
+    assert p.contact_energy_type == 1  # the reference's default: the log barrier
+    p.contact_energy_type = 2
+    assert lib.lcsx_solver_create(C.byref(p), 0, C.byref(h)) != 0
+    assert b"contact_energy_type" in lib.lcsx_last_error()
+    NewtonSolver(scenes.Params(contact_energy_type=0))  # accepted

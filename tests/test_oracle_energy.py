@@ -22,9 +22,12 @@ def fd_check(o, x, rows, h=1e-6, rtol=2e-4):
     o.set("x", x)
 
 
-def test_cloth_energy_gradient():
-    """Inertia, BW stretch + shear, bending and ground barrier (cloth pressed near the floor)."""
+@pytest.mark.parametrize("contact_type", [1, 0])
+def test_cloth_energy_gradient(contact_type):
+    """Inertia, BW stretch + shear, bending and ground contact (cloth pressed near the floor), with
+    the log barrier (1) and the quadratic penalty (0)."""
     sc = scenes.square_cloth(17)
+    sc.params.contact_energy_type = contact_type
     sc.meshes[0].verts[:, 1] = np.float32(1.5e-3)  # inside the ground band (offset 0.5 mm + d_hat 1 mm)
     o = oracle.Oracle(sc, 64)
     for s in ("frame_begin", "predict", "iter_begin"):
@@ -38,10 +41,13 @@ def test_cloth_energy_gradient():
     fd_check(o, x, free[:: max(1, len(free) // 12)])
 
 
-def test_contact_energy_gradient():
-    """Barrier energy of the DCD contact list: two sheets 1.2 mm apart (contact band 1.5 mm)."""
+@pytest.mark.parametrize("contact_type", [1, 0])
+def test_contact_energy_gradient(contact_type):
+    """Energy of the DCD contact list, log barrier (1) or quadratic penalty (0): two sheets 1.2 mm
+    apart (contact band 1.5 mm)."""
     sc = scenes.small_two_sheets(9, gap=1.2e-3)
     sc.params.use_floor = False
+    sc.params.contact_energy_type = contact_type
     o = oracle.Oracle(sc, 64)
     for s in ("frame_begin", "predict", "iter_begin", "material", "update_contacts", "contact_assemble"):
         o.stage(s)
@@ -82,3 +88,33 @@ def test_energy_line_search_descends():
                 assert e < e0 + 1.2e-7
         o.stage("frame_end")
     assert np.isfinite(o.get("x")).all()
+
+
+@pytest.mark.parametrize("y", [1.2e-3, 3e-4])
+def test_quadratic_ground_energy_closed_form(y):
+    """Flat cloth at height y, inside the ground band (or closer than its offset, where the
+    barrier has no value but the penalty does): the ground term of the quadratic form is
+    1/2 kappa A (d_hat + offset - y)^2 with A the cloth's area, and its Hessian adds kappa times each
+    vertex's area to the yy entry of the diagonal blocks."""
+    def staged(floor):
+        sc = scenes.square_cloth(9)
+        sc.meshes[0].verts[:, 1] = np.float32(y)
+        sc.params.contact_energy_type = 0
+        sc.params.use_floor = floor
+        o = oracle.Oracle(sc, 64)
+        for s in ("frame_begin", "predict", "iter_begin"):
+            o.stage(s)
+        o.set("x", o.get("x_outer"))
+        o.stage("material")
+        return o
+    a, b = staged(True), staged(False)
+    area, th, dh = a.get("vert_area"), a.get("offset"), a.get("dhat")
+    free = a.get("is_fixed") == 0
+    kappa = scenes.Params().stiffness_collision
+    y32 = float(np.float32(y))
+    expect = 0.5 * kappa * np.sum(area[free] * (dh[free] + th[free] - y32) ** 2)
+    got = a.energy()[0] - b.energy()[0]
+    assert expect > 0
+    assert abs(got - expect) <= 1e-6 * expect
+    dyy = a.get("diag").reshape(-1, 9)[:, 4] - b.get("diag").reshape(-1, 9)[:, 4]
+    assert np.allclose(dyy[free], kappa * area[free], rtol=1e-9)

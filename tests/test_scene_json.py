@@ -123,3 +123,22 @@ def test_rigid_material_fields_from_json(tmp_path):
     s = load_product(str(p))
     assert s.num_verts == 8 + 4
     assert np.all(s.read("offset")[:8] == 0.0)  # a rigid solid has no contact thickness
+
+
+def test_solver_options_from_json(tmp_path):
+    """contact_energy_type (0 quadratic, 1 log barrier) and use_energy_linesearch are read from the
+    scene keys by both loaders; another contact_energy_type is rejected by both."""
+    import json
+    js = {"contact_energy_type": 0, "use_energy_linesearch": True,
+          "shells": [{"model_name": "square2.obj", "shell_type": "Cloth"}]}
+    p = tmp_path / "opts.json"
+    p.write_text(json.dumps(js))
+    prm = scenes.from_json(str(p), SCENE_DIR).params
+    assert (prm.contact_energy_type, prm.use_energy_linesearch) == (0, True)
+    load_product(str(p))
+    js["contact_energy_type"] = 3
+    p.write_text(json.dumps(js))
+    with pytest.raises(ValueError):
+        scenes.from_json(str(p), SCENE_DIR)
+    with pytest.raises(LcsxError, match="contact_energy_type"):
+        NewtonSolver().load_scene_json(str(p), SCENE_DIR)
