@@ -57,6 +57,10 @@ typedef struct lcsx_material
                                      solid (affine body of a closed surface, contact thickness 0)
                                      (RigidMaterial, physical_material.h:69-77)              */
     float rigid_stiffness;        /* RigidMaterial::stiffness: orthogonality kappa          */
+    int   stretch_spring;         /* cloth stretch model: 0 Baraff-Witkin FEM (FEM_BW98, the
+                                     reference's default), 1 a spring on every edge with stiffness
+                                     thickness * E / (2 (1 + nu)) (ConstitutiveStretchModelCloth::
+                                     Spring, physical_material.h:10-15, init_sim_data.cpp:205-370) */
 } lcsx_material;
 
 /* FixedPointAnimationInfo, Solver/Initializer/init_mesh_data.h:39-93. */

@@ -18,7 +18,7 @@ from typing import Dict, List, Optional
 import numpy as np
 
 from . import scenes  # noqa: F401
-from .scenes import BODY_TYPES, FIXED_METHODS, Anim, Material, Params, Scene
+from .scenes import BODY_TYPES, FIXED_METHODS, STRETCH_MODELS, Anim, Material, Params, Scene
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "liblcsx.so")
@@ -32,7 +32,8 @@ class LcsxError(RuntimeError):
 class lcsx_material(C.Structure):
     _fields_ = [("thickness", C.c_float), ("youngs_modulus", C.c_float), ("poisson_ratio", C.c_float),
                 ("area_bending_stiffness", C.c_float), ("density", C.c_float), ("mass", C.c_float),
-                ("d_hat", C.c_float), ("body", C.c_int), ("rigid_stiffness", C.c_float)]
+                ("d_hat", C.c_float), ("body", C.c_int), ("rigid_stiffness", C.c_float),
+                ("stretch_spring", C.c_int)]
 
 
 class lcsx_fixed_anim(C.Structure):
@@ -158,6 +159,7 @@ def material_to_c(m: Material) -> lcsx_material:
     c.area_bending_stiffness, c.density, c.mass, c.d_hat = m.area_bending_stiffness, m.density, m.mass, m.d_hat
     c.body = BODY_TYPES[m.body]
     c.rigid_stiffness = m.rigid_stiffness
+    c.stretch_spring = STRETCH_MODELS[m.stretch_model]
     return c
 
 
@@ -175,6 +177,7 @@ def anim_to_c(a: Anim) -> lcsx_fixed_anim:
 _NP = {"f": np.float32, "u": np.uint32, "q": np.uint64, "b": np.uint8, "d": np.float64}
 # dtype of each readable buffer (see lcsx_solver_read)
 _DTYPES: Dict[str, str] = {n: "u" for n in ("faces", "edges", "hinges", "csr", "stretch_faces", "bending_hinges",
+                                            "stretch_springs",
                                              "fixed_verts", "pairs_vf", "pairs_ee", "contact_idx", "contact_type",
                                              "ccd_rest_vf", "ccd_rest_ee")}
 _DTYPES["is_fixed"] = "b"

@@ -137,7 +137,8 @@ bool is_static_buffer(const std::string& n)
 {
     static const char* k[] = {"rest_x", "faces", "edges", "hinges", "csr", "stretch_faces", "bending_hinges",
                               "fixed_verts", "is_fixed", "face_area", "edge_area", "vert_area", "vert_mass", "offset",
-                              "dhat", "sf_Dm", "sf_area", "sf_mu", "sf_lambda", "bh_angle", "bh_hbar", "bh_k"};
+                              "dhat", "sf_Dm", "sf_area", "sf_mu", "sf_lambda", "bh_angle", "bh_hbar", "bh_k",
+                              "stretch_springs", "sp_rest", "sp_k"};
     for (const char* c : k)
         if (n == c) return true;
     return false;
@@ -176,6 +177,9 @@ bool read_named(Solver& S, const std::string& name, Blob& out)
     if (name == "csr") { out.set(S.S.csr); return true; }
     if (name == "stretch_faces") { out.set(S.S.sf); return true; }
     if (name == "bending_hinges") { out.set(S.S.bh); return true; }
+    if (name == "stretch_springs") { out.set(S.S.sp); return true; }
+    if (name == "sp_rest") { out.set(S.S.sp_rest); return true; }
+    if (name == "sp_k") { out.set(S.S.sp_k); return true; }
     if (name == "fixed_verts") { out.set(S.S.fixed_verts); return true; }
     if (name == "is_fixed") { out.set(S.S.is_fixed); return true; }
     if (name == "face_area") { out.set(S.S.face_area); return true; }
@@ -339,6 +343,7 @@ void lcsx_material_default(lcsx_material* m)
     m->d_hat                  = 2e-3f;
     m->body                   = 0;
     m->rigid_stiffness        = 1e6f;
+    m->stretch_spring         = 0;
 }
 
 lcsx_status lcsx_solver_create(const lcsx_params* params, int device, lcsx_solver** out)

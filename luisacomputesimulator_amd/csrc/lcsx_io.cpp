@@ -338,6 +338,11 @@ bool load_scene_json(const std::string& path, const std::string& mesh_dir, lcsx_
                 num(mat->get("youngs_modulus"), m.mat.youngs_modulus);
                 num(mat->get("poisson_ratio"), m.mat.poisson_ratio);
                 num(mat->get("area_bending_stiffness"), m.mat.area_bending_stiffness);
+                // stretch_model: "Spring" or the enum value 0 selects springs, anything else FEM_BW98
+                // (app_simulation_demo_config.cpp:455-475)
+                const json::Value* sm = mat->get("stretch_model");
+                if (sm && sm->kind == json::Value::Str) m.mat.stretch_spring = sm->s == "Spring";
+                else if (sm && sm->kind == json::Value::Num) m.mat.stretch_spring = (int)sm->n == 0;
             }
         }
         const json::Value* fps = sh.get("fixed_points");

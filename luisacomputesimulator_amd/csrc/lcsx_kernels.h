@@ -118,6 +118,9 @@ struct VertArgs
     const uint32_t* v_sf;
     const uint32_t* v_bh_ptr;
     const uint32_t* v_bh;
+    const uint32_t* v_sp_ptr;
+    const uint32_t* v_sp;
+    const float*    spring_stage;
     const float*    face_stage;
     const float*    hinge_stage;
     float*          cgB;
@@ -185,7 +188,9 @@ struct EnergyArgs
     float           h2inv, kD, kappa, floor_y, scaling;
     int             use_floor, fp64;
     int             quadratic;  // contact energy form: 1 quadratic penalty, 0 log barrier
-    uint32_t        nsf, nbh, nb;
+    uint32_t        nsf, nbh, nb, nsp;
+    const uint32_t* sp;
+    const float2*   sp_c;   // rest length, stiffness
     const uint32_t* sf;
     const float4*   sf_Dm;
     const float4*   sf_c;   // area, mu, lambda
@@ -283,7 +288,8 @@ void launch_face_stage(uint32_t nsf, const uint32_t* sf, const float4* Dm, const
 void launch_hinge_stage(uint32_t nbh, const uint32_t* bh, const float4* c, float scaling, const float* x, float* stage,
                         hipStream_t s);
 void launch_vert_assemble(const VertArgs& a, hipStream_t s);
-void launch_offdiag(uint32_t nblocks, const uint32_t* ptr, const uint32_t* con, const float* fs, const float* hs, float* val,
+void launch_spring_stage(uint32_t nsp, const uint32_t* sp, const float2* c, const float* x, float* stage, hipStream_t s);
+void launch_offdiag(uint32_t nblocks, const uint32_t* ptr, const uint32_t* con, const float* ss, const float* fs, const float* hs, float* val,
                     hipStream_t s);
 void launch_dcd(const DcdArgs& a, int kind, unsigned grid, hipStream_t s);
 // Per-vertex contact lists (cnt: nv + 1 scratch counters, ptr: nv + 1, list: 4 per contact), and

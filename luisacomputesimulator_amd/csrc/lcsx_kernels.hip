@@ -141,6 +141,38 @@ __global__ void k_frame_end(uint32_t nv, float* __restrict__ x_step, const float
 }
 
 // ---------------------------------------------------------------- material
+// fn_evaluate_spring, newton_solver.cpp:1002-1048. Staging per spring (16 floats): the gradient of
+// its first vertex (the second's is the negation), then the row-major block He (the diagonal
+// blocks of both vertices are He, the off-diagonal ones -He). The length is clamped below at
+// FLT_EPSILON and the Hessian's lateral term at 0 (a compressed spring keeps its PSD part).
+constexpr float kSpringMinLength = 1.19209290e-7f;
+__global__ __launch_bounds__(kB) void k_spring_stage(uint32_t nsp, const uint32_t* __restrict__ sp,
+                                                      const float2* __restrict__ c, const float* __restrict__ x,
+                                                      float* __restrict__ stage)
+{
+    uint32_t e = blockIdx.x * kB + threadIdx.x;
+    if (e >= nsp) return;
+    const f3     diff = ld3(x, sp[2 * e]) - ld3(x, sp[2 * e + 1]);
+    const float2 rk   = c[e];
+    const float  l    = fmaxf(sqrtf(dot(diff, diff)), kSpringMinLength);
+    const float  C    = l - rk.x;
+    const f3     dir  = diff / l;
+    const float  xinv = 1.0f / l, xsi = xinv * xinv;
+    const f3     g    = (rk.y * dir) * C;
+    const float  s1 = rk.y * xsi, s2 = rk.y * fmaxf(1.0f - rk.x * xinv, 0.0f);
+    const float  d[3] = {diff.x, diff.y, diff.z};
+    float*       S    = stage + 16 * (size_t)e;
+    S[0] = g.x;
+    S[1] = g.y;
+    S[2] = g.z;
+    for (int r = 0; r < 3; r++)
+        for (int cc = 0; cc < 3; cc++)
+        {
+            const float o    = d[r] * d[cc];
+            S[3 + 3 * r + cc] = s1 * o + s2 * ((r == cc ? 1.0f : 0.0f) - xsi * o);
+        }
+}
+
 // fn_evaluate_stretch_face, newton_solver.cpp:1053-1100. Staging per face (64 floats):
 // g0 g1 g2 | H00 H11 H22 | H01 H02 H12 (row-major 3x3 blocks), 1 float pad.
 __global__ __launch_bounds__(kB) void k_face_stage(uint32_t nsf, const uint32_t* __restrict__ sf,
@@ -238,6 +270,14 @@ __global__ __launch_bounds__(kB) void k_vert_assemble(VertArgs a)
             H.m[4] = H.m[4] + k2;
         }
     }
+    for (uint32_t p = a.v_sp_ptr[i]; p < a.v_sp_ptr[i + 1]; p++)
+    {
+        uint32_t     e = a.v_sp[p] >> 2, l = a.v_sp[p] & 3u;
+        const float* S = a.spring_stage + 16 * (size_t)e;
+        const f3     g = mk(S[0], S[1], S[2]);
+        grad           = grad + (l == 0 ? -g : g);
+        for (int k = 0; k < 9; k++) H.m[k] = H.m[k] + S[3 + k];
+    }
     for (uint32_t p = a.v_sf_ptr[i]; p < a.v_sf_ptr[i + 1]; p++)
     {
         uint32_t     f = a.v_sf[p] >> 2, l = a.v_sf[p] & 3u;
@@ -264,7 +304,8 @@ __global__ __launch_bounds__(kB) void k_vert_assemble(VertArgs a)
 // Off-diagonal blocks: one thread per SELL block sums its contributors in order
 // (faces of the row vertex, then hinges) and writes the block once.
 __global__ __launch_bounds__(kB) void k_offdiag(uint32_t nblocks, const uint32_t* __restrict__ ptr,
-                                                 const uint32_t* __restrict__ con, const float* __restrict__ face_stage,
+                                                 const uint32_t* __restrict__ con, const float* __restrict__ spring_stage,
+                                                 const float* __restrict__ face_stage,
                                                  const float* __restrict__ hinge_stage, float* __restrict__ val)
 {
     uint32_t b = blockIdx.x * kB + threadIdx.x;
@@ -273,7 +314,13 @@ __global__ __launch_bounds__(kB) void k_offdiag(uint32_t nblocks, const uint32_t
     for (uint32_t p = ptr[b]; p < ptr[b + 1]; p++)
     {
         uint32_t c = con[p], el = c & 0x07ffffffu, la = (c >> 29) & 3u, lb = (c >> 27) & 3u;
-        if (c & 0x80000000u)
+        if ((c & 0x80000000u) && la == 3u && lb == 3u)
+        {
+            // a spring (no hinge block pairs a local index with itself)
+            const float* S = spring_stage + 16 * (size_t)el;
+            for (int k = 0; k < 9; k++) H.m[k] = H.m[k] + (-S[3 + k]);
+        }
+        else if (c & 0x80000000u)
         {
             const float* S = hinge_stage + 16 * (size_t)el;
             f3    ga = hinge_g(S, la), gb = hinge_g(S, lb);
@@ -1219,6 +1266,24 @@ __global__ __launch_bounds__(kB) void k_energy_faces(EnergyArgs a)
     scene_add(e, sc, a.out);
 }
 
+// fn_calc_energy_spring, solver_interface.cpp:603-630: 1/2 k (|x1 - x0| - L)^2
+__global__ __launch_bounds__(kB) void k_energy_springs(EnergyArgs a)
+{
+    const uint32_t e  = blockIdx.x * kB + threadIdx.x;
+    double         en = 0.0;
+    uint32_t       sc = kInvalid;
+    if (e < a.nsp)
+    {
+        const uint32_t v0 = a.sp[2 * e], v1 = a.sp[2 * e + 1];
+        sc                = a.vert_scene[v0];
+        const f3     diff = ld3(a.x, v1) - ld3(a.x, v0);
+        const float2 rk   = a.sp_c[e];
+        const float  C    = sqrtf(dot(diff, diff)) - rk.x;
+        en                = (double)(((0.5f * rk.y) * C) * C);
+    }
+    scene_add(en, sc, a.out);
+}
+
 __global__ __launch_bounds__(kB) void k_energy_hinges(EnergyArgs a)
 {
     const uint32_t h  = blockIdx.x * kB + threadIdx.x;
@@ -1699,10 +1764,14 @@ void launch_hinge_stage(uint32_t nbh, const uint32_t* bh, const float4* c, float
     if (nbh) k_hinge_stage<<<grid_for(nbh, kB), kB, 0, s>>>(nbh, bh, c, scaling, x, stage);
 }
 void launch_vert_assemble(const VertArgs& a, hipStream_t s) { k_vert_assemble<<<grid_for(a.nv, kB), kB, 0, s>>>(a); }
-void launch_offdiag(uint32_t nblocks, const uint32_t* ptr, const uint32_t* con, const float* fs, const float* hs, float* val,
-                    hipStream_t s)
+void launch_spring_stage(uint32_t nsp, const uint32_t* sp, const float2* c, const float* x, float* stage, hipStream_t s)
 {
-    if (nblocks) k_offdiag<<<grid_for(nblocks, kB), kB, 0, s>>>(nblocks, ptr, con, fs, hs, val);
+    if (nsp) k_spring_stage<<<grid_for(nsp, kB), kB, 0, s>>>(nsp, sp, c, x, stage);
+}
+void launch_offdiag(uint32_t nblocks, const uint32_t* ptr, const uint32_t* con, const float* ss, const float* fs, const float* hs,
+                    float* val, hipStream_t s)
+{
+    if (nblocks) k_offdiag<<<grid_for(nblocks, kB), kB, 0, s>>>(nblocks, ptr, con, ss, fs, hs, val);
 }
 void launch_abd_material(const AbdArgs& a, hipStream_t s)
 {
@@ -1776,6 +1845,7 @@ void launch_energy(const EnergyArgs& a, unsigned contact_grid, uint32_t nscenes,
 {
     k_fill_f64<<<grid_for(nscenes, kB), kB, 0, s>>>(a.out, nscenes);
     if (a.nv) k_energy_rows<<<grid_for(a.nv, kB), kB, 0, s>>>(a);
+    if (a.nsp) k_energy_springs<<<grid_for(a.nsp, kB), kB, 0, s>>>(a);
     if (a.nsf) k_energy_faces<<<grid_for(a.nsf, kB), kB, 0, s>>>(a);
     if (a.nbh) k_energy_hinges<<<grid_for(a.nbh, kB), kB, 0, s>>>(a);
     if (a.nb) k_energy_bodies<<<grid_for(a.nb, kB), kB, 0, s>>>(a);

@@ -329,6 +329,7 @@ void Setup::build(const std::vector<MeshDesc>& meshes)
     {
         const uint32_t* t = &faces[3 * f];
         if (vert_body[t[0]] != ~0u) continue;  // rigid meshes have no stretch terms
+        if (meshes[face_mesh[f]].mat.stretch_spring) continue;  // springs instead
         if (is_fixed[t[0]] && is_fixed[t[1]] && is_fixed[t[2]]) continue;
         sf.insert(sf.end(), t, t + 3);
         float d[4];
@@ -353,7 +354,22 @@ void Setup::build(const std::vector<MeshDesc>& meshes)
         bh_hbar.push_back((A1 + A2) / (3.0f * length(p0 - p1)));
         bh_k.push_back(meshes[vert_mesh[q[0]]].mat.area_bending_stiffness);
     }
-    const uint32_t nsf = (uint32_t)(sf.size() / 3), nbh = (uint32_t)(bh.size() / 4);
+    // stretch springs of the spring-model cloth meshes, one per edge with a free end
+    // (init_sim_data.cpp:205-235, 336-365): rest length |x0 - x1|, stiffness thickness * mu
+    for (uint32_t e = 0; e < ne; e++)
+    {
+        const uint32_t* q = &edges[2 * e];
+        const lcsx_material& mt = meshes[edge_mesh[e]].mat;
+        if (vert_body[q[0]] != ~0u || !mt.stretch_spring) continue;
+        if (is_fixed[q[0]] && is_fixed[q[1]]) continue;
+        sp.insert(sp.end(), q, q + 2);
+        sp_rest.push_back(length(at(rest_x, q[0]) - at(rest_x, q[1])));
+        double mu = mt.youngs_modulus / (2.0 * (1.0 + mt.poisson_ratio));
+        sp_k.push_back((float)(mu * mt.thickness));
+    }
+    const uint32_t nsf = (uint32_t)(sf.size() / 3), nbh = (uint32_t)(bh.size() / 4), nsp = (uint32_t)(sp.size() / 2);
+    if (std::max(std::max(nsf, nbh), nsp) >= (1u << 27))
+        throw std::runtime_error("more than 2^27 stretch or bending elements");
 
     // material neighbour lists (sorted), reference CSR layout
     std::vector<std::vector<uint32_t>> adj(nv);
@@ -361,6 +377,11 @@ void Setup::build(const std::vector<MeshDesc>& meshes)
         auto& l = adj[a];
         if (std::find(l.begin(), l.end(), b) == l.end()) l.push_back(b);
     };
+    for (uint32_t e = 0; e < nsp; e++)
+    {
+        add(sp[2 * e], sp[2 * e + 1]);
+        add(sp[2 * e + 1], sp[2 * e]);
+    }
     for (uint32_t f = 0; f < nsf; f++)
         for (int i = 0; i < 3; i++)
             for (int j = 0; j < 3; j++)
@@ -401,6 +422,7 @@ void Setup::build(const std::vector<MeshDesc>& meshes)
         for (uint32_t e = 0; e < ne_; e++)
             for (int k = 0; k < N; k++) lst[fill[el[N * e + k]]++] = (e << 2) | (uint32_t)k;
     };
+    build_lists(sp, 2, v_sp_ptr, v_sp);
     build_lists(sf, 3, v_sf_ptr, v_sf);
     build_lists(bh, 4, v_bh_ptr, v_bh);
 
@@ -435,8 +457,9 @@ void Setup::build(const std::vector<MeshDesc>& meshes)
             }
         }
     }
-    // contributors per SELL block: faces of the row vertex in face order, then hinges in order
-    // (counted, then filled in the same visiting order)
+    // contributors per SELL block: springs of the row vertex in spring order, then faces, then
+    // hinges (counted, then filled in the same visiting order). A spring's code is a hinge code
+    // with both local indices 3, which no hinge block has (its two local indices differ).
     auto slot_of = [&](uint32_t r, uint32_t c) {
         const auto& l = adj[r];
         uint32_t    k = (uint32_t)(std::lower_bound(l.begin(), l.end(), c) - l.begin());
@@ -445,6 +468,11 @@ void Setup::build(const std::vector<MeshDesc>& meshes)
     auto visit = [&](auto&& emit) {
         for (uint32_t r = 0; r < nv; r++)
         {
+            for (uint32_t p = v_sp_ptr[r]; p < v_sp_ptr[r + 1]; p++)
+            {
+                uint32_t e = v_sp[p] >> 2, a = v_sp[p] & 3;
+                emit(slot_of(r, sp[2 * e + (a ^ 1u)]), 0x80000000u | (3u << 29) | (3u << 27) | e);
+            }
             for (uint32_t p = v_sf_ptr[r]; p < v_sf_ptr[r + 1]; p++)
             {
                 uint32_t f = v_sf[p] >> 2, a = v_sf[p] & 3;

@@ -37,12 +37,14 @@ struct Setup
     // culled material elements
     std::vector<uint32_t> sf;                      // 3 per stretch face
     std::vector<float>    sf_Dm, sf_area, sf_mu, sf_lambda;
+    std::vector<uint32_t> sp;                      // stretch springs, 2 per spring (edge order)
+    std::vector<float>    sp_rest, sp_k;           // rest length, stiffness
     std::vector<uint32_t> bh;                      // 4 per bending hinge
     std::vector<float>    bh_angle, bh_hbar, bh_k;
     // reference CSR of material neighbours: nv+1 prefixes (offset by nv+1), then sorted columns
     std::vector<uint32_t> csr;
     // vertex -> (element << 2 | local) gather lists, in element order
-    std::vector<uint32_t> v_sf_ptr, v_sf, v_bh_ptr, v_bh;
+    std::vector<uint32_t> v_sf_ptr, v_sf, v_bh_ptr, v_bh, v_sp_ptr, v_sp;
     // SELL-64 layout of the off-diagonal blocks: slice s holds rows [64 s, 64 s + 64), width
     // sl_off[s+1] - sl_off[s]; block (row r, k-th neighbour) lives at (sl_off[s] + k) * 64 + (r % 64).
     uint32_t              nslices = 0, nblocks = 0;

@@ -262,6 +262,9 @@ void Solver::upload_state_constants()
     v_sf.upload(S.v_sf, s);
     v_bh_ptr.upload(S.v_bh_ptr, s);
     v_bh.upload(S.v_bh, s);
+    sp.upload(S.sp, s);
+    v_sp_ptr.upload(S.v_sp_ptr, s);
+    v_sp.upload(S.v_sp, s);
     sl_off.upload(S.sl_off, s);
     sell_col.upload(S.sell_col, s);
     blk_ptr.upload(S.blk_ptr, s);
@@ -288,6 +291,11 @@ void Solver::upload_state_constants()
     sf_Dm.upload(dm, s);
     sf_c.upload(cc, s);
     bh_c.upload(hc, s);
+    const size_t        nsp = S.sp.size() / 2;
+    std::vector<float2> spc(nsp);
+    for (size_t e = 0; e < nsp; e++) spc[e] = make_float2(S.sp_rest[e], S.sp_k[e]);
+    sp_c.upload(spc, s);
+    spring_stage.alloc(16 * std::max<size_t>(nsp, 1));
     face_stage.alloc(64 * std::max<size_t>(nsf, 1));
     hinge_stage.alloc(16 * std::max<size_t>(nbh, 1));
     LCSX_HIP(hipStreamSynchronize(s));
@@ -379,7 +387,8 @@ void Solver::iter_begin()
 void Solver::material()
 {
     tic();
-    const uint32_t nsf = (uint32_t)(S.sf.size() / 3), nbh = (uint32_t)(S.bh.size() / 4);
+    const uint32_t nsf = (uint32_t)(S.sf.size() / 3), nbh = (uint32_t)(S.bh.size() / 4), nsp = (uint32_t)(S.sp.size() / 2);
+    launch_spring_stage(nsp, sp.p, sp_c.p, x.p, spring_stage.p, stream);
     launch_face_stage(nsf, sf.p, sf_Dm.p, sf_c.p, x.p, face_stage.p, stream);
     launch_hinge_stage(nbh, bh.p, bh_c.p, prm.bending_scaling, x.p, hinge_stage.p, stream);
     VertArgs a{};
@@ -403,13 +412,16 @@ void Solver::material()
     a.v_sf     = v_sf.p;
     a.v_bh_ptr = v_bh_ptr.p;
     a.v_bh     = v_bh.p;
+    a.v_sp_ptr = v_sp_ptr.p;
+    a.v_sp     = v_sp.p;
+    a.spring_stage = spring_stage.p;
     a.face_stage  = face_stage.p;
     a.hinge_stage = hinge_stage.p;
     a.cgB      = cgB.p;
     a.diag     = diag.p;
     a.diag_mat = diag_mat.p;
     launch_vert_assemble(a, stream);
-    launch_offdiag(S.nblocks, blk_ptr.p, blk_contrib.p, face_stage.p, hinge_stage.p, sell_val.p, stream);
+    launch_offdiag(S.nblocks, blk_ptr.p, blk_contrib.p, spring_stage.p, face_stage.p, hinge_stage.p, sell_val.p, stream);
     if (!S.bodies.empty())
     {
         AbdArgs b{};
@@ -925,6 +937,9 @@ std::vector<double> Solver::energy()
     a.sf_Dm      = sf_Dm.p;
     a.sf_c       = sf_c.p;
     a.nbh        = (uint32_t)(S.bh.size() / 4);
+    a.nsp        = (uint32_t)(S.sp.size() / 2);
+    a.sp         = sp.p;
+    a.sp_c       = sp_c.p;
     a.bh         = bh.p;
     a.bh_c       = bh_c.p;
     a.nb         = (uint32_t)S.bodies.size();

@@ -41,8 +41,10 @@ struct Solver
     DBuf<float>    rest_x, offset, dhat, vert_mass, vert_area, face_area, edge_area, targets;
     DBuf<uint8_t>  is_fixed;
     DBuf<uint32_t> faces, edges, sf, bh, v_sf_ptr, v_sf, v_bh_ptr, v_bh, sl_off, sell_col, blk_ptr, blk_contrib, fixed_ids;
+    DBuf<uint32_t> sp, v_sp_ptr, v_sp;
     DBuf<float4>   sf_Dm, sf_c, bh_c;
-    DBuf<float>    face_stage, hinge_stage;
+    DBuf<float2>   sp_c;
+    DBuf<float>    face_stage, hinge_stage, spring_stage;
     Lbvh           tree_face, tree_edge;
     bool           sorted_ee = false;  // edge-edge candidates by the sorted self-query
     // collision buffers

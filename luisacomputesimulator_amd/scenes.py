@@ -49,6 +49,7 @@ class Anim:
 
 
 BODY_TYPES = {"cloth": 0, "rigid_shell": 1, "rigid_solid": 2}
+STRETCH_MODELS = {"FEM_BW98": 0, "Spring": 1}  # value: lcsx_material.stretch_spring
 
 
 @dataclasses.dataclass
@@ -67,10 +68,14 @@ class Material:
     d_hat: float = 2e-3
     body: str = "cloth"
     rigid_stiffness: float = 1e6
+    # cloth stretch model (ConstitutiveStretchModelCloth): "FEM_BW98" (Baraff-Witkin, the default)
+    # or "Spring" (a spring on every edge, stiffness thickness * E / (2 (1 + nu)))
+    stretch_model: str = "FEM_BW98"
 
     def pack(self) -> List[float]:
         return [self.thickness, self.youngs_modulus, self.poisson_ratio, self.area_bending_stiffness,
-                self.density, self.mass, self.d_hat, float(BODY_TYPES[self.body]), self.rigid_stiffness]
+                self.density, self.mass, self.d_hat, float(BODY_TYPES[self.body]), self.rigid_stiffness,
+                float(STRETCH_MODELS[self.stretch_model])]
 
 
 @dataclasses.dataclass
@@ -493,6 +498,12 @@ def from_json(path: str, mesh_dir: str) -> Scene:
                     mat.rigid_stiffness = float(m["stiffness"])
                 if m.get("is_shell") is False:
                     mat.body = "rigid_solid"
+            else:
+                sm = m.get("stretch_model")
+                if isinstance(sm, str):
+                    mat.stretch_model = "Spring" if sm == "Spring" else "FEM_BW98"
+                elif isinstance(sm, (int, float)) and not isinstance(sm, bool):
+                    mat.stretch_model = "Spring" if int(sm) == 0 else "FEM_BW98"
         fixed = []
         for fp in sh.get("fixed_points", []):
             fi = fp.get("fixed_info", {})

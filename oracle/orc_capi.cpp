@@ -72,6 +72,9 @@ bool lookup(Scene<T>& s, const std::string& name, View<T>& v, ContactCache<T>& c
     if (name == "csr") return u32(s.csr);
     if (name == "stretch_faces") return u32(s.sf);
     if (name == "bending_hinges") return u32(s.bh);
+    if (name == "stretch_springs") return u32(s.sp);
+    if (name == "sp_rest") return fl(s.sp_rest);
+    if (name == "sp_k") return fl(s.sp_k);
     if (name == "fixed_verts") return u32(s.fixed_verts);
     if (name == "face_area") return fl(s.face_area);
     if (name == "edge_area") return fl(s.edge_area);
@@ -206,6 +209,7 @@ int orc_add_mesh(void* p, const float* x, int nv, const int32_t* tris, int nt, c
         m.mat.body        = (int)mat[7];
         m.mat.rigid_kappa = mat[8];
     }
+    if (nm > 9) m.mat.spring = (int)mat[9];
     Handle* h = (Handle*)p;
     h->s32.meshes.push_back(m);
     h->s64.meshes.push_back(m);

@@ -32,6 +32,7 @@ struct Material
     double thickness = 2e-3, youngs = 1e5, poisson = 0.25, bending = 5e-3, density = 1e3, mass = 0.0, d_hat = 2e-3;
     int    body        = 0;    // 0 cloth, 1 rigid shell, 2 rigid solid (RigidMaterial, is_shell)
     double rigid_kappa = 1e6;  // RigidMaterial::stiffness (orthogonality)
+    int    spring      = 0;    // stretch model: 0 FEM_BW98, 1 Spring (a spring per edge)
 };
 struct Anim
 {
@@ -81,6 +82,9 @@ struct Scene
     std::vector<uint32_t> sf;  // stretch faces (culled)
     std::vector<T>        sf_Dm, sf_area, sf_mu, sf_lambda;
     std::vector<uint32_t> bh;  // bending hinges (culled)
+    std::vector<uint32_t> sp;  // stretch springs (edges of spring-model meshes with a free end)
+    std::vector<T>        sp_rest, sp_k;
+    std::vector<uint16_t> sp_off;  // offsets_in_adjlist (2 per spring)
     std::vector<T>        bh_angle, bh_hbar, bh_k;
     std::vector<uint32_t> csr;              // reference layout: n+1 prefixes (offset by n+1), then columns
     std::vector<uint16_t> sf_off, bh_off;   // offsets_in_adjlist (6 per face, 12 per hinge)
@@ -484,6 +488,7 @@ struct Scene
         {
             const uint32_t* t = &faces[3 * f];
             if (vert_body[t[0]] != ~0u) continue;
+            if (meshes[face_mesh[f]].mat.spring) continue;  // init_sim_data.cpp:240-255: FEM_BW98 only
             if (is_fixed[t[0]] && is_fixed[t[1]] && is_fixed[t[2]]) continue;
             sf.push_back(t[0]); sf.push_back(t[1]); sf.push_back(t[2]);
             float d[4];
@@ -514,12 +519,31 @@ struct Scene
             bh_hbar.push_back(T((A1 + A2) / (3.0f * L0)));
             bh_k.push_back(T((float)meshes[hinge_mesh[h]].mat.bending));
         }
+        // stretch springs (init_sim_data.cpp:205-235, 336-365): every edge of a Spring-model cloth
+        // mesh with a free end; rest length |x0 - x1| of the rest positions, stiffness thickness * mu
+        sp.clear(); sp_rest.clear(); sp_k.clear();
+        for (uint32_t e = 0; e < ne; e++)
+        {
+            const uint32_t* q = &edges[2 * e];
+            const Material& mt = meshes[edge_mesh[e]].mat;
+            if (vert_body[q[0]] != ~0u || !mt.spring) continue;
+            if (is_fixed[q[0]] && is_fixed[q[1]]) continue;
+            sp.push_back(q[0]); sp.push_back(q[1]);
+            sp_rest.push_back(T(length(rxf[q[0]] - rxf[q[1]])));
+            double mu = (float)mt.youngs / (2.0 * (1.0 + (float)mt.poisson));
+            sp_k.push_back(T((float)(mu * (float)mt.thickness)));
+        }
         // ---- CSR adjacency of material force vertices (init_sim_data.cpp:760-896)
         std::vector<std::vector<uint32_t>> adj(nr);
         auto ins = [&](uint32_t a, uint32_t b) {
             auto& l = adj[a];
             if (std::find(l.begin(), l.end(), b) == l.end()) l.push_back(b);
         };
+        for (size_t e = 0; e < sp.size() / 2; e++)
+        {
+            ins(sp[2 * e], sp[2 * e + 1]);
+            ins(sp[2 * e + 1], sp[2 * e]);
+        }
         for (size_t f = 0; f < sf.size() / 3; f++)
             for (int i = 0; i < 3; i++)
                 for (int j = 0; j < 3; j++)
@@ -555,6 +579,8 @@ struct Scene
         sf_off.clear(); bh_off.clear();
         for (size_t f = 0; f < sf.size() / 3; f++) offs(&sf[3 * f], 3, sf_off);
         for (size_t h = 0; h < bh.size() / 4; h++) offs(&bh[4 * h], 4, bh_off);
+        sp_off.clear();
+        for (size_t e = 0; e < sp.size() / 2; e++) offs(&sp[2 * e], 2, sp_off);
         // ---- state (restart_system, solver_interface.cpp:184-205); a body starts at q = (0, I)
         x.assign(rest_x.begin(), rest_x.end());
         for (uint32_t b = 0; b < nb; b++)
@@ -680,6 +706,29 @@ struct Scene
             hgh[e].stiff = stiff;
             hgh[e].coef  = stiff * d;
         }
+        // springs (fn_evaluate_spring, newton_solver.cpp:1002-1048): gradient of the first vertex
+        // (the second's is its negation) and He (diagonal blocks He, off-diagonal -He)
+        const size_t     nsp = sp.size() / 2;
+        std::vector<V>     sp_g(nsp);
+        std::vector<M3<T>> sp_H(nsp);
+        for (size_t e = 0; e < nsp; e++)
+        {
+            V     diff = x[sp[2 * e]] - x[sp[2 * e + 1]];
+            T     L = sp_rest[e], k = sp_k[e];
+            T     l = std::max(length(diff), T(kEpsilon));
+            T     C = l - L;
+            V     dir = diff / l;
+            T     xinv = T(1) / l, xsi = xinv * xinv;
+            M3<T> xxT = outer(diff, diff), Ic = M3<T>::diag(T(1));
+            M3<T> lat;
+            for (int r = 0; r < 3; r++)
+                for (int c = 0; c < 3; c++) lat.m[r][c] = Ic.m[r][c] - xsi * xxT.m[r][c];
+            sp_g[e] = (k * dir) * C;
+            sp_H[e] = (k * xsi) * xxT + (k * std::max(T(1) - L * xinv, T(0))) * lat;
+        }
+        std::vector<std::vector<uint32_t>> v_sp(nv);
+        for (size_t e = 0; e < nsp; e++)
+            for (int j = 0; j < 2; j++) v_sp[sp[2 * e + j]].push_back((uint32_t)e);
         std::vector<std::vector<uint32_t>> v_sf(nv), v_bh(nv);
         for (size_t f = 0; f < nsf; f++)
             for (int j = 0; j < 3; j++) v_sf[sf[3 * f + j]].push_back((uint32_t)f);
@@ -688,6 +737,14 @@ struct Scene
         for (uint32_t i = 0; i < nv; i++)
         {
             const uint32_t base = csr[i];
+            for (uint32_t e : v_sp[i])
+            {
+                const int a = sp[2 * e] == i ? 0 : 1;
+                cgB[i]      = cgB[i] + (a == 0 ? -sp_g[e] : sp_g[e]);
+                diag[i]     = diag[i] + sp_H[e];
+                uint32_t slot = base + sp_off[2 * e + a];
+                offdiag[slot] = offdiag[slot] + T(-1) * sp_H[e];
+            }
             for (uint32_t f : v_sf[i])
             {
                 int a = 0;
@@ -1106,6 +1163,12 @@ struct Scene
                 T dh = dhat[i], th = offset[i], dist = x[i].y - fy;
                 if (dist - th < dh) e_gr += (double)ground_e(prm.contact_quadratic, dist, th, dh, kap * vert_area[i]);
             }
+        }
+        for (size_t e = 0; e < sp.size() / 2; e++)  // fn_calc_energy_spring, solver_interface.cpp:603-630
+        {
+            V diff = x[sp[2 * e + 1]] - x[sp[2 * e]];
+            T C    = std::sqrt(dot(diff, diff)) - sp_rest[e];
+            e_st += (double)(((T(0.5f) * sp_k[e]) * C) * C);
         }
         for (size_t f = 0; f < sf.size() / 3; f++)
         {
